@@ -1,0 +1,994 @@
+// api.hip — the C ABI of libmpmx_hip.so (include/mpmx.h): context lifecycle,
+// substep stages, field access, statistics and kernel timing.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "mpmx_internal.h"
+#include "precision.h"
+
+using namespace mpmx;
+
+namespace mpmx {
+
+// ---- device kernels owned by the API --------------------------------------
+__global__ void k_upload_fp32(PState s, const double* __restrict__ x, const double* __restrict__ v,
+                              const double* __restrict__ m, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        s.x[3 * i + a] = (float)x[3 * i + a];
+        s.v[3 * i + a] = (float)v[3 * i + a];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        s.B[(int64_t)k * n + i] = 0.f;
+        s.F[(int64_t)k * n + i] = (k % 3 == k / 3) ? 1.f : 0.f;
+    }
+    s.c[i] = 1.f;      // Particle_kernal.cu:451-454
+    s.g[i] = 1.f;
+    s.alpha[i] = 0.f;
+    s.psi[i] = 0.f;
+    s.mass[i] = (float)m[i];
+    s.id[i] = i;
+}
+__global__ void k_upload_mixed(MState s, const double* __restrict__ x, const double* __restrict__ v,
+                               const double* __restrict__ m, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        s.x[3 * i + a] = (float)x[3 * i + a];
+        s.v[3 * i + a] = f2bf((float)v[3 * i + a]);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        s.B[(int64_t)k * n + i] = f2bf(0.f);
+        s.F[(int64_t)k * n + i] = f2h(0.f);
+    }
+    s.c[i] = f2bf(1.f);
+    s.g[i] = f2bf(1.f);
+    s.alpha[i] = f2bf(0.f);
+    s.psi[i] = f2bf(0.f);
+    s.mass[i] = (float)m[i];
+    s.id[i] = i;
+}
+
+// FP64 sums of mass, momentum and kinetic energy
+__global__ void k_conservation(const float* __restrict__ mass, const float* __restrict__ v,
+                               const uint16_t* __restrict__ vb, int n, double* out) {
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double m = mass[i];
+        double vv[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) vv[a] = vb ? (double)bf2f(vb[3 * i + a]) : (double)v[3 * i + a];
+        acc[0] += m;
+        acc[1] += m * vv[0];
+        acc[2] += m * vv[1];
+        acc[3] += m * vv[2];
+        acc[4] += 0.5 * m * (vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        double a = acc[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&out[q], a);
+    }
+}
+
+__global__ void k_max_speed2(const float* __restrict__ v, const uint16_t* __restrict__ vb, int n,
+                             unsigned* out) {
+    float best = 0.f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float s2 = 0.f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float va = vb ? bf2f(vb[3 * i + a]) : v[3 * i + a];
+            s2 += va * va;
+        }
+        best = fmaxf(best, s2);
+    }
+    best = wave_max(best);
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(best));
+}
+
+// cell tables (Transformer_kernal.cu:15-36): head flags, then ids
+__global__ void k_cell_flags(const uint32_t* __restrict__ keys, int n, int* flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+__global__ void k_cell_ids(const uint32_t* __restrict__ keys, int n, const int* __restrict__ excl,
+                           int* p2c, int* c2p, const int* num_cell) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool head = (i == 0 || keys[i] != keys[i - 1]);
+    p2c[i] = excl[i] + (head ? 1 : 0);
+    if (head) c2p[excl[i]] = i;
+    if (i == n - 1) c2p[*num_cell] = n;
+}
+
+void timer_push(mpmx_ctx* c, const char* name, bool begin) {
+    if (!c->timing) return;
+    TimerEntry* t = nullptr;
+    for (auto& e : c->timers)
+        if (e.name == name) t = &e;
+    if (!t) {
+        c->timers.push_back(TimerEntry{name, {}});
+        t = &c->timers.back();
+    }
+    if (begin) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+        (void)hipEventRecord(a, c->stream);
+        t->ev.push_back({a, b});
+    } else if (!t->ev.empty()) {
+        (void)hipEventRecord(t->ev.back().second, c->stream);
+    }
+}
+
+}  // namespace mpmx
+
+// ---- helpers ----------------------------------------------------------------
+static mpmx_status fail(mpmx_ctx* c, mpmx_status s, const std::string& msg) {
+    if (c) c->last_error = msg;
+    return s;
+}
+static mpmx_status hipfail(mpmx_ctx* c, hipError_t e, const char* where) {
+    return fail(c, MPMX_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(c, expr)                                     \
+    do {                                                    \
+        hipError_t _e = (expr);                             \
+        if (_e != hipSuccess) return hipfail(c, _e, #expr); \
+    } while (0)
+
+template <typename T>
+static hipError_t dalloc(T** p, int64_t count) {
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    return hipMalloc((void**)p, sizeof(T) * (size_t)count);
+}
+template <typename T>
+static void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+static int ceil_log2(int64_t v) {
+    int b = 0;
+    while ((1ll << b) < v) ++b;
+    return b;
+}
+
+// C^-1 by Gauss-Jordan with full pivoting, as Inverse (ExplicitTimeIntegrator.cu:71-129)
+static void gauss_jordan3(const double in[9], double mat[9]) {
+    for (int i = 0; i < 9; ++i) mat[i] = in[i];
+    int swapR[3], swapC[3], pivot[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i) {
+        int pr = 0, pc = 0;
+        double maxValue = 0;
+        for (int j = 0; j < 3; ++j)
+            if (pivot[j] != 1)
+                for (int k = 0; k < 3; ++k)
+                    if (pivot[k] == 0 && mat[j * 3 + k] > maxValue) { maxValue = mat[j * 3 + k]; pr = j; pc = k; }
+        if (pr != pc)
+            for (int j = 0; j < 3; ++j) std::swap(mat[pr * 3 + j], mat[pc * 3 + j]);
+        swapC[i] = pc;
+        swapR[i] = pr;
+        ++pivot[i];
+        const double inv = 1.0 / mat[pc * 3 + pc];
+        mat[pc * 3 + pc] = 1;
+        for (int j = 0; j < 3; ++j) mat[pc * 3 + j] *= inv;
+        for (int j = 0; j < 3; ++j)
+            if (j != pc) {
+                const double ratio = mat[j * 3 + pc];
+                mat[j * 3 + pc] = 0;
+                for (int k = 0; k < 3; ++k) mat[j * 3 + k] -= mat[pc * 3 + k] * ratio;
+            }
+    }
+    for (int i = 0; i < 3; ++i)
+        if (swapR[i] != swapC[i])
+            for (int j = 0; j < 3; ++j) std::swap(mat[j * 3 + swapC[i]], mat[j * 3 + swapR[i]]);
+}
+
+static void make_dev_params(mpmx_ctx* c) {
+    const mpmx_params& P = c->P;
+    DevParams& d = c->dp;
+    d.N = P.N;
+    d.transfer = P.transfer;
+    d.phase_field = P.phase_field;
+    d.dot_mode = P.dot_mode;
+    d.wall = P.wall_cells;
+    d.grip_lo = P.grip_lo;
+    d.grip_hi = P.grip_hi;
+    d.vpage = P.vpage;
+    d.dx = (float)(1.0 / P.N);
+    d.inv_dx = (float)P.N;
+    d.Dinv = (float)(4.0 * P.N * P.N);
+    d.grip_v = (float)P.grip_v;
+    // Materials.cpp:4-14
+    const double lambda = P.E * P.nu / ((1 + P.nu) * (1 - 2 * P.nu));
+    const double mu = P.E / (2 * (1 + P.nu));
+    const double kappa = P.E / (3 * (1 - 2 * P.nu));
+    d.lambda = (float)lambda;
+    d.mu = (float)mu;
+    d.kappa = (float)kappa;
+    d.V0 = (float)P.V0;
+    const double C[9] = {2 * mu + lambda, lambda, lambda, lambda, 2 * mu + lambda, lambda,
+                         lambda, lambda, 2 * mu + lambda};
+    double Ci[9];
+    gauss_jordan3(C, Ci);
+    for (int k = 0; k < 9; ++k) d.Cinv[k] = (float)Ci[k];
+    d.yield = (float)P.yield_stress;
+    d.hard = (float)P.hardening;
+    d.alpha_c = (float)P.alpha_c;
+    d.dlambda_cap = (float)P.dlambda_cap;
+    d.eps_cap = (float)P.eps_cap;
+    d.Gc0 = (float)P.Gc0;
+    d.residual_c = (float)P.residual_c;
+    d.ps = (float)P.ps;
+    d.theta = (float)P.theta;
+    d.M = (float)P.M;
+    for (int a = 0; a < 3; ++a) d.gravity[a] = (float)P.gravity[a];
+    d.vmax = (float)P.vmax;
+    d.cg_tol = (float)P.cg_tol;
+    d.crit_stretch = (float)P.crit_stretch;
+    d.L0 = (float)(0.5 / P.N);
+    d.sq23f = sqrtf(2.0f / 3.0f);
+    d.svd_gap = (float)c->cfg.svd_gap;
+}
+
+static mpmx_status read_counts(mpmx_ctx* c) {
+    HIPCHK(c, hipMemcpyAsync(c->hc, c->dc, sizeof(DevCounts), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MPMX_OK;
+}
+static mpmx_status check_err(mpmx_ctx* c) {
+    mpmx_status s = read_counts(c);
+    if (s != MPMX_OK) return s;
+    const unsigned e = c->hc->err;
+    if (e & kErrCapacity) return fail(c, MPMX_ERR_CAPACITY, "page table capacity exceeded");
+    if (e & kErrNonFinite) return fail(c, MPMX_ERR_NONFINITE, "non-finite particle state");
+    if (e & kErrDomain) return fail(c, MPMX_ERR_OUT_OF_DOMAIN, "a particle left [2dx, 1-2dx]");
+    if (e & kErrBreakdown) return fail(c, MPMX_ERR_CG_BREAKDOWN, "phase-field PCG breakdown");
+    return MPMX_OK;
+}
+
+static void free_particles(mpmx_ctx* c) {
+    for (int b = 0; b < 2; ++b) {
+        PState& s = c->ps[b];
+        dfree(s.x); dfree(s.v); dfree(s.B); dfree(s.F); dfree(s.c); dfree(s.alpha); dfree(s.psi); dfree(s.g);
+        dfree(s.mass); dfree(s.id);
+        MState& m = c->ms[b];
+        dfree(m.x); dfree(m.v); dfree(m.B); dfree(m.F); dfree(m.c); dfree(m.alpha); dfree(m.psi); dfree(m.g);
+        dfree(m.mass); dfree(m.id);
+    }
+    dfree(c->sigma); dfree(c->FP); dfree(c->vol);
+    dfree(c->keys); dfree(c->keys_alt); dfree(c->vals); dfree(c->vals_alt); dfree(c->hist);
+    dfree(c->pmask); dfree(c->nmask); dfree(c->wcount); dfree(c->scount);
+    dfree(c->page_key); dfree(c->page2particle); dfree(c->adj); dfree(c->nvp); dfree(c->vpo); dfree(c->target);
+    dfree(c->grid); dfree(c->grid_p2g);
+    dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q);
+    dfree(c->scan_tmp);
+}
+
+static mpmx_status alloc_grid(mpmx_ctx* c, int64_t cap_tp) {
+    dfree(c->grid); dfree(c->grid_p2g);
+    dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q);
+    c->cap_tp = cap_tp;
+    HIPCHK(c, dalloc(&c->grid, cap_tp * kPageFloats));
+    if (c->keep_p2g) HIPCHK(c, dalloc(&c->grid_p2g, cap_tp * kPageFloats));
+    HIPCHK(c, dalloc(&c->cg_x, cap_tp * 64));
+    HIPCHK(c, dalloc(&c->cg_r, cap_tp * 64));
+    HIPCHK(c, dalloc(&c->cg_z, cap_tp * 64));
+    HIPCHK(c, dalloc(&c->cg_s, cap_tp * 64));
+    HIPCHK(c, dalloc(&c->cg_q, cap_tp * 64));
+    HIPCHK(c, fill_zero(c, c->grid, cap_tp * kPageFloats * 4));
+    HIPCHK(c, fill_zero(c, c->cg_x, cap_tp * 64 * 4));
+    return MPMX_OK;
+}
+
+// ---- public API --------------------------------------------------------------
+extern "C" {
+
+void mpmx_default_params(mpmx_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->N = 128;
+    p->transfer = 0;
+    p->phase_field = 1;
+    p->dot_mode = 0;
+    p->cg_max_iter = 100;
+    p->vpage = 256;
+    p->wall_cells = 4;
+    p->grip_lo = -1;
+    p->grip_hi = 1 << 30;
+    p->grip_v = 0;
+    p->E = 1e5;
+    p->nu = 0.3;
+    p->V0 = 1.0 / (128.0 * 128.0 * 128.0 * 8.0);
+    p->yield_stress = 800;
+    p->hardening = 3;
+    p->alpha_c = 1;
+    p->dlambda_cap = 100;
+    p->eps_cap = 3;
+    p->Gc0 = 3;
+    p->residual_c = 0.1;
+    p->ps = 0.01;
+    p->theta = (double)0.0001f;
+    p->M = 50;
+    p->gravity[0] = 0;
+    p->gravity[1] = -10;
+    p->gravity[2] = 0;
+    p->vmax = (double)11.137f;
+    p->cg_tol = (double)1e-6f;
+    p->crit_stretch = 0;
+}
+
+void mpmx_default_config(mpmx_config* c) {
+    std::memset(c, 0, sizeof(*c));
+    c->precision = MPMX_FP32;
+    c->svd_gap = 1e-4;
+    c->cg_check_every = 1;
+}
+
+const char* mpmx_status_string(mpmx_status s) {
+    switch (s) {
+    case MPMX_OK: return "ok";
+    case MPMX_ERR_HIP: return "HIP runtime error";
+    case MPMX_ERR_ARG: return "invalid argument";
+    case MPMX_ERR_CAPACITY: return "capacity exceeded";
+    case MPMX_ERR_OUT_OF_DOMAIN: return "particle out of domain";
+    case MPMX_ERR_NONFINITE: return "non-finite state";
+    case MPMX_ERR_CG_BREAKDOWN: return "PCG breakdown";
+    case MPMX_ERR_NO_DEVICE: return "no HIP device";
+    }
+    return "unknown";
+}
+
+const char* mpmx_last_error(const mpmx_ctx* c) { return c ? c->last_error.c_str() : "null context"; }
+
+int mpmx_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+mpmx_status mpmx_create(const mpmx_params* p, const mpmx_config* cfg, mpmx_ctx** out) {
+    if (!p || !out) return MPMX_ERR_ARG;
+    *out = nullptr;
+    if (p->N < 16 || p->N > 1024 || (p->N & 3)) return MPMX_ERR_ARG;
+    if (p->vpage != 64 && p->vpage != 128 && p->vpage != 256) return MPMX_ERR_ARG;
+    if (p->transfer < 0 || p->transfer > 1) return MPMX_ERR_ARG;
+    if (mpmx_device_count() <= 0) return MPMX_ERR_NO_DEVICE;
+    mpmx_ctx* c = new mpmx_ctx();
+    c->P = *p;
+    if (cfg) c->cfg = *cfg;
+    else mpmx_default_config(&c->cfg);
+    if (c->cfg.svd_gap <= 0) c->cfg.svd_gap = 1e-4;
+    c->device = c->cfg.device;
+    if (hipSetDevice(c->device) != hipSuccess) {
+        delete c;
+        return MPMX_ERR_NO_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return MPMX_ERR_HIP;
+    }
+    if (hipMalloc((void**)&c->dc, sizeof(DevCounts)) != hipSuccess ||
+        hipHostMalloc((void**)&c->hc, sizeof(DevCounts), hipHostMallocDefault) != hipSuccess) {
+        delete c;
+        return MPMX_ERR_HIP;
+    }
+    std::memset(c->hc, 0, sizeof(DevCounts));
+    (void)hipMemcpy(c->dc, c->hc, sizeof(DevCounts), hipMemcpyHostToDevice);
+    make_dev_params(c);
+    *out = c;
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_destroy(mpmx_ctx* c) {
+    if (!c) return MPMX_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& t : c->timers)
+        for (auto& e : t.ev) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    free_particles(c);
+    if (c->dc) (void)hipFree(c->dc);
+    if (c->hc) (void)hipHostFree(c->hc);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_set_particles(mpmx_ctx* c, int64_t n, const double* x, const double* v, const double* mass) {
+    if (!c || !x || !v || !mass || n <= 0 || n > (1ll << 30)) return fail(c, MPMX_ERR_ARG, "bad particle arrays");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int N = c->P.N;
+    const double lo = 2.0 / N, hi = 1.0 - 2.0 / N;
+    for (int64_t i = 0; i < 3 * n; ++i)
+        if (!(x[i] >= lo && x[i] <= hi)) return fail(c, MPMX_ERR_OUT_OF_DOMAIN, "particle outside [2dx, 1-2dx]");
+    free_particles(c);
+    c->n = n;
+    c->cap_n = std::max<int64_t>(n, c->cfg.max_particles);
+    const bool mixed = c->cfg.precision == MPMX_MIXED;
+    for (int b = 0; b < 2; ++b) {
+        if (mixed) {
+            MState& s = c->ms[b];
+            HIPCHK(c, dalloc(&s.x, 3 * n));
+            HIPCHK(c, dalloc(&s.v, 3 * n));
+            HIPCHK(c, dalloc(&s.B, 9 * n));
+            HIPCHK(c, dalloc(&s.F, 9 * n));
+            HIPCHK(c, dalloc(&s.c, n));
+            HIPCHK(c, dalloc(&s.alpha, n));
+            HIPCHK(c, dalloc(&s.psi, n));
+            HIPCHK(c, dalloc(&s.g, n));
+            HIPCHK(c, dalloc(&s.mass, n));
+            HIPCHK(c, dalloc(&s.id, n));
+        } else {
+            PState& s = c->ps[b];
+            HIPCHK(c, dalloc(&s.x, 3 * n));
+            HIPCHK(c, dalloc(&s.v, 3 * n));
+            HIPCHK(c, dalloc(&s.B, 9 * n));
+            HIPCHK(c, dalloc(&s.F, 9 * n));
+            HIPCHK(c, dalloc(&s.c, n));
+            HIPCHK(c, dalloc(&s.alpha, n));
+            HIPCHK(c, dalloc(&s.psi, n));
+            HIPCHK(c, dalloc(&s.g, n));
+            HIPCHK(c, dalloc(&s.mass, n));
+            HIPCHK(c, dalloc(&s.id, n));
+        }
+    }
+    HIPCHK(c, dalloc(&c->sigma, 9 * n));
+    HIPCHK(c, dalloc(&c->FP, n));
+    HIPCHK(c, dalloc(&c->vol, n));
+    HIPCHK(c, dalloc(&c->keys, n));
+    HIPCHK(c, dalloc(&c->keys_alt, n));
+    HIPCHK(c, dalloc(&c->vals, n));
+    HIPCHK(c, dalloc(&c->vals_alt, n));
+    c->sort_tiles = (int)((n + 2047) / 2048);
+    HIPCHK(c, dalloc(&c->hist, (int64_t)256 * c->sort_tiles));
+    c->key_bits = 6 + 3 * std::max(1, ceil_log2(N / 4));
+    // pages
+    c->axis_bits = ceil_log2(N / 4 + 1);
+    const int64_t space = 1ll << (3 * c->axis_bits);
+    c->mask_words = std::max<int64_t>(1, space / 64);
+    HIPCHK(c, dalloc(&c->pmask, c->mask_words));
+    HIPCHK(c, dalloc(&c->nmask, c->mask_words));
+    HIPCHK(c, dalloc(&c->wcount, c->mask_words));
+    HIPCHK(c, dalloc(&c->scount, c->mask_words));
+    const int64_t pages_domain = (int64_t)(N / 4) * (N / 4) * (N / 4);
+    const int64_t pages_nb = (int64_t)(N / 4 + 1) * (N / 4 + 1) * (N / 4 + 1);
+    c->cap_p = std::min<int64_t>(n, pages_domain);
+    const int64_t tp_bound = std::min<int64_t>(8 * c->cap_p, pages_nb);
+    c->cap_vp = n / c->P.vpage + c->cap_p + 1;
+    HIPCHK(c, dalloc(&c->page_key, tp_bound));
+    HIPCHK(c, dalloc(&c->page2particle, c->cap_p + 1));
+    HIPCHK(c, dalloc(&c->adj, 7 * c->cap_p));
+    HIPCHK(c, dalloc(&c->nvp, c->cap_p + 1));
+    HIPCHK(c, dalloc(&c->vpo, c->cap_p + 1));
+    HIPCHK(c, dalloc(&c->target, c->cap_vp));
+    int64_t tiles = (256ll * c->sort_tiles + 1023) / 1024;
+    tiles = std::max(tiles, (c->mask_words + 1023) / 1024);
+    tiles = std::max(tiles, (c->cap_p + 1023) / 1024);
+    tiles = std::max(tiles, (n + 1023) / 1024);
+    c->scan_tmp_cap = tiles + 1;
+    HIPCHK(c, dalloc(&c->scan_tmp, c->scan_tmp_cap));
+    // upload
+    double *dx_ = nullptr, *dv_ = nullptr, *dm_ = nullptr;
+    HIPCHK(c, dalloc(&dx_, 3 * n));
+    HIPCHK(c, dalloc(&dv_, 3 * n));
+    HIPCHK(c, dalloc(&dm_, n));
+    HIPCHK(c, hipMemcpyAsync(dx_, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dv_, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dm_, mass, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    c->cur = 0;
+    if (mixed) k_upload_mixed<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(c->ms[0], dx_, dv_, dm_, (int)n);
+    else k_upload_fp32<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(c->ps[0], dx_, dv_, dm_, (int)n);
+    HIPCHK(c, hipGetLastError());
+    std::memset(c->hc, 0, sizeof(DevCounts));
+    HIPCHK(c, hipMemcpyAsync(c->dc, c->hc, sizeof(DevCounts), hipMemcpyHostToDevice, c->stream));
+    // keys, first canonical sort, first tables (with the bound as capacity)
+    HIPCHK(c, compute_keys(c));
+    HIPCHK(c, sort_and_gather(c));
+    c->cap_tp = tp_bound;
+    HIPCHK(c, build_tables(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(dx_); dfree(dv_); dfree(dm_);
+    mpmx_status s = check_err(c);
+    if (s != MPMX_OK) return s;
+    int64_t cap = c->cfg.max_pages;
+    if (cap <= 0) cap = std::min<int64_t>(tp_bound, std::max<int64_t>(4 * (int64_t)c->hc->num_total, 4096));
+    cap = std::min(cap, tp_bound);
+    s = alloc_grid(c, cap);
+    if (s != MPMX_OK) return s;
+    c->sorted = true;
+    c->substeps = 0;
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_bin_sort(mpmx_ctx* c) {
+    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->sorted) {
+        MPMX_TIMED(c, "sort", HIPCHK(c, sort_and_gather(c)));
+        c->sorted = true;
+    }
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_rebuild(mpmx_ctx* c) {
+    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
+    MPMX_TIMED(c, "tables", HIPCHK(c, build_tables(c)));
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_stress(mpmx_ctx* c) {
+    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
+    MPMX_TIMED(c, "stress", HIPCHK(c, launch_stress(c)));
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_p2g(mpmx_ctx* c, double dt) {
+    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
+    MPMX_TIMED(c, "p2g", HIPCHK(c, launch_p2g(c, (float)dt)));
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_grid_update(mpmx_ctx* c, double dt) {
+    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
+    MPMX_TIMED(c, "grid_update", HIPCHK(c, launch_grid_update(c, (float)dt)));
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_phase_field(mpmx_ctx* c, double dt) {
+    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
+    HIPCHK(c, launch_pcg(c, (float)dt));
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_g2p(mpmx_ctx* c, double dt) {
+    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
+    MPMX_TIMED(c, "g2p", HIPCHK(c, launch_g2p(c, (float)dt)));
+    c->sorted = false;
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_substep(mpmx_ctx* c, double dt) {
+    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
+    HIPCHK(c, hipSetDevice(c->device));
+    mpmx_status s;
+    if ((s = mpmx_bin_sort(c)) != MPMX_OK) return s;
+    if ((s = mpmx_rebuild(c)) != MPMX_OK) return s;
+    if ((s = mpmx_stress(c)) != MPMX_OK) return s;
+    if ((s = mpmx_p2g(c, dt)) != MPMX_OK) return s;
+    if ((s = mpmx_grid_update(c, dt)) != MPMX_OK) return s;
+    if ((s = mpmx_phase_field(c, dt)) != MPMX_OK) return s;
+    if ((s = mpmx_g2p(c, dt)) != MPMX_OK) return s;
+    ++c->substeps;
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_run(mpmx_ctx* c, int nsteps, double dt) {
+    for (int k = 0; k < nsteps; ++k) {
+        mpmx_status s = mpmx_substep(c, dt);
+        if (s != MPMX_OK) return s;
+    }
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_sync(mpmx_ctx* c) {
+    if (!c) return MPMX_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return check_err(c);
+}
+
+mpmx_status mpmx_compute_dt(mpmx_ctx* c, double dt_default, double cfl, double* dt_out) {
+    if (!c || !dt_out || c->n == 0) return MPMX_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    unsigned* d = nullptr;
+    HIPCHK(c, dalloc(&d, 1));
+    HIPCHK(c, hipMemsetAsync(d, 0, sizeof(unsigned), c->stream));
+    const bool mixed = c->cfg.precision == MPMX_MIXED;
+    k_max_speed2<<<256, 256, 0, c->stream>>>(mixed ? nullptr : c->ps[c->cur].v, mixed ? c->ms[c->cur].v : nullptr,
+                                             (int)c->n, d);
+    unsigned h = 0;
+    HIPCHK(c, hipMemcpyAsync(&h, d, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(d);
+    float s2;
+    std::memcpy(&s2, &h, 4);
+    double dt = dt_default;
+    const double vmax = std::sqrt((double)s2);
+    // Simulator.cpp:282
+    if (vmax > 0 && cfl / c->P.N / vmax < dt_default) dt = cfl / c->P.N / vmax;
+    *dt_out = dt;
+    return MPMX_OK;
+}
+
+static int field_width(int f) {
+    switch (f) {
+    case MPMX_FIELD_X: case MPMX_FIELD_V: return 3;
+    case MPMX_FIELD_F: case MPMX_FIELD_B: case MPMX_FIELD_SIGMA: return 9;
+    default: return 1;
+    }
+}
+
+// copy a device field of the current (sorted) order to doubles in sorted order
+static mpmx_status download_sorted(mpmx_ctx* c, int f, std::vector<double>& out) {
+    const int64_t n = c->n;
+    const int w = field_width(f);
+    out.assign((size_t)(n * w), 0.0);
+    const bool mixed = c->cfg.precision == MPMX_MIXED;
+    const PState& s = c->ps[c->cur];
+    const MState& m = c->ms[c->cur];
+    auto get_f32 = [&](const float* p, int64_t count, std::vector<float>& h) -> hipError_t {
+        h.resize((size_t)count);
+        return hipMemcpy(h.data(), p, sizeof(float) * count, hipMemcpyDeviceToHost);
+    };
+    auto get_u16 = [&](const uint16_t* p, int64_t count, std::vector<uint16_t>& h) -> hipError_t {
+        h.resize((size_t)count);
+        return hipMemcpy(h.data(), p, sizeof(uint16_t) * count, hipMemcpyDeviceToHost);
+    };
+    std::vector<float> hf;
+    std::vector<uint16_t> hu;
+    // AoS (3 per particle) fields
+    if (f == MPMX_FIELD_X) {
+        HIPCHK(c, get_f32(mixed ? m.x : s.x, 3 * n, hf));
+        for (int64_t i = 0; i < 3 * n; ++i) out[i] = hf[i];
+        return MPMX_OK;
+    }
+    if (f == MPMX_FIELD_V) {
+        if (mixed) {
+            HIPCHK(c, get_u16(m.v, 3 * n, hu));
+            for (int64_t i = 0; i < 3 * n; ++i) out[i] = bf2f(hu[i]);
+        } else {
+            HIPCHK(c, get_f32(s.v, 3 * n, hf));
+            for (int64_t i = 0; i < 3 * n; ++i) out[i] = hf[i];
+        }
+        return MPMX_OK;
+    }
+    // component-major 9-wide fields -> per particle column-major
+    if (f == MPMX_FIELD_F || f == MPMX_FIELD_B || f == MPMX_FIELD_SIGMA) {
+        if (f == MPMX_FIELD_SIGMA || !mixed) {
+            const float* p = f == MPMX_FIELD_SIGMA ? c->sigma : (f == MPMX_FIELD_F ? s.F : s.B);
+            HIPCHK(c, get_f32(p, 9 * n, hf));
+            for (int64_t i = 0; i < n; ++i)
+                for (int k = 0; k < 9; ++k) out[9 * i + k] = hf[k * n + i];
+        } else {
+            HIPCHK(c, get_u16(f == MPMX_FIELD_F ? m.F : m.B, 9 * n, hu));
+            for (int64_t i = 0; i < n; ++i)
+                for (int k = 0; k < 9; ++k)
+                    out[9 * i + k] = f == MPMX_FIELD_F ? (double)h2f(hu[k * n + i]) + ((k % 4) == 0 ? 1.0 : 0.0)
+                                                       : (double)bf2f(hu[k * n + i]);
+        }
+        return MPMX_OK;
+    }
+    const float* pf = nullptr;
+    const uint16_t* pu = nullptr;
+    switch (f) {
+    case MPMX_FIELD_C: pf = s.c; pu = m.c; break;
+    case MPMX_FIELD_ALPHA: pf = s.alpha; pu = m.alpha; break;
+    case MPMX_FIELD_PSI: pf = s.psi; pu = m.psi; break;
+    case MPMX_FIELD_G: pf = s.g; pu = m.g; break;
+    case MPMX_FIELD_MASS: pf = mixed ? m.mass : s.mass; pu = nullptr; break;
+    case MPMX_FIELD_FP: pf = c->FP; pu = nullptr; break;
+    case MPMX_FIELD_VOL: pf = c->vol; pu = nullptr; break;
+    default: return fail(c, MPMX_ERR_ARG, "unknown field");
+    }
+    if (mixed && pu) {
+        HIPCHK(c, get_u16(pu, n, hu));
+        for (int64_t i = 0; i < n; ++i) out[i] = bf2f(hu[i]);
+    } else {
+        HIPCHK(c, get_f32(pf, n, hf));
+        for (int64_t i = 0; i < n; ++i) out[i] = hf[i];
+    }
+    return MPMX_OK;
+}
+
+static mpmx_status download_ids(mpmx_ctx* c, std::vector<int>& ids) {
+    ids.resize((size_t)c->n);
+    const int* p = c->cfg.precision == MPMX_MIXED ? c->ms[c->cur].id : c->ps[c->cur].id;
+    HIPCHK(c, hipMemcpy(ids.data(), p, sizeof(int) * c->n, hipMemcpyDeviceToHost));
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_get_field(mpmx_ctx* c, int f, double* host, size_t count) {
+    if (!c || !host || c->n == 0) return MPMX_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int w = field_width(f);
+    if (count < (size_t)(c->n * w)) return fail(c, MPMX_ERR_ARG, "output buffer too small");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> sorted;
+    std::vector<int> ids;
+    mpmx_status s = download_sorted(c, f, sorted);
+    if (s != MPMX_OK) return s;
+    if ((s = download_ids(c, ids)) != MPMX_OK) return s;
+    for (int64_t i = 0; i < c->n; ++i) {
+        const int o = ids[i];
+        if (o < 0 || o >= c->n) return fail(c, MPMX_ERR_HIP, "corrupt particle ids");
+        for (int k = 0; k < w; ++k) host[(size_t)o * w + k] = sorted[(size_t)i * w + k];
+    }
+    return check_err(c);
+}
+
+mpmx_status mpmx_set_field(mpmx_ctx* c, int f, const double* host, size_t count) {
+    if (!c || !host || c->n == 0) return MPMX_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int w = field_width(f);
+    if (count < (size_t)(c->n * w)) return fail(c, MPMX_ERR_ARG, "input buffer too small");
+    if (f == MPMX_FIELD_X || f == MPMX_FIELD_SIGMA || f == MPMX_FIELD_FP || f == MPMX_FIELD_VOL)
+        return fail(c, MPMX_ERR_ARG, "field is derived or fixes the order; not settable");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<int> ids;
+    mpmx_status s = download_ids(c, ids);
+    if (s != MPMX_OK) return s;
+    const int64_t n = c->n;
+    const bool mixed = c->cfg.precision == MPMX_MIXED;
+    const PState& ps = c->ps[c->cur];
+    const MState& ms = c->ms[c->cur];
+    std::vector<float> hf;
+    std::vector<uint16_t> hu;
+    auto put = [&](void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); };
+    if (f == MPMX_FIELD_V) {
+        hf.resize(3 * n);
+        for (int64_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) hf[3 * i + a] = (float)host[3 * ids[i] + a];
+        if (mixed) {
+            hu.resize(3 * n);
+            for (int64_t i = 0; i < 3 * n; ++i) hu[i] = f2bf(hf[i]);
+            HIPCHK(c, put(ms.v, hu.data(), 2 * hu.size()));
+        } else {
+            HIPCHK(c, put(ps.v, hf.data(), 4 * hf.size()));
+        }
+        return MPMX_OK;
+    }
+    if (f == MPMX_FIELD_F || f == MPMX_FIELD_B) {
+        hf.resize(9 * n);
+        for (int64_t i = 0; i < n; ++i)
+            for (int k = 0; k < 9; ++k) hf[k * n + i] = (float)host[9 * ids[i] + k];
+        if (mixed) {
+            hu.resize(9 * n);
+            for (int64_t i = 0; i < n; ++i)
+                for (int k = 0; k < 9; ++k)
+                    hu[k * n + i] = f == MPMX_FIELD_F ? f2h(hf[k * n + i] - ((k % 4) == 0 ? 1.f : 0.f))
+                                                      : f2bf(hf[k * n + i]);
+            HIPCHK(c, put(f == MPMX_FIELD_F ? ms.F : ms.B, hu.data(), 2 * hu.size()));
+        } else {
+            HIPCHK(c, put(f == MPMX_FIELD_F ? ps.F : ps.B, hf.data(), 4 * hf.size()));
+        }
+        return MPMX_OK;
+    }
+    hf.resize(n);
+    for (int64_t i = 0; i < n; ++i) hf[i] = (float)host[ids[i]];
+    float* pf = nullptr;
+    uint16_t* pu = nullptr;
+    switch (f) {
+    case MPMX_FIELD_C: pf = ps.c; pu = ms.c; break;
+    case MPMX_FIELD_ALPHA: pf = ps.alpha; pu = ms.alpha; break;
+    case MPMX_FIELD_PSI: pf = ps.psi; pu = ms.psi; break;
+    case MPMX_FIELD_G: pf = ps.g; pu = ms.g; break;
+    case MPMX_FIELD_MASS: pf = mixed ? ms.mass : ps.mass; break;
+    default: return fail(c, MPMX_ERR_ARG, "unknown field");
+    }
+    if (mixed && pu) {
+        hu.resize(n);
+        for (int64_t i = 0; i < n; ++i) hu[i] = f2bf(hf[i]);
+        HIPCHK(c, put(pu, hu.data(), 2 * hu.size()));
+    } else {
+        HIPCHK(c, put(pf, hf.data(), 4 * hf.size()));
+    }
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_get_order(mpmx_ctx* c, int32_t* order, uint64_t* keys) {
+    if (!c || c->n == 0) return MPMX_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (order) {
+        std::vector<int> ids;
+        mpmx_status s = download_ids(c, ids);
+        if (s != MPMX_OK) return s;
+        std::memcpy(order, ids.data(), sizeof(int) * c->n);
+    }
+    if (keys) {
+        std::vector<uint32_t> k((size_t)c->n);
+        HIPCHK(c, hipMemcpy(k.data(), c->keys, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost));
+        // (Morton page key << 6 | cell) -> 64-bit SPGrid offset (page << 12 | cell << 2)
+        for (int64_t i = 0; i < c->n; ++i)
+            keys[i] = ((uint64_t)(k[i] >> 6) << 12) | ((uint64_t)(k[i] & 63u) << 2);
+    }
+    return check_err(c);
+}
+
+mpmx_status mpmx_get_tables(mpmx_ctx* c, uint64_t* page_key, int32_t* page2particle, int32_t* adj, int32_t* vpo,
+                            int32_t* target, int32_t* particle2cell, int32_t* cell2particle, int32_t counts[4]) {
+    if (!c || c->n == 0) return MPMX_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    mpmx_status s = read_counts(c);
+    if (s != MPMX_OK) return s;
+    const int np = std::min<int64_t>(c->hc->num_page, c->cap_p);
+    const int nt = std::min<int64_t>(c->hc->num_total, c->cap_tp);
+    const int nv = std::min<int64_t>(c->hc->num_vp, c->cap_vp);
+    const int n = (int)c->n;
+    int num_cell = 0;
+    if (particle2cell || cell2particle || counts) {
+        int *flags = nullptr, *excl = nullptr, *p2c = nullptr, *c2p = nullptr, *nc = nullptr;
+        HIPCHK(c, dalloc(&flags, n));
+        HIPCHK(c, dalloc(&excl, n));
+        HIPCHK(c, dalloc(&p2c, n));
+        HIPCHK(c, dalloc(&c2p, n + 1));
+        HIPCHK(c, dalloc(&nc, 1));
+        k_cell_flags<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, flags);
+        HIPCHK(c, scan_exclusive(c, flags, excl, n, nullptr, n, nc));
+        k_cell_ids<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, excl, p2c, c2p, nc);
+        HIPCHK(c, hipMemcpyAsync(&num_cell, nc, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (particle2cell) HIPCHK(c, hipMemcpy(particle2cell, p2c, sizeof(int) * n, hipMemcpyDeviceToHost));
+        if (cell2particle) HIPCHK(c, hipMemcpy(cell2particle, c2p, sizeof(int) * (num_cell + 1), hipMemcpyDeviceToHost));
+        dfree(flags); dfree(excl); dfree(p2c); dfree(c2p); dfree(nc);
+    }
+    if (page_key) {
+        std::vector<uint32_t> pk((size_t)std::max(nt, 1));
+        HIPCHK(c, hipMemcpy(pk.data(), c->page_key, sizeof(uint32_t) * nt, hipMemcpyDeviceToHost));
+        for (int i = 0; i < nt; ++i) page_key[i] = pk[i];
+    }
+    if (page2particle) HIPCHK(c, hipMemcpy(page2particle, c->page2particle, sizeof(int) * (np + 1), hipMemcpyDeviceToHost));
+    if (adj)
+        for (int q = 0; q < 7; ++q)
+            HIPCHK(c, hipMemcpy(adj + (size_t)q * np, c->adj + (size_t)q * c->cap_p, sizeof(int) * np,
+                                hipMemcpyDeviceToHost));
+    if (vpo) HIPCHK(c, hipMemcpy(vpo, c->vpo, sizeof(int) * (np + 1), hipMemcpyDeviceToHost));
+    if (target) {
+        HIPCHK(c, hipMemcpy(target, c->target, sizeof(int) * nv, hipMemcpyDeviceToHost));
+        for (int v = 0; v < nv; ++v) target[v] += 1;   // the reference's d_targetPage is 1-based
+    }
+    if (counts) {
+        counts[0] = c->hc->num_page;
+        counts[1] = c->hc->num_total;
+        counts[2] = c->hc->num_vp;
+        counts[3] = num_cell;
+    }
+    return check_err(c);
+}
+
+static mpmx_status get_grid_from(mpmx_ctx* c, const float* src, int ch, float* host, size_t count) {
+    if (!c || !host || ch < 0 || ch > 8) return MPMX_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    mpmx_status s = read_counts(c);
+    if (s != MPMX_OK) return s;
+    const int64_t nt = std::min<int64_t>(c->hc->num_total, c->cap_tp);
+    if (count < (size_t)(nt * 64)) return fail(c, MPMX_ERR_ARG, "output buffer too small");
+    if (ch == 8) {
+        HIPCHK(c, hipMemcpy(host, c->cg_x, sizeof(float) * nt * 64, hipMemcpyDeviceToHost));
+        return MPMX_OK;
+    }
+    std::vector<float> g((size_t)(nt * kPageFloats));
+    HIPCHK(c, hipMemcpy(g.data(), src, sizeof(float) * nt * kPageFloats, hipMemcpyDeviceToHost));
+    for (int64_t p = 0; p < nt; ++p)
+        for (int k = 0; k < 64; ++k) host[p * 64 + k] = g[p * kPageFloats + ch * 64 + k];
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_get_grid(mpmx_ctx* c, int ch, float* host, size_t count) {
+    return get_grid_from(c, c ? c->grid : nullptr, ch, host, count);
+}
+
+mpmx_status mpmx_get_grid_p2g(mpmx_ctx* c, int ch, float* host, size_t count) {
+    if (!c) return MPMX_ERR_ARG;
+    if (!c->keep_p2g) {
+        // enable the copy from the next grid update on
+        c->keep_p2g = true;
+        HIPCHK(c, dalloc(&c->grid_p2g, c->cap_tp * kPageFloats));
+        return fail(c, MPMX_ERR_ARG, "P2G copy enabled; call again after the next grid update");
+    }
+    return get_grid_from(c, c->grid_p2g, ch, host, count);
+}
+
+mpmx_status mpmx_get_stats(mpmx_ctx* c, mpmx_stats* out) {
+    if (!c || !out) return MPMX_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::memset(out, 0, sizeof(*out));
+    out->num_particles = c->n;
+    out->substeps = c->substeps;
+    if (c->n > 0) {
+        double* d = nullptr;
+        HIPCHK(c, dalloc(&d, 5));
+        HIPCHK(c, hipMemsetAsync(d, 0, sizeof(double) * 5, c->stream));
+        const bool mixed = c->cfg.precision == MPMX_MIXED;
+        k_conservation<<<256, 256, 0, c->stream>>>(mixed ? c->ms[c->cur].mass : c->ps[c->cur].mass,
+                                                   mixed ? nullptr : c->ps[c->cur].v,
+                                                   mixed ? c->ms[c->cur].v : nullptr, (int)c->n, d);
+        double h[5];
+        HIPCHK(c, hipMemcpyAsync(h, d, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        dfree(d);
+        out->mass = h[0];
+        out->momentum[0] = h[1];
+        out->momentum[1] = h[2];
+        out->momentum[2] = h[3];
+        out->kinetic_energy = h[4];
+    }
+    mpmx_status s = read_counts(c);
+    if (s != MPMX_OK) return s;
+    out->num_page = c->hc->num_page;
+    out->num_total_page = c->hc->num_total;
+    out->num_virtual_page = c->hc->num_vp;
+    out->cg_iters = c->hc->cg_iters;
+    out->cg_breakdown = c->hc->cg_breakdown;
+    out->cg_norm = __builtin_bit_cast(float, c->hc->norm_key);
+    return check_err(c);
+}
+
+mpmx_status mpmx_timing_begin(mpmx_ctx* c) {
+    if (!c) return MPMX_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto& t : c->timers)
+        for (auto& e : t.ev) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    c->timers.clear();
+    c->timing = true;
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_timing_end(mpmx_ctx* c, mpmx_kernel_time* out, int max_entries, int* n_entries) {
+    if (!c) return MPMX_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->timing = false;
+    int k = 0;
+    for (auto& t : c->timers) {
+        double total = 0;
+        for (auto& e : t.ev) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) total += ms;
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        if (out && k < max_entries) {
+            std::memset(out[k].name, 0, sizeof(out[k].name));
+            std::strncpy(out[k].name, t.name.c_str(), sizeof(out[k].name) - 1);
+            out[k].launches = (int)t.ev.size();
+            out[k].total_ms = total;
+        }
+        ++k;
+    }
+    c->timers.clear();
+    if (n_entries) *n_entries = k;
+    return check_err(c);
+}
+
+mpmx_status mpmx_time_run(mpmx_ctx* c, int nsteps, double dt, double* ms) {
+    if (!c || !ms) return MPMX_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipEventRecord(a, c->stream));
+    mpmx_status s = mpmx_run(c, nsteps, dt);
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    HIPCHK(c, hipEventSynchronize(b));
+    float t = 0;
+    HIPCHK(c, hipEventElapsedTime(&t, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = t;
+    if (s != MPMX_OK) return s;
+    return check_err(c);
+}
+
+}  // extern "C"

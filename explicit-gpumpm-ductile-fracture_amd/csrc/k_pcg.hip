@@ -1,0 +1,287 @@
+// k_pcg.hip — the phase-field solve: Jacobi-preconditioned CG on the grid,
+// (D + M K) x = b, with D_i = sum_p V w (FP M + 1/dt) and
+// (K s)_i = sum_p grad w_ip . (4 L0^2 V_p sum_j grad w_jp s_j), L0 = dx/2.
+//
+// Same iteration as the reference driver loop (ExplicitTimeIntegrator.cu:914-968)
+// with its reductions: the "dot product" is the maximum over 128-entry blocks
+// of the block sums and the norm is max|r| (dot_mode 1 gives true inner
+// products).  Differences of form, not of maths:
+//  - AxG2P_APIC and AxP2G_APIC (G2PKernal.cu:1006-1123, P2GKernal.cu:1162-1326)
+//    are one kernel: a workgroup gathers grad w . s for its particles from the
+//    staged s tile and scatters M grad w . col back, with no round trip of the
+//    per-particle vector col through HBM;
+//  - the diagonal term s_i V w (FP M + 1/dt) summed over particles is s_i D_i,
+//    applied per node instead of per particle;
+//  - D is computed once per substep (the reference rebuilds it every
+//    iteration, preConditionP2G_APIC);
+//  - the convergence test, the breakdown test and the scalars live on the
+//    device; every kernel returns at once after convergence, so a fixed number
+//    of launches can be queued (or captured) without host round trips.
+#include "access.h"
+
+namespace mpmx {
+
+__device__ __forceinline__ float dot_value(const DevParams& dp, unsigned key, float sum) {
+    return dp.dot_mode == 0 ? fkey_dec(key) : sum;
+}
+
+// q += M grad w . col for every particle, col = V dx^2 sum_j grad w_j s_j
+template <int BLOCK, typename ST>
+__global__ __launch_bounds__(BLOCK) void k_cg_apply(DevParams dp, ST s, const float* __restrict__ vol, Tables t,
+                                                    const float* __restrict__ cgs, float* __restrict__ cgq,
+                                                    int n, DevCounts* dc) {
+    if (dc->cg_done) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        dc->rho_old = dot_value(dp, dc->rho_key, dc->rho_sum);
+        dc->sq_key = 0u;
+        dc->sq_sum = 0.f;
+        dc->cg_iters += 1;
+    }
+    __shared__ float ts[kTileN];
+    __shared__ float tq[kTileN];
+    int nvp = dc->num_vp;
+    nvp = nvp < t.cap_vp ? nvp : (int)t.cap_vp;
+    int vb, ve;
+    block_range(nvp, vb, ve);
+    const float para_scale = 4.0f * dp.L0 * dp.L0 * dp.inv_dx * dp.inv_dx;  // dx^2 / dx^2 with grads in 1/dx
+    for (int vp = vb; vp < ve; ++vp) {
+        int p, s0, e;
+        vpage_range(t, vp, BLOCK, n, p, s0, e);
+        for (int q = threadIdx.x; q < kTileN; q += BLOCK) {
+            const int a = q / 36, b = (q / 6) % 6, cc = q % 6;
+            const int64_t gn = tile_node(t, p, a, b, cc);
+            ts[q] = gn >= 0 ? cgs[gn] : 0.f;
+            tq[q] = 0.f;
+        }
+        __syncthreads();
+        const int i = s0 + threadIdx.x;
+        if (i < e) {
+            int pc[3];
+            unmorton(t.page_key[p], pc[0], pc[1], pc[2]);
+            float d0[3], w[3][3], gw[3][3];
+            int l[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float xa = ld_x(s, i, a);
+                const int b = base_node(xa, dp.N);
+                d0[a] = xa * dp.inv_dx - (float)b;
+                int la = b - 4 * pc[a];
+                l[a] = la < 0 ? 0 : (la > 3 ? 3 : la);
+                bspline_grad(d0[a], w[a], gw[a]);
+            }
+            float gs[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                    for (int kk = 0; kk < 3; ++kk) {
+                        const float sv = ts[(l[0] + ii) * 36 + (l[1] + jj) * 6 + (l[2] + kk)];
+                        gs[0] += gw[0][ii] * w[1][jj] * w[2][kk] * sv;
+                        gs[1] += w[0][ii] * gw[1][jj] * w[2][kk] * sv;
+                        gs[2] += w[0][ii] * w[1][jj] * gw[2][kk] * sv;
+                    }
+            // col in units where grad w is per cell: col = V * 4 L0^2 * grad(s) ; scatter M grad w . col
+            const float sc = vol[i] * para_scale * dp.M;
+            const float c0 = gs[0] * sc, c1 = gs[1] * sc, c2 = gs[2] * sc;
+#pragma unroll
+            for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                    for (int kk = 0; kk < 3; ++kk) {
+                        const float hw = gw[0][ii] * w[1][jj] * w[2][kk] * c0 + w[0][ii] * gw[1][jj] * w[2][kk] * c1 +
+                                         w[0][ii] * w[1][jj] * gw[2][kk] * c2;
+                        atomicAdd(&tq[(l[0] + ii) * 36 + (l[1] + jj) * 6 + (l[2] + kk)], hw);
+                    }
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < kTileN; q += BLOCK) {
+            const float val = tq[q];
+            if (val != 0.f) {
+                const int a = q / 36, b = (q / 6) % 6, cc = q % 6;
+                const int64_t gn = tile_node(t, p, a, b, cc);
+                if (gn >= 0) atomicAdd(&cgq[gn], val);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// helper: grid-stride over 128-node blocks, one wave each, two nodes per lane
+#define FOR_128_BLOCKS(nt)                                                   \
+    const int lane = threadIdx.x & 63;                                       \
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;           \
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;                        \
+    const int nblk = ((nt) + 1) / 2;                                         \
+    for (int blk = wave; blk < nblk; blk += nwaves)
+
+// q = q + s D; sq = "dot"(s, q)
+__global__ __launch_bounds__(256) void k_cg_dot(DevParams dp, const float* __restrict__ grid,
+                                                const float* __restrict__ cgs, float* __restrict__ cgq,
+                                                int64_t cap_tp, DevCounts* dc) {
+    if (dc->cg_done) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        dc->norm_key = 0u;
+        dc->rho_key = 0u;
+        dc->rho_sum = 0.f;
+    }
+    int nt = dc->num_total;
+    nt = nt < cap_tp ? nt : (int)cap_tp;
+    FOR_128_BLOCKS(nt) {
+        const int n0 = blk * 128 + lane * 2;
+        float part = 0.f;
+        if ((n0 >> 6) < nt) {
+            const float* D = grid + (int64_t)(n0 >> 6) * kPageFloats + 7 * 64 + (n0 & 63);
+            const float2 sv = *(const float2*)(cgs + n0);
+            float2 qv = *(const float2*)(cgq + n0);
+            qv.x += sv.x * D[0];
+            qv.y += sv.y * D[1];
+            *(float2*)(cgq + n0) = qv;
+            part = sv.x * qv.x + sv.y * qv.y;
+        }
+        const float bs = wave_sum(part);
+        if (lane == 0) {
+            if (dp.dot_mode == 0) atomicMax(&dc->sq_key, fkey(bs));
+            else atomicAdd(&dc->sq_sum, bs);
+        }
+    }
+}
+
+// alpha = rho_old / sq; x += alpha s; r -= alpha q; q = 0; norm = max|r|;
+// z = r / D where |D| >= 1e-30 (else z keeps its value, as precondition_subd);
+// rho = "dot"(r, z).  Only nodes touched by particles (sum w > 0) change.
+__global__ __launch_bounds__(256) void k_cg_update(DevParams dp, const float* __restrict__ grid,
+                                                   float* __restrict__ cgx, float* __restrict__ cgr,
+                                                   float* __restrict__ cgz, const float* __restrict__ cgs,
+                                                   float* __restrict__ cgq, int64_t cap_tp, DevCounts* dc) {
+    if (dc->cg_done) return;
+    const float alpha = dc->rho_old / dot_value(dp, dc->sq_key, dc->sq_sum);
+    int nt = dc->num_total;
+    nt = nt < cap_tp ? nt : (int)cap_tp;
+    FOR_128_BLOCKS(nt) {
+        const int n0 = blk * 128 + lane * 2;
+        float part = 0.f, rmax = 0.f;
+        if ((n0 >> 6) < nt) {
+            const float* gp = grid + (int64_t)(n0 >> 6) * kPageFloats + (n0 & 63);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int nn = n0 + h;
+                if (gp[5 * 64 + h] > 0.f) {
+                    const float D = gp[7 * 64 + h];
+                    cgx[nn] += cgs[nn] * alpha;
+                    const float r = cgr[nn] - cgq[nn] * alpha;
+                    cgr[nn] = r;
+                    float z = cgz[nn];
+                    if (!(fabsf(D) < 1e-30f)) {
+                        z = r / D;
+                        cgz[nn] = z;
+                    }
+                    rmax = fmaxf(rmax, fabsf(r));
+                    part += r * z;
+                }
+                cgq[nn] = 0.f;
+            }
+        }
+        const float bs = wave_sum(part);
+        const float bm = wave_max(rmax);
+        if (lane == 0) {
+            atomicMax(&dc->norm_key, __float_as_uint(bm));
+            if (dp.dot_mode == 0) atomicMax(&dc->rho_key, fkey(bs));
+            else atomicAdd(&dc->rho_sum, bs);
+        }
+    }
+}
+
+// convergence / breakdown test, then s = s beta + z
+__global__ __launch_bounds__(256) void k_cg_direction(DevParams dp, const float* __restrict__ grid,
+                                                      const float* __restrict__ cgz, float* __restrict__ cgs,
+                                                      int64_t cap_tp, DevCounts* dc) {
+    if (dc->cg_done) return;
+    const float norm = __uint_as_float(dc->norm_key);
+    const bool converged = norm < dp.cg_tol;
+    const float rho_old = dc->rho_old;
+    if (converged || rho_old == 0.f) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            dc->cg_done = 1;
+            if (!converged) {
+                dc->cg_breakdown = 1;
+                atomicOr(&dc->err, kErrBreakdown);
+            }
+        }
+        return;
+    }
+    const float beta = dot_value(dp, dc->rho_key, dc->rho_sum) / rho_old;
+    int nt = dc->num_total;
+    nt = nt < cap_tp ? nt : (int)cap_tp;
+    FOR_128_BLOCKS(nt) {
+        const int n0 = blk * 128 + lane * 2;
+        if ((n0 >> 6) < nt) {
+            const float* gp = grid + (int64_t)(n0 >> 6) * kPageFloats + (n0 & 63);
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (gp[5 * 64 + h] > 0.f) cgs[n0 + h] = cgs[n0 + h] * beta + cgz[n0 + h];
+        }
+    }
+}
+
+template <int BLOCK, typename ST>
+static void apply_dispatch(mpmx_ctx* c, ST st) {
+    const int G = transfer_grid_blocks(c);
+    const Tables t{c->page_key, c->page2particle, c->adj, c->vpo, c->target, c->cap_p, c->cap_tp, c->cap_vp};
+    k_cg_apply<BLOCK, ST><<<G, BLOCK, 0, c->stream>>>(c->dp, st, c->vol, t, c->cg_s, c->cg_q, (int)c->n, c->dc);
+}
+template <typename ST>
+static void apply_block(mpmx_ctx* c, ST st) {
+    switch (c->P.vpage) {
+    case 64: apply_dispatch<64>(c, st); break;
+    case 128: apply_dispatch<128>(c, st); break;
+    default: apply_dispatch<256>(c, st); break;
+    }
+}
+
+static int node_blocks(mpmx_ctx* c) {
+    int64_t blocks = (c->cap_tp * 64 / 128 + 3) / 4;
+    blocks = blocks < (int64_t)c->num_cu * 16 ? blocks : (int64_t)c->num_cu * 16;
+    return blocks < 1 ? 1 : (int)blocks;
+}
+
+// one PCG iteration: four launches, all no-ops once the device flag says done
+static hipError_t pcg_iteration(mpmx_ctx* c) {
+    MPMX_TIMED(c, "cg_apply", (c->cfg.precision == MPMX_MIXED) ? apply_block(c, c->ms[c->cur])
+                                                               : apply_block(c, c->ps[c->cur]));
+    const int nb = node_blocks(c);
+    MPMX_TIMED(c, "cg_vector", ({
+                   k_cg_dot<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_s, c->cg_q, c->cap_tp, c->dc);
+                   k_cg_update<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_x, c->cg_r, c->cg_z, c->cg_s,
+                                                          c->cg_q, c->cap_tp, c->dc);
+                   k_cg_direction<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_z, c->cg_s, c->cap_tp, c->dc);
+               }));
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg(mpmx_ctx* c, float dt) {
+    (void)dt;
+    if (!c->P.phase_field) return hipSuccess;
+    const int every = c->cfg.cg_check_every > 0 ? c->cfg.cg_check_every : 1;
+    int it = 0;
+    while (it < c->P.cg_max_iter) {
+        const int chunk = (c->P.cg_max_iter - it) < every ? (c->P.cg_max_iter - it) : every;
+        for (int k = 0; k < chunk; ++k) {
+            hipError_t e = pcg_iteration(c);
+            if (e != hipSuccess) return e;
+        }
+        it += chunk;
+        if (it >= c->P.cg_max_iter) break;
+        // read the convergence flag
+        hipError_t e = hipMemcpyAsync(&c->hc->cg_done, &c->dc->cg_done, sizeof(int), hipMemcpyDeviceToHost,
+                                      c->stream);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return e;
+        if (c->hc->cg_done) break;
+    }
+    return hipSuccess;
+}
+
+}  // namespace mpmx

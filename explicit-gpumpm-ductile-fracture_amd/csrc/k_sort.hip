@@ -1,0 +1,328 @@
+// k_sort.hip — cell keys, device-wide exclusive scan, stable LSD radix sort of
+// the particle keys, and the gather of every particle field into sorted order.
+//
+// Replaces Particles::sort_by_offsets / reorder (Particle_kernal.cu:167-249,
+// 484-573) and the recursive block scan (cuda_kernal_tools.cu:15-203).  The
+// reference buckets particles by (hash-numbered page, cell) with atomics, so
+// its page order and within-cell order change from run to run; here the sort
+// is a stable radix sort of the Morton key, giving one canonical order: key
+// ascending, ties in previous sorted order.
+#include "mpmx_internal.h"
+
+namespace mpmx {
+
+// ---------------------------------------------------------------- fills
+__global__ void k_fill_zero_u32(uint32_t* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+// zero count_dev[0] * per_unit floats (count clamped to cap_units)
+__global__ void k_fill_zero_count(float* p, const int* count_dev, int64_t per_unit, int64_t cap_units) {
+    int64_t units = *count_dev;
+    units = units < cap_units ? units : cap_units;
+    const int64_t n = units * per_unit;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.f;
+}
+
+hipError_t fill_zero(mpmx_ctx* c, void* p, int64_t bytes) {
+    const int64_t words = (bytes + 3) / 4;
+    int64_t blocks = (words + 255) / 256;
+    blocks = blocks < 8192 ? blocks : 8192;
+    if (blocks < 1) blocks = 1;
+    k_fill_zero_u32<<<(int)blocks, 256, 0, c->stream>>>((uint32_t*)p, words);
+    return hipGetLastError();
+}
+hipError_t fill_zero_dev_count(mpmx_ctx* c, float* p, const int* count_dev, int64_t per_unit,
+                               int64_t cap_units) {
+    int64_t blocks = (cap_units * per_unit + 255) / 256;
+    blocks = blocks < 8192 ? blocks : 8192;
+    if (blocks < 1) blocks = 1;
+    k_fill_zero_count<<<(int)blocks, 256, 0, c->stream>>>(p, count_dev, per_unit, cap_units);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- scan
+// Exclusive scan of int32 in three phases: 1024-item tile scans, a single-block
+// scan of the tile sums, and the tile offsets added back.  The item count comes
+// either from the host (n_dev == nullptr) or from device memory, so the same
+// launches serve tables whose sizes are only known on the device.
+constexpr int kScanTile = 1024;
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// block of 256 threads: exclusive scan of one value per thread, returns total
+__device__ __forceinline__ int block_excl_scan256(int v, int& total, int* lds4) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int inc = wave_incl_scan(v);
+    if (lane == 63) lds4[w] = inc;
+    __syncthreads();
+    int off = 0;
+    total = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int s = lds4[q];
+        if (q < w) off += s;
+        total += s;
+    }
+    __syncthreads();
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tiles(const int* in, int* out, int* sums, const int* n_dev,
+                                                    int64_t n_host) {
+    __shared__ int lds4[4];
+    int64_t n = n_dev ? (int64_t)*n_dev : n_host;
+    n = n < n_host ? n : n_host;   // n_host is the capacity when n_dev is given
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    int v[4];
+    int local = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t i = base + threadIdx.x * 4 + q;
+        v[q] = (i < n) ? in[i] : 0;
+        local += v[q];
+    }
+    int total;
+    int off = block_excl_scan256(local, total, lds4);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t i = base + threadIdx.x * 4 + q;
+        if (i < n) out[i] = off;
+        off += v[q];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// single block: exclusive scan of tiles sums in place, total to *total_dev
+__global__ __launch_bounds__(256) void k_scan_sums(int* sums, int ntiles, int* total_dev) {
+    __shared__ int lds4[4];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < ntiles; base += 1024) {
+        int v[4];
+        int local = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = base + threadIdx.x * 4 + q;
+            v[q] = i < ntiles ? sums[i] : 0;
+            local += v[q];
+        }
+        int total;
+        int off = block_excl_scan256(local, total, lds4) + carry;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = base + threadIdx.x * 4 + q;
+            if (i < ntiles) sums[i] = off;
+            off += v[q];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total_dev) *total_dev = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(int* out, const int* sums, const int* n_dev, int64_t n_host) {
+    int64_t n = n_dev ? (int64_t)*n_dev : n_host;
+    n = n < n_host ? n : n_host;
+    const int add = sums[blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t i = base + threadIdx.x * 4 + q;
+        if (i < n) out[i] += add;
+    }
+}
+
+hipError_t scan_exclusive(mpmx_ctx* c, const int* in, int* out, int64_t n_cap, const int* n_dev,
+                          int64_t n_host, int* total_dev) {
+    const int64_t tiles = (n_cap + kScanTile - 1) / kScanTile;
+    if (tiles < 1) return hipSuccess;
+    if (tiles > c->scan_tmp_cap) return hipErrorInvalidValue;
+    // with a device count, n_host carries the capacity bound
+    const int64_t bound = n_dev ? n_cap : n_host;
+    k_scan_tiles<<<(int)tiles, 256, 0, c->stream>>>(in, out, c->scan_tmp, n_dev, bound);
+    k_scan_sums<<<1, 256, 0, c->stream>>>(c->scan_tmp, (int)tiles, total_dev);
+    k_scan_add<<<(int)tiles, 256, 0, c->stream>>>(out, c->scan_tmp, n_dev, bound);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- keys
+// cal_cellId (Particle_kernal.cu:158-165): key of the quadratic base node
+__global__ void k_keys(const float* __restrict__ x, uint32_t* __restrict__ keys, int n, int N,
+                       DevCounts* dc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
+    const float lo = 2.0f / N, hi = 1.0f - 2.0f / N;
+    if (!(x0 >= lo && x0 <= hi && x1 >= lo && x1 <= hi && x2 >= lo && x2 <= hi))
+        atomicOr(&dc->err, kErrDomain);
+    keys[i] = key_of(base_node(x0, N), base_node(x1, N), base_node(x2, N));
+}
+
+hipError_t compute_keys(mpmx_ctx* c) {
+    const int n = (int)c->n;
+    const float* x = c->cfg.precision == MPMX_MIXED ? c->ms[c->cur].x : c->ps[c->cur].x;
+    k_keys<<<(n + 255) / 256, 256, 0, c->stream>>>(x, c->keys, n, c->P.N, c->dc);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- radix sort
+constexpr int kRsThreads = 256;
+constexpr int kRsItems = 8;
+constexpr int kRsTile = kRsThreads * kRsItems;
+
+// per-tile digit histogram, digit-major: hist[d * tiles + t]
+__global__ __launch_bounds__(kRsThreads) void k_radix_hist(const uint32_t* __restrict__ keys, int n,
+                                                           int shift, uint32_t* __restrict__ hist,
+                                                           int tiles) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kRsTile;
+#pragma unroll
+    for (int j = 0; j < kRsItems; ++j) {
+        const int i = base + j * kRsThreads + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// stable scatter: items are ranked in (round, wave, lane) order, which is their
+// index order inside the tile; same-digit peers inside a wave are found with
+// eight 64-lane ballots
+__global__ __launch_bounds__(kRsThreads) void k_radix_scatter(
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint32_t* __restrict__ okeys,
+    uint32_t* __restrict__ ovals, int n, int shift, const uint32_t* __restrict__ offs, int tiles) {
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t wcnt[4][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    running[threadIdx.x] = offs[threadIdx.x * tiles + blockIdx.x];
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int j = 0; j < kRsItems; ++j) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wcnt[q][threadIdx.x] = 0;
+        __syncthreads();
+        const int i = blockIdx.x * kRsTile + j * kRsThreads + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t k = valid ? keys[i] : 0u;
+        const uint32_t d = (k >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int bit = (d >> b) & 1;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = __popcll(peers & lt);
+        if (valid && (peers & lt) == 0ull) wcnt[w][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = running[d] + rank;
+            for (int q = 0; q < w; ++q) pos += wcnt[q][d];
+            okeys[pos] = k;
+            ovals[pos] = vals ? vals[i] : (uint32_t)i;
+        }
+        __syncthreads();
+        running[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] + wcnt[2][threadIdx.x] +
+                                wcnt[3][threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- gather
+__global__ void k_gather_fp32(const uint32_t* __restrict__ perm, PState s, PState d, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = (int)perm[i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        d.x[3 * i + a] = s.x[3 * o + a];
+        d.v[3 * i + a] = s.v[3 * o + a];
+    }
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+        d.B[(int64_t)a * n + i] = s.B[(int64_t)a * n + o];
+        d.F[(int64_t)a * n + i] = s.F[(int64_t)a * n + o];
+    }
+    d.c[i] = s.c[o];
+    d.alpha[i] = s.alpha[o];
+    d.psi[i] = s.psi[o];
+    d.g[i] = s.g[o];
+    d.mass[i] = s.mass[o];
+    d.id[i] = s.id[o];
+}
+
+__global__ void k_gather_mixed(const uint32_t* __restrict__ perm, MState s, MState d, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = (int)perm[i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        d.x[3 * i + a] = s.x[3 * o + a];
+        d.v[3 * i + a] = s.v[3 * o + a];
+    }
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+        d.B[(int64_t)a * n + i] = s.B[(int64_t)a * n + o];
+        d.F[(int64_t)a * n + i] = s.F[(int64_t)a * n + o];
+    }
+    d.c[i] = s.c[o];
+    d.alpha[i] = s.alpha[o];
+    d.psi[i] = s.psi[o];
+    d.g[i] = s.g[o];
+    d.mass[i] = s.mass[o];
+    d.id[i] = s.id[o];
+}
+
+hipError_t sort_and_gather(mpmx_ctx* c) {
+    const int n = (int)c->n;
+    const int tiles = c->sort_tiles;
+    uint32_t* kb[2] = {c->keys, c->keys_alt};
+    uint32_t* vb[2] = {c->vals, c->vals_alt};
+    int ki = 0;    // buffer holding the current keys
+    int vi = -1;   // buffer holding the current permutation (-1: identity)
+    const int passes = (c->key_bits + 7) / 8;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        const int ko = 1 - ki;
+        const int vo = (vi == 1) ? 0 : 1;
+        k_radix_hist<<<tiles, kRsThreads, 0, c->stream>>>(kb[ki], n, shift, c->hist, tiles);
+        hipError_t e = scan_exclusive(c, (const int*)c->hist, (int*)c->hist, (int64_t)256 * tiles,
+                                      nullptr, (int64_t)256 * tiles, nullptr);
+        if (e != hipSuccess) return e;
+        k_radix_scatter<<<tiles, kRsThreads, 0, c->stream>>>(kb[ki], vi < 0 ? nullptr : vb[vi], kb[ko],
+                                                             vb[vo], n, shift, c->hist, tiles);
+        ki = ko;
+        vi = vo;
+    }
+    // keep the sorted keys in c->keys and the permutation (new -> old) in c->vals
+    hipError_t e = hipSuccess;
+    if (ki != 0)
+        e = hipMemcpyAsync(c->keys, kb[ki], sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess && vi != 0)
+        e = hipMemcpyAsync(c->vals, vb[vi], sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) return e;
+    const int nxt = 1 - c->cur;
+    if (c->cfg.precision == MPMX_MIXED)
+        k_gather_mixed<<<(n + 255) / 256, 256, 0, c->stream>>>(c->vals, c->ms[c->cur], c->ms[nxt], n);
+    else
+        k_gather_fp32<<<(n + 255) / 256, 256, 0, c->stream>>>(c->vals, c->ps[c->cur], c->ps[nxt], n);
+    c->cur = nxt;
+    return hipGetLastError();
+}
+
+}  // namespace mpmx

@@ -1,0 +1,254 @@
+// k_stress.hip — per-particle von Mises return mapping with hardening,
+// psi+ history, phase-field coefficient and Kirchhoff stress.
+//
+// Same maths as applyVonMises (ExplicitTimeIntegrator.cu:132-370), including the
+// reference's choices: the yield test uses the full norm of the trial Kirchhoff
+// stress, dlambda = y / ((2/3) g tr tau) capped at 100, alpha is stored clamped
+// to alpha_c while the unclamped value feeds G_c, Hencky strain components
+// above 3 are rescaled by 3/|eps|, and psi+ takes the deviatoric energy only
+// under compression (J < 1).  C^-1 (the reference's in-kernel Gauss-Jordan
+// Inverse, :71-129) is the same for every particle and comes precomputed in
+// DevParams.
+//
+// One lane per particle; F, sigma component-major (coalesced across lanes).
+#include "mpmx_internal.h"
+#include "svd3.h"
+#include "precision.h"
+
+namespace mpmx {
+
+struct StressIn {
+    float F[3][3];
+    float c, g, alpha, psi;
+};
+struct StressOut {
+    float F[3][3];
+    float g, alpha, psi, FP, vol;
+    float sigma[3][3];
+};
+
+template <typename T>
+__device__ __forceinline__ void usv(const T (&U)[3][3], const T (&d)[3], const T (&V)[3][3], T (&P)[3][3]) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+            P[r][cc] = d[0] * U[r][0] * V[cc][0] + d[1] * U[r][1] * V[cc][1] + d[2] * U[r][2] * V[cc][2];
+}
+
+// the return mapping in precision T (float, or double for the near-degenerate fallback)
+template <typename T>
+__device__ __forceinline__ void von_mises(const DevParams& dp, const StressIn& in, StressOut& out,
+                                          const T (&U)[3][3], T (&S)[3], const T (&V)[3][3]) {
+    const T lambda = dp.lambda, mu = dp.mu, kappa = dp.kappa;
+    T g = in.g;
+    T alpha = in.alpha;
+    T Fm[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) Fm[r][cc] = in.F[r][cc];
+
+    const T c0 = T(2) * mu + lambda;
+    T eps[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) eps[a] = (S[a] > T(0)) ? log(S[a]) : T(0);
+    T tau_tr[3];
+    tau_tr[0] = c0 * eps[0] + lambda * eps[1] + lambda * eps[2];
+    tau_tr[1] = lambda * eps[0] + c0 * eps[1] + lambda * eps[2];
+    tau_tr[2] = lambda * eps[0] + lambda * eps[1] + c0 * eps[2];
+    const T tau_tr_F = sqrt(tau_tr[0] * tau_tr[0] + tau_tr[1] * tau_tr[1] + tau_tr[2] * tau_tr[2]);
+    const T tau_tr_sum = tau_tr[0] + tau_tr[1] + tau_tr[2];
+    const T divider = tau_tr_sum / T(3);
+    T tau_dev_tr[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) tau_dev_tr[a] = g * (tau_tr[a] - divider);
+
+    const T p23 = T(2.0f / 3.0f);
+    const T sq23 = sqrt(p23);
+    const T Rh = T(dp.sq23f) * T(dp.yield) - (T(dp.sq23f) * T(dp.yield) - T(dp.yield)) / exp(T(dp.hard) * alpha);
+    const T y = tau_tr_F - g * Rh * sq23;
+    if (y >= T(1e-12)) {
+        T dlambda = T(0);
+        const T bss = p23 * g * tau_tr_sum;
+        if (bss > T(0)) dlambda = y / bss;
+        if (dlambda > T(dp.dlambda_cap)) dlambda = T(dp.dlambda_cap);
+        if (alpha < T(dp.alpha_c)) {
+            alpha += sq23 * dlambda;
+            out.alpha = (float)alpha;
+        }
+        if (alpha > T(dp.alpha_c)) out.alpha = dp.alpha_c;
+        const T tdn = sqrt(tau_dev_tr[0] * tau_dev_tr[0] + tau_dev_tr[1] * tau_dev_tr[1] + tau_dev_tr[2] * tau_dev_tr[2]);
+        const T taudiv = (T(1) / tdn) * p23 * g * tau_tr_sum * dlambda;
+        const T third = (T(1) / T(3)) * tau_tr_sum;
+        T tau_nn[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) tau_nn[a] = (tau_dev_tr[a] - tau_dev_tr[a] * taudiv) + third;
+        T eps_nn[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            eps_nn[a] = T(dp.Cinv[3 * a]) * tau_nn[0] + T(dp.Cinv[3 * a + 1]) * tau_nn[1] +
+                        T(dp.Cinv[3 * a + 2]) * tau_nn[2];
+        const T normS = sqrt(eps_nn[0] * eps_nn[0] + eps_nn[1] * eps_nn[1] + eps_nn[2] * eps_nn[2]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (eps_nn[a] > T(dp.eps_cap)) eps_nn[a] = eps_nn[a] * (T(dp.eps_cap) / normS);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) S[a] = exp(eps_nn[a]);
+        usv(U, S, V, Fm);
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) out.F[r][cc] = (float)Fm[r][cc];
+
+    const T J = S[0] * S[1] * S[2];
+    const T J_bar = pow(J, -p23);
+    T be[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) be[a] = J_bar * S[a] * S[a];
+    const T be_sum = be[0] + be[1] + be[2];
+    const T ln_J = (J > T(0)) ? log(J) : T(0);
+    const T Psi_vol = (kappa * T(0.5)) * (((J * J - T(1)) * T(0.5)) - ln_J);
+    const T Psi_dev = (mu * T(0.5)) * (be_sum - T(3));
+    const T Psi_pos = (J < T(1)) ? Psi_dev : Psi_dev + Psi_vol;
+    const T smax = fmax(S[0], fmax(S[1], S[2]));
+    T psi = in.psi;
+    if (smax >= T(dp.crit_stretch) && Psi_pos > psi && Psi_pos < T(1e6)) psi = Psi_pos;
+    out.psi = (float)psi;
+
+    const T pp = alpha / T(2);
+    const T G_C = (pp < T(dp.ps)) ? T(dp.Gc0)
+                                  : T(dp.Gc0) * (((T(1) - T(dp.residual_c)) * exp(T(dp.ps) - pp)) + T(dp.residual_c));
+    const T theta = dp.theta;
+    out.FP = (float)(T(4) * T(dp.L0) * (T(1) - theta) * psi * (T(1) / G_C) + T(1));
+
+    const T tau_vol = (kappa * T(0.5)) * (J * J - T(1));
+    const T bthird = (T(1) / T(3)) * be_sum;
+    g = T(in.c) * T(in.c) * (T(1) - theta) + theta;
+    out.g = (float)g;
+    T P_hat[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const T td = mu * (be[a] - bthird);
+        P_hat[a] = (J < T(1)) ? g * td + tau_vol : g * (td + tau_vol);
+    }
+    T P[3][3];
+    usv(U, P_hat, V, P);
+    // sigma = V0 P F^T
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+            out.sigma[r][cc] = (float)((P[r][0] * Fm[cc][0] + P[r][1] * Fm[cc][1] + P[r][2] * Fm[cc][2]) * T(dp.V0));
+    const T det = Fm[0][0] * (Fm[1][1] * Fm[2][2] - Fm[1][2] * Fm[2][1]) -
+                  Fm[0][1] * (Fm[1][0] * Fm[2][2] - Fm[1][2] * Fm[2][0]) +
+                  Fm[0][2] * (Fm[1][0] * Fm[2][1] - Fm[1][1] * Fm[2][0]);
+    out.vol = (float)(T(dp.V0) * det);
+}
+
+__device__ __forceinline__ void stress_one(const DevParams& dp, const StressIn& in, StressOut& out,
+                                           bool allow_fp64, int* fallbacks) {
+    out.alpha = in.alpha;
+    float A[3][3], U[3][3], S[3], V[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) A[r][cc] = in.F[r][cc];
+    svd3<float>(A, U, S, V);
+    const float gap = fminf(fabsf(S[0] - S[1]), fabsf(S[1] - S[2]));
+    if (allow_fp64 && gap < dp.svd_gap) {
+        // near-degenerate singular values: redo the decomposition and the
+        // return mapping in FP64
+        double Ad[3][3], Ud[3][3], Sd[3], Vd[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) Ad[r][cc] = A[r][cc];
+        svd3<double>(Ad, Ud, Sd, Vd);
+        von_mises<double>(dp, in, out, Ud, Sd, Vd);
+        atomicAdd(fallbacks, 1);
+        return;
+    }
+    von_mises<float>(dp, in, out, U, S, V);
+}
+
+__global__ __launch_bounds__(256) void k_stress_fp32(DevParams dp, PState s, float* __restrict__ sigma,
+                                                     float* __restrict__ FP, float* __restrict__ vol, int n,
+                                                     DevCounts* dc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    StressIn in;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) in.F[r][cc] = s.F[(int64_t)(r + 3 * cc) * n + i];
+    in.c = s.c[i];
+    in.g = s.g[i];
+    in.alpha = s.alpha[i];
+    in.psi = s.psi[i];
+    StressOut out;
+    stress_one(dp, in, out, false, &dc->svd_fallbacks);
+    bool finite = true;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            s.F[(int64_t)(r + 3 * cc) * n + i] = out.F[r][cc];
+            sigma[(int64_t)(r + 3 * cc) * n + i] = out.sigma[r][cc];
+            finite = finite && isfinite(out.sigma[r][cc]);
+        }
+    s.g[i] = out.g;
+    s.alpha[i] = out.alpha;
+    s.psi[i] = out.psi;
+    FP[i] = out.FP;
+    vol[i] = out.vol;
+    if (!finite) atomicOr(&dc->err, kErrNonFinite);
+}
+
+__global__ __launch_bounds__(256) void k_stress_mixed(DevParams dp, MState s, float* __restrict__ sigma,
+                                                      float* __restrict__ FP, float* __restrict__ vol, int n,
+                                                      DevCounts* dc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    StressIn in;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+            in.F[r][cc] = h2f(s.F[(int64_t)(r + 3 * cc) * n + i]) + (r == cc ? 1.0f : 0.0f);
+    in.c = bf2f(s.c[i]);
+    in.g = bf2f(s.g[i]);
+    in.alpha = bf2f(s.alpha[i]);
+    in.psi = bf2f(s.psi[i]);
+    StressOut out;
+    stress_one(dp, in, out, true, &dc->svd_fallbacks);
+    bool finite = true;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            s.F[(int64_t)(r + 3 * cc) * n + i] = f2h(out.F[r][cc] - (r == cc ? 1.0f : 0.0f));
+            sigma[(int64_t)(r + 3 * cc) * n + i] = out.sigma[r][cc];
+            finite = finite && isfinite(out.sigma[r][cc]);
+        }
+    s.g[i] = f2bf(out.g);
+    s.alpha[i] = f2bf(out.alpha);
+    s.psi[i] = f2bf(out.psi);
+    FP[i] = out.FP;
+    vol[i] = out.vol;
+    if (!finite) atomicOr(&dc->err, kErrNonFinite);
+}
+
+hipError_t launch_stress(mpmx_ctx* c) {
+    const int n = (int)c->n;
+    if (c->cfg.precision == MPMX_MIXED)
+        k_stress_mixed<<<(n + 255) / 256, 256, 0, c->stream>>>(c->dp, c->ms[c->cur], c->sigma, c->FP, c->vol,
+                                                               n, c->dc);
+    else
+        k_stress_fp32<<<(n + 255) / 256, 256, 0, c->stream>>>(c->dp, c->ps[c->cur], c->sigma, c->FP, c->vol,
+                                                              n, c->dc);
+    return hipGetLastError();
+}
+
+}  // namespace mpmx

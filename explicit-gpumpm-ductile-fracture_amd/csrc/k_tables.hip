@@ -1,0 +1,166 @@
+// k_tables.hip — sparse page tables from the sorted particle keys.
+//
+// Replaces DomainTransformer::rebuild (Transformer.cpp:125-136,
+// Transformer_kernal.cu:15-209).  The reference numbers pages through a hash
+// table with atomicAdd, so ids depend on scheduling.  Here the occupied pages
+// and their +x/+y/+z neighbours are set in two dense bitmasks over the Morton
+// page space (one bit per page: 2 MiB at 512^3), and a page's id is its rank
+// in the mask: particle pages first, in ascending key order, then
+// neighbour-only pages, also ascending.  Ranks are popcounts plus a scanned
+// per-word prefix, so the adjacency lookup needs no hash probing.  All sizes
+// stay on the device (DevCounts), so the whole build can be graph-captured.
+#include "mpmx_internal.h"
+
+namespace mpmx {
+
+// neighbour q (0..6) of a page = offset (i,j,k) with q+1 = i*4 + j*2 + k
+// (Transformer.cpp:81-88)
+__device__ __forceinline__ uint32_t neighbor_key(uint32_t pk, int q) {
+    int x, y, z;
+    unmorton(pk, x, y, z);
+    const int o = q + 1;
+    return morton((uint32_t)(x + ((o >> 2) & 1)), (uint32_t)(y + ((o >> 1) & 1)), (uint32_t)(z + (o & 1)));
+}
+
+__device__ __forceinline__ int page_rank(uint32_t k, const uint64_t* __restrict__ pmask,
+                                         const uint64_t* __restrict__ nmask, const int* __restrict__ wpre,
+                                         const int* __restrict__ spre, int num_page) {
+    const uint32_t w = k >> 6, b = k & 63u;
+    const uint64_t below = b ? ((~0ull) >> (64 - b)) : 0ull;
+    const uint64_t P = pmask[w];
+    if ((P >> b) & 1ull) return wpre[w] + __popcll(P & below);
+    const uint64_t S = nmask[w] & ~P;
+    if ((S >> b) & 1ull) return num_page + spre[w] + __popcll(S & below);
+    return -1;
+}
+
+// markPageBoundary + buildHashMapFromPage + supplementAdjacentPages, as bits
+__global__ void k_mark_pages(const uint32_t* __restrict__ keys, int n, uint64_t* pmask, uint64_t* nmask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t pk = keys[i] >> 6;
+    if (i > 0 && (keys[i - 1] >> 6) == pk) return;
+    atomicOr((unsigned long long*)&pmask[pk >> 6], 1ull << (pk & 63u));
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        const uint32_t nk = neighbor_key(pk, q);
+        atomicOr((unsigned long long*)&nmask[nk >> 6], 1ull << (nk & 63u));
+    }
+}
+
+__global__ void k_word_counts(const uint64_t* __restrict__ pmask, const uint64_t* __restrict__ nmask,
+                              int* wcount, int* scount, int64_t words) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    const uint64_t P = pmask[w];
+    wcount[w] = __popcll(P);
+    scount[w] = __popcll(nmask[w] & ~P);
+}
+
+__global__ void k_page_keys(const uint64_t* __restrict__ pmask, const uint64_t* __restrict__ nmask,
+                            const int* __restrict__ wpre, const int* __restrict__ spre, int64_t words,
+                            uint32_t* page_key, int64_t cap_p, int64_t cap_tp, DevCounts* dc) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int num_page = dc->num_page;
+    if (w == 0) {
+        const int total = num_page + dc->num_supp;
+        dc->num_total = total;
+        if (num_page > cap_p || total > cap_tp) atomicOr(&dc->err, kErrCapacity);
+    }
+    if (w >= words) return;
+    uint64_t P = pmask[w];
+    int id = wpre[w];
+    while (P) {
+        const int b = __ffsll((unsigned long long)P) - 1;
+        if (id < cap_p && id < cap_tp) page_key[id] = (uint32_t)(w * 64 + b);
+        ++id;
+        P &= P - 1;
+    }
+    uint64_t S = nmask[w] & ~pmask[w];
+    id = num_page + spre[w];
+    while (S) {
+        const int b = __ffsll((unsigned long long)S) - 1;
+        if (id < cap_tp) page_key[id] = (uint32_t)(w * 64 + b);
+        ++id;
+        S &= S - 1;
+    }
+}
+
+// markBlockOffset on page boundaries (Transformer_kernal.cu:22-36): first particle of each page
+__global__ void k_page_heads(const uint32_t* __restrict__ keys, int n, const uint64_t* __restrict__ pmask,
+                             const uint64_t* __restrict__ nmask, const int* __restrict__ wpre,
+                             const int* __restrict__ spre, int* page2particle, int64_t cap_p,
+                             const DevCounts* dc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int num_page = dc->num_page;
+    if (i == n - 1 && num_page <= cap_p) page2particle[num_page] = n;
+    const uint32_t pk = keys[i] >> 6;
+    if (i > 0 && (keys[i - 1] >> 6) == pk) return;
+    const int id = page_rank(pk, pmask, nmask, wpre, spre, num_page);
+    if (id >= 0 && id < cap_p) page2particle[id] = i;
+}
+
+// establishPageTopology + markPageSize (Transformer_kernal.cu:98-120)
+__global__ void k_page_topology(const uint32_t* __restrict__ page_key, const int* __restrict__ page2particle,
+                                const uint64_t* __restrict__ pmask, const uint64_t* __restrict__ nmask,
+                                const int* __restrict__ wpre, const int* __restrict__ spre, int* adj,
+                                int* nvp, int64_t cap_p, int vpage, DevCounts* dc) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int num_page = dc->num_page;
+    num_page = num_page < cap_p ? num_page : (int)cap_p;
+    if (p >= num_page) return;
+    const uint32_t pk = page_key[p];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        const int id = page_rank(neighbor_key(pk, q), pmask, nmask, wpre, spre, dc->num_page);
+        if (id < 0) atomicOr(&dc->err, kErrCapacity);
+        adj[(int64_t)q * cap_p + p] = id < 0 ? 0 : id;
+    }
+    const int cnt = page2particle[p + 1] - page2particle[p];
+    nvp[p] = (cnt + vpage - 1) / vpage;
+}
+
+// markVirtualPageOffset + inclusive scan (Transformer_kernal.cu:122-126, 196-208)
+__global__ void k_vtargets(int* vpo, int* target, int64_t cap_p, int64_t cap_vp, DevCounts* dc) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int num_page = dc->num_page;
+    num_page = num_page < cap_p ? num_page : (int)cap_p;
+    if (p >= num_page) return;
+    const int nv = dc->num_vp;
+    if (p == 0 && nv > cap_vp) atomicOr(&dc->err, kErrCapacity);
+    const int b = vpo[p];
+    const int e = (p + 1 < num_page) ? vpo[p + 1] : nv;
+    if (p == num_page - 1) vpo[num_page] = nv;
+    for (int v = b; v < e && v < cap_vp; ++v) target[v] = p;
+}
+
+hipError_t build_tables(mpmx_ctx* c) {
+    const int n = (int)c->n;
+    const int64_t words = c->mask_words;
+    hipError_t e;
+    if ((e = fill_zero(c, c->pmask, words * 8)) != hipSuccess) return e;
+    if ((e = fill_zero(c, c->nmask, words * 8)) != hipSuccess) return e;
+    k_mark_pages<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask, c->nmask);
+    k_word_counts<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pmask, c->nmask, c->wcount,
+                                                                      c->scount, words);
+    if ((e = scan_exclusive(c, c->wcount, c->wcount, words, nullptr, words, &c->dc->num_page)) != hipSuccess)
+        return e;
+    if ((e = scan_exclusive(c, c->scount, c->scount, words, nullptr, words, &c->dc->num_supp)) != hipSuccess)
+        return e;
+    k_page_keys<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pmask, c->nmask, c->wcount, c->scount,
+                                                                    words, c->page_key, c->cap_p, c->cap_tp,
+                                                                    c->dc);
+    k_page_heads<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask, c->nmask, c->wcount,
+                                                          c->scount, c->page2particle, c->cap_p, c->dc);
+    const int pblocks = (int)((c->cap_p + 255) / 256);
+    k_page_topology<<<pblocks, 256, 0, c->stream>>>(c->page_key, c->page2particle, c->pmask, c->nmask,
+                                                     c->wcount, c->scount, c->adj, c->nvp, c->cap_p,
+                                                     c->P.vpage, c->dc);
+    if ((e = scan_exclusive(c, c->nvp, c->vpo, c->cap_p, &c->dc->num_page, 0, &c->dc->num_vp)) != hipSuccess)
+        return e;
+    k_vtargets<<<pblocks, 256, 0, c->stream>>>(c->vpo, c->target, c->cap_p, c->cap_vp, c->dc);
+    return hipGetLastError();
+}
+
+}  // namespace mpmx

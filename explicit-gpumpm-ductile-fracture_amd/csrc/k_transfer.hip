@@ -1,0 +1,426 @@
+// k_transfer.hip — particle-to-grid transfer, grid update with boundaries,
+// and grid-to-particle transfer (MLS and APIC).
+//
+//   P2G  replaces P2G_MLS / P2G_APIC (P2GKernal.cu:6-155, 512-702) and the two
+//        phase-field particle passes volP2G_APIC / preConditionP2G_APIC
+//        (P2GKernal.cu:897-1157): their per-node sums are sum(V w) and
+//        sum(V FP w) times node constants, so they ride along as two more
+//        channels of the same pass.
+//   grid undateGrid_kernal + collideWithGround / PullGrid (TimeIntegrator.cu:63-261)
+//        plus the PCG set-up of ExplicitTimeIntegrator.cu:914-935.
+//   G2P  replaces G2P_MLS / G2P_APIC (G2PKernal.cu:32-441) and update_grid_phase
+//        (ExplicitTimeIntegrator.cu:632-642), whose increment x - c is formed
+//        while the tile is staged.
+//
+// One workgroup works through a contiguous run of virtual pages (<= vpage
+// particles of one 4x4x4-cell page, sorted by cell).  The 6x6x6 node tile that
+// the page's particles touch is staged in LDS; P2G accumulates it with LDS
+// float atomics and flushes only non-zero entries to HBM with global float
+// atomics (8 pages: the page and its 7 +x/+y/+z neighbours).
+#include "access.h"
+
+namespace mpmx {
+
+constexpr int kP2GCh = 8;   // m, p0, p1, p2, sum w c, sum w, sum V w, sum V FP w
+
+template <int BLOCK, int TRANSFER, typename ST>
+__global__ __launch_bounds__(BLOCK) void k_p2g(DevParams dp, ST s, const float* __restrict__ sigma,
+                                               const float* __restrict__ FP, const float* __restrict__ vol,
+                                               Tables t, float* __restrict__ grid, float dt, int n,
+                                               const DevCounts* __restrict__ dc) {
+    __shared__ float tile[kP2GCh * kTileN];
+    int nvp = dc->num_vp;
+    nvp = nvp < t.cap_vp ? nvp : (int)t.cap_vp;
+    int vb, ve;
+    block_range(nvp, vb, ve);
+    for (int vp = vb; vp < ve; ++vp) {
+        for (int q = threadIdx.x; q < kP2GCh * kTileN; q += BLOCK) tile[q] = 0.f;
+        __syncthreads();
+        int p, s0, e;
+        vpage_range(t, vp, BLOCK, n, p, s0, e);
+        const int i = s0 + threadIdx.x;
+        if (i < e) {
+            int pc[3];
+            unmorton(t.page_key[p], pc[0], pc[1], pc[2]);
+            float d0[3], w[3][3], gw[3][3];
+            int l[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float xa = ld_x(s, i, a);
+                const int b = base_node(xa, dp.N);
+                d0[a] = xa * dp.inv_dx - (float)b;
+                int la = b - 4 * pc[a];
+                l[a] = la < 0 ? 0 : (la > 3 ? 3 : la);
+                if (TRANSFER == 0) bspline(d0[a], w[a]);
+                else bspline_grad(d0[a], w[a], gw[a]);
+            }
+            const float m = ld_m(s, i);
+            const float cphase = ld_c(s, i);
+            const float V = vol[i];
+            const float VF = V * FP[i];
+            float v[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) v[a] = ld_v(s, i, a);
+            // affine matrix per node offset in units of dx: p_r = w (a_r + sum_c Qd(r,c) o_c)
+            float Qd[9], sg[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const float Bk = ld_B(s, i, k, n);
+                const float sk = sigma[(int64_t)k * n + i];
+                if (TRANSFER == 0) Qd[k] = (Bk * m - sk * dt) * dp.Dinv * dp.dx;
+                else { Qd[k] = Bk * dp.Dinv * dp.dx * m; sg[k] = sk * dt * dp.inv_dx; }
+            }
+            float ar[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                ar[r] = m * v[r] - (Qd[r] * d0[0] + Qd[r + 3] * d0[1] + Qd[r + 6] * d0[2]);
+#pragma unroll
+            for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                for (int jj = 0; jj < 3; ++jj) {
+                    const float wxy = w[0][ii] * w[1][jj];
+#pragma unroll
+                    for (int kk = 0; kk < 3; ++kk) {
+                        const float wt = wxy * w[2][kk];
+                        float pr[3];
+#pragma unroll
+                        for (int r = 0; r < 3; ++r)
+                            pr[r] = wt * (ar[r] + Qd[r] * ii + Qd[r + 3] * jj + Qd[r + 6] * kk);
+                        if (TRANSFER == 1) {
+                            // + dt * (-sigma grad w), gradient in units of 1/dx
+                            const float g0 = gw[0][ii] * w[1][jj] * w[2][kk];
+                            const float g1 = w[0][ii] * gw[1][jj] * w[2][kk];
+                            const float g2 = w[0][ii] * w[1][jj] * gw[2][kk];
+#pragma unroll
+                            for (int r = 0; r < 3; ++r) pr[r] -= sg[r] * g0 + sg[r + 3] * g1 + sg[r + 6] * g2;
+                        }
+                        const int node = (l[0] + ii) * 36 + (l[1] + jj) * 6 + (l[2] + kk);
+                        atomicAdd(&tile[0 * kTileN + node], wt * m);
+                        atomicAdd(&tile[1 * kTileN + node], pr[0]);
+                        atomicAdd(&tile[2 * kTileN + node], pr[1]);
+                        atomicAdd(&tile[3 * kTileN + node], pr[2]);
+                        atomicAdd(&tile[4 * kTileN + node], wt * cphase);
+                        atomicAdd(&tile[5 * kTileN + node], wt);
+                        atomicAdd(&tile[6 * kTileN + node], wt * V);
+                        atomicAdd(&tile[7 * kTileN + node], wt * VF);
+                    }
+                }
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < kP2GCh * kTileN; q += BLOCK) {
+            const float val = tile[q];
+            if (val != 0.f) {
+                const int ch = q / kTileN, node = q - ch * kTileN;
+                const int a = node / 36, b = (node / 6) % 6, cc = node % 6;
+                const int64_t gn = tile_node(t, p, a, b, cc);
+                if (gn >= 0) atomicAdd(&grid[(gn >> 6) * kPageFloats + ch * 64 + (gn & 63)], val);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// PCG scalars start from zero before the grid update accumulates rho into them
+__global__ void k_cg_reset(DevParams dp, DevCounts* dc) {
+    if (threadIdx.x != 0) return;
+    dc->cg_done = dp.phase_field ? 0 : 1;
+    dc->cg_iters = 0;
+    dc->cg_breakdown = 0;
+    dc->sq_key = 0u;
+    dc->rho_key = 0u;
+    dc->norm_key = 0u;
+    dc->sq_sum = 0.f;
+    dc->rho_sum = 0.f;
+    dc->rho_old = 0.f;
+}
+
+// undateGrid_kernal + collideWithGround/PullGrid + PCG initialisation.
+// One wavefront per 128 consecutive nodes (= two pages), two nodes per lane;
+// the 128-node blocks are exactly the reference's 128-entry dot-product blocks
+// (ExplicitTimeIntegrator.cu:573-609, 731-747).
+__global__ __launch_bounds__(256) void k_grid_update(DevParams dp, const uint32_t* __restrict__ page_key,
+                                                     float* __restrict__ grid, float* __restrict__ cgx,
+                                                     float* __restrict__ cgr, float* __restrict__ cgz,
+                                                     float* __restrict__ cgs, float* __restrict__ cgq,
+                                                     float dt, int64_t cap_tp, DevCounts* dc) {
+    int nt = dc->num_total;
+    nt = nt < cap_tp ? nt : (int)cap_tp;
+    const int nblk = (nt + 1) / 2;
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int blk = wave; blk < nblk; blk += nwaves) {
+        float rz = 0.f;
+        const int n0 = blk * 128 + lane * 2;
+        const int pg = n0 >> 6;
+        if (pg < nt) {
+            int px, py, pz;
+            unmorton(page_key[pg], px, py, pz);
+            float* gp = grid + (int64_t)pg * kPageFloats;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int cell = (n0 & 63) + h;
+                const int64_t nn = (int64_t)pg * 64 + cell;
+                const float W = gp[5 * 64 + cell];
+                float r = 0.f, z = 0.f;
+                if (W > 0.f) {
+                    const float m = gp[cell];
+                    const float inv = 1.0f / m;
+                    const float c = gp[4 * 64 + cell] / W;
+                    float v0 = gp[1 * 64 + cell] * inv + dp.gravity[0] * dt;
+                    float v1 = gp[2 * 64 + cell] * inv + dp.gravity[1] * dt;
+                    float v2 = gp[3 * 64 + cell] * inv + dp.gravity[2] * dt;
+                    const int gi = px * 4 + (cell >> 4), gj = py * 4 + ((cell >> 2) & 3), gk = pz * 4 + (cell & 3);
+                    if (dp.grip_lo >= 0) {
+                        if (gi <= dp.grip_lo) { v0 = -dp.grip_v; v1 = 0.f; v2 = 0.f; }
+                        if (gi >= dp.grip_hi) { v0 = dp.grip_v; v1 = 0.f; v2 = 0.f; }
+                    }
+                    const int lo = dp.wall, hi = dp.N - dp.wall;
+                    if (gi <= lo && v0 < 0.f) { v0 = 0.f; v1 = 0.f; v2 = 0.f; }
+                    if (gi > hi && v0 > 0.f) { v0 = 0.f; v1 = 0.f; v2 = 0.f; }
+                    if (gj <= lo && v1 < 0.f) { v0 = 0.f; v1 = 0.f; v2 = 0.f; }
+                    if (gj > hi && v1 > 0.f) { v0 = 0.f; v1 = 0.f; v2 = 0.f; }
+                    if (gk <= lo && v2 < 0.f) { v0 = 0.f; v1 = 0.f; v2 = 0.f; }
+                    if (gk > hi && v2 > 0.f) { v0 = 0.f; v1 = 0.f; v2 = 0.f; }
+                    gp[1 * 64 + cell] = v0;
+                    gp[2 * 64 + cell] = v1;
+                    gp[3 * 64 + cell] = v2;
+                    gp[4 * 64 + cell] = c;
+                    const float VW = gp[6 * 64 + cell], VFW = gp[7 * 64 + cell];
+                    const float rhs = VW * (c / dt + dp.M);
+                    const float D = dp.M * VFW + VW / dt;
+                    gp[6 * 64 + cell] = rhs;
+                    gp[7 * 64 + cell] = D;
+                    r = rhs;
+                    z = (fabsf(D) < 1e-30f) ? 0.f : r / D;
+                }
+                if (dp.phase_field) {
+                    cgx[nn] = 0.f;
+                    cgr[nn] = r;
+                    cgz[nn] = z;
+                    cgs[nn] = z;
+                    cgq[nn] = 0.f;
+                }
+                rz += r * z;
+            }
+        }
+        if (dp.phase_field) {
+            const float bs = wave_sum(rz);
+            if (lane == 0) {
+                if (dp.dot_mode == 0) atomicMax(&dc->rho_key, fkey(bs));
+                else atomicAdd(&dc->rho_sum, bs);
+            }
+        }
+    }
+}
+
+template <int BLOCK, int TRANSFER, typename ST>
+__global__ __launch_bounds__(BLOCK) void k_g2p(DevParams dp, ST s, Tables t, const float* __restrict__ grid,
+                                               const float* __restrict__ cgx, uint32_t* __restrict__ keys,
+                                               float dt, int n, DevCounts* dc) {
+    __shared__ float tv[4 * kTileN];
+    int nvp = dc->num_vp;
+    nvp = nvp < t.cap_vp ? nvp : (int)t.cap_vp;
+    const bool breakdown = dc->cg_breakdown != 0;
+    int vb, ve;
+    block_range(nvp, vb, ve);
+    const float lo = 2.0f * dp.dx, hi = 1.0f - 2.0f * dp.dx;
+    for (int vp = vb; vp < ve; ++vp) {
+        int p, s0, e;
+        vpage_range(t, vp, BLOCK, n, p, s0, e);
+        for (int q = threadIdx.x; q < kTileN; q += BLOCK) {
+            const int a = q / 36, b = (q / 6) % 6, cc = q % 6;
+            const int64_t gn = tile_node(t, p, a, b, cc);
+            float v0 = 0.f, v1 = 0.f, v2 = 0.f, dcn = 0.f;
+            if (gn >= 0) {
+                const float* gp = grid + (gn >> 6) * kPageFloats + (gn & 63);
+                v0 = gp[64];
+                v1 = gp[128];
+                v2 = gp[192];
+                // damage increment x - c (update_grid_phase); touched nodes only
+                if (dp.phase_field && gp[5 * 64] > 0.f) dcn = (breakdown ? 1e20f : cgx[gn]) - gp[4 * 64];
+            }
+            tv[q] = v0;
+            tv[kTileN + q] = v1;
+            tv[2 * kTileN + q] = v2;
+            tv[3 * kTileN + q] = dcn;
+        }
+        __syncthreads();
+        const int i = s0 + threadIdx.x;
+        if (i < e) {
+            int pc[3];
+            unmorton(t.page_key[p], pc[0], pc[1], pc[2]);
+            float d0[3], w[3][3], gw[3][3];
+            int l[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float xa = ld_x(s, i, a);
+                const int b = base_node(xa, dp.N);
+                d0[a] = xa * dp.inv_dx - (float)b;
+                int la = b - 4 * pc[a];
+                l[a] = la < 0 ? 0 : (la > 3 ? 3 : la);
+                if (TRANSFER == 0) bspline(d0[a], w[a]);
+                else bspline_grad(d0[a], w[a], gw[a]);
+            }
+            float vp3[3] = {0.f, 0.f, 0.f}, dcs = 0.f;
+            float Bn[9], G[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) { Bn[k] = 0.f; G[k] = 0.f; }
+#pragma unroll
+            for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                    for (int kk = 0; kk < 3; ++kk) {
+                        const int node = (l[0] + ii) * 36 + (l[1] + jj) * 6 + (l[2] + kk);
+                        const float wt = w[0][ii] * w[1][jj] * w[2][kk];
+                        const float vi[3] = {tv[node], tv[kTileN + node], tv[2 * kTileN + node]};
+                        dcs += wt * tv[3 * kTileN + node];
+                        const float o[3] = {((float)ii - d0[0]) * dp.dx, ((float)jj - d0[1]) * dp.dx,
+                                            ((float)kk - d0[2]) * dp.dx};
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) {
+                            vp3[r] += wt * vi[r];
+#pragma unroll
+                            for (int cc = 0; cc < 3; ++cc) Bn[r + 3 * cc] += wt * vi[r] * o[cc];
+                        }
+                        if (TRANSFER == 1) {
+                            const float g3[3] = {gw[0][ii] * w[1][jj] * w[2][kk] * dp.inv_dx,
+                                                 w[0][ii] * gw[1][jj] * w[2][kk] * dp.inv_dx,
+                                                 w[0][ii] * w[1][jj] * gw[2][kk] * dp.inv_dx};
+#pragma unroll
+                            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                                for (int cc = 0; cc < 3; ++cc) G[r + 3 * cc] += vi[r] * g3[cc];
+                        }
+                    }
+            if (TRANSFER == 0) {
+                const float nv = sqrtf(vp3[0] * vp3[0] + vp3[1] * vp3[1] + vp3[2] * vp3[2]);
+                if (nv > dp.vmax) {
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) vp3[r] = vp3[r] / nv * dp.vmax;
+                }
+            }
+            const float pc0 = ld_c(s, i);
+            const float nc = pc0 + dcs;
+            st_c(s, i, fmaxf(fminf(pc0, nc), 0.f));
+            float xn[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                st_v(s, i, r, vp3[r]);
+                xn[r] = ld_x(s, i, r) + vp3[r] * dt;
+                st_x(s, i, r, xn[r]);
+            }
+            float A[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) A[k] = (TRANSFER == 0) ? Bn[k] * dt * dp.Dinv : G[k] * dt;
+            A[0] += 1.f;
+            A[4] += 1.f;
+            A[8] += 1.f;
+            float F[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) F[k] = ld_F(s, i, k, n);
+            bool finite = true;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    const float f = A[r] * F[3 * cc] + A[r + 3] * F[3 * cc + 1] + A[r + 6] * F[3 * cc + 2];
+                    finite = finite && isfinite(f);
+                    st_F(s, i, r + 3 * cc, n, f);
+                }
+#pragma unroll
+            for (int k = 0; k < 9; ++k) st_B(s, i, k, n, Bn[k]);
+            if (!(xn[0] >= lo && xn[0] <= hi && xn[1] >= lo && xn[1] <= hi && xn[2] >= lo && xn[2] <= hi))
+                atomicOr(&dc->err, isfinite(xn[0] + xn[1] + xn[2]) ? kErrDomain : kErrNonFinite);
+            if (!finite) atomicOr(&dc->err, kErrNonFinite);
+            keys[i] = key_of(base_node(xn[0], dp.N), base_node(xn[1], dp.N), base_node(xn[2], dp.N));
+        }
+        __syncthreads();
+    }
+}
+
+static Tables tables_of(mpmx_ctx* c) {
+    return Tables{c->page_key, c->page2particle, c->adj, c->vpo, c->target, c->cap_p, c->cap_tp, c->cap_vp};
+}
+
+int transfer_grid_blocks(mpmx_ctx* c) {
+    // enough workgroups to fill every CU several times over; a multiple of 8 so
+    // that the XCD-aware remap of block ids applies
+    int g = c->num_cu * 8;
+    const int64_t need = c->cap_vp;
+    if (need < g) g = (int)((need + 7) / 8 * 8);
+    return g < 8 ? 8 : g;
+}
+
+template <int BLOCK, typename ST>
+static void p2g_dispatch(mpmx_ctx* c, ST st, float dt) {
+    const int G = transfer_grid_blocks(c);
+    const Tables t = tables_of(c);
+    if (c->P.transfer == 0)
+        k_p2g<BLOCK, 0, ST><<<G, BLOCK, 0, c->stream>>>(c->dp, st, c->sigma, c->FP, c->vol, t, c->grid, dt,
+                                                        (int)c->n, c->dc);
+    else
+        k_p2g<BLOCK, 1, ST><<<G, BLOCK, 0, c->stream>>>(c->dp, st, c->sigma, c->FP, c->vol, t, c->grid, dt,
+                                                        (int)c->n, c->dc);
+}
+
+template <typename ST>
+static void p2g_block(mpmx_ctx* c, ST st, float dt) {
+    switch (c->P.vpage) {
+    case 64: p2g_dispatch<64>(c, st, dt); break;
+    case 128: p2g_dispatch<128>(c, st, dt); break;
+    default: p2g_dispatch<256>(c, st, dt); break;
+    }
+}
+
+hipError_t launch_p2g(mpmx_ctx* c, float dt) {
+    hipError_t e = fill_zero_dev_count(c, c->grid, &c->dc->num_total, kPageFloats, c->cap_tp);
+    if (e != hipSuccess) return e;
+    if (c->cfg.precision == MPMX_MIXED) p2g_block(c, c->ms[c->cur], dt);
+    else p2g_block(c, c->ps[c->cur], dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_update(mpmx_ctx* c, float dt) {
+    if (c->keep_p2g && c->grid_p2g) {
+        hipError_t e = hipMemcpyAsync(c->grid_p2g, c->grid, sizeof(float) * c->cap_tp * kPageFloats,
+                                      hipMemcpyDeviceToDevice, c->stream);
+        if (e != hipSuccess) return e;
+    }
+    int64_t blocks = (c->cap_tp * 64 / 128 + 3) / 4;
+    blocks = blocks < (int64_t)c->num_cu * 16 ? blocks : (int64_t)c->num_cu * 16;
+    if (blocks < 1) blocks = 1;
+    k_cg_reset<<<1, 64, 0, c->stream>>>(c->dp, c->dc);
+    k_grid_update<<<(int)blocks, 256, 0, c->stream>>>(c->dp, c->page_key, c->grid, c->cg_x, c->cg_r, c->cg_z,
+                                                       c->cg_s, c->cg_q, dt, c->cap_tp, c->dc);
+    return hipGetLastError();
+}
+
+template <int BLOCK, typename ST>
+static void g2p_dispatch(mpmx_ctx* c, ST st, float dt) {
+    const int G = transfer_grid_blocks(c);
+    const Tables t = tables_of(c);
+    if (c->P.transfer == 0)
+        k_g2p<BLOCK, 0, ST><<<G, BLOCK, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys, dt, (int)c->n,
+                                                        c->dc);
+    else
+        k_g2p<BLOCK, 1, ST><<<G, BLOCK, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys, dt, (int)c->n,
+                                                        c->dc);
+}
+template <typename ST>
+static void g2p_block(mpmx_ctx* c, ST st, float dt) {
+    switch (c->P.vpage) {
+    case 64: g2p_dispatch<64>(c, st, dt); break;
+    case 128: g2p_dispatch<128>(c, st, dt); break;
+    default: g2p_dispatch<256>(c, st, dt); break;
+    }
+}
+
+hipError_t launch_g2p(mpmx_ctx* c, float dt) {
+    if (c->cfg.precision == MPMX_MIXED) g2p_block(c, c->ms[c->cur], dt);
+    else g2p_block(c, c->ps[c->cur], dt);
+    return hipGetLastError();
+}
+
+}  // namespace mpmx

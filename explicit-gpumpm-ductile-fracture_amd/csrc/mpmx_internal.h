@@ -1,0 +1,318 @@
+// mpmx_internal.h — context layout and device helpers shared by the kernels of
+// libmpmx_hip.so.  Not part of the public ABI (include/mpmx.h is).
+//
+// HBM layout (FP32 plain path), all buffers owned by the context:
+//  particles  (sorted "current" order, double-buffered for the gather):
+//      x, v       float[3n] AoS (reference vector3T layout)
+//      B, F       float[9n] component-major: M(r,c) of particle i at m[(r+3c)*n + i]
+//                 (reference d_F/d_B layout, Particles.cpp:92-93)
+//      c, alpha, psi, g, mass  float[n];  id int[n] (original particle id)
+//  per-substep: sigma float[9n], FP float[n], vol float[n]
+//  keys       uint32[n]: (Morton page key << 6) | cell, order-equivalent to the
+//             reference's 64-bit SPGrid offset (Particle_kernal.cu:158-165)
+//  pages      page_key uint32[cap_tp] (particle pages ascending, then
+//             neighbour-only pages ascending), page2particle int[cap_p+1],
+//             adj int[7*cap_p], vpo int[cap_p+1], target int[cap_vp] (0-based)
+//  grid       float[cap_tp][8][64]: per page, 8 channels of 64 cells (256 B each)
+//             0 m | 1-3 momentum -> velocity | 4 sum w c -> c | 5 sum w |
+//             6 sum V w -> PCG rhs | 7 sum V FP w -> Jacobi diagonal
+//  PCG        x, r, z, s, q float[cap_tp*64] page-major (the reference's d_grid_*)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/mpmx.h"
+
+namespace mpmx {
+
+constexpr int kGridCh = 8;       // channels per page in the grid slab
+constexpr int kPageFloats = kGridCh * 64;
+constexpr int kTile = 6;         // 6^3 node tile of one page's particles
+constexpr int kTileN = kTile * kTile * kTile;
+
+enum ErrBits : unsigned {
+    kErrDomain = 1u,
+    kErrNonFinite = 2u,
+    kErrCapacity = 4u,
+    kErrBreakdown = 8u,
+};
+
+// per-scene constants, passed by value to every kernel
+struct DevParams {
+    int N;
+    int transfer;
+    int phase_field;
+    int dot_mode;
+    int wall;
+    int grip_lo, grip_hi;
+    int vpage;
+    float dx, inv_dx, Dinv;
+    float grip_v;
+    float lambda, mu, kappa, V0;
+    float Cinv[9];
+    float yield, hard, alpha_c, dlambda_cap, eps_cap;
+    float Gc0, residual_c, ps, theta, M;
+    float gravity[3];
+    float vmax, cg_tol, crit_stretch;
+    float L0;
+    float sq23f;   // sqrt(2/3) as float
+    float svd_gap; // mixed precision FP64 SVD threshold
+};
+
+// device-resident counters and PCG scalars
+struct DevCounts {
+    int num_page;
+    int num_total;
+    int num_vp;
+    int num_supp;
+    unsigned err;
+    int cg_done;
+    int cg_iters;
+    int cg_breakdown;
+    unsigned sq_key;    // ordered-float max accumulators
+    unsigned rho_key;
+    unsigned norm_key;
+    float rho_old;
+    float sq_sum;       // true-dot accumulators
+    float rho_sum;
+    int num_cell;
+    int svd_fallbacks;
+    int pad[16];
+};
+
+struct PState {
+    float* x = nullptr;
+    float* v = nullptr;
+    float* B = nullptr;
+    float* F = nullptr;
+    float* c = nullptr;
+    float* alpha = nullptr;
+    float* psi = nullptr;
+    float* g = nullptr;
+    float* mass = nullptr;
+    int* id = nullptr;
+};
+
+// mixed-precision particle storage: x float AoS, F as half(F-I), the rest bf16
+struct MState {
+    float* x = nullptr;
+    uint16_t* v = nullptr;   // bf16[3n] AoS
+    uint16_t* B = nullptr;   // bf16[9n]
+    uint16_t* F = nullptr;   // fp16[9n] of F - I
+    uint16_t* c = nullptr;   // bf16
+    uint16_t* alpha = nullptr;
+    uint16_t* psi = nullptr;
+    uint16_t* g = nullptr;
+    float* mass = nullptr;
+    int* id = nullptr;
+};
+
+struct TimerEntry {
+    std::string name;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+
+}  // namespace mpmx
+
+struct mpmx_ctx {
+    mpmx_params P;
+    mpmx_config cfg;
+    mpmx::DevParams dp;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n = 0, cap_n = 0;
+    int cur = 0;
+    mpmx::PState ps[2];
+    mpmx::MState ms[2];
+    float* sigma = nullptr;
+    float* FP = nullptr;
+    float* vol = nullptr;
+    // sort
+    uint32_t* keys = nullptr;
+    uint32_t* keys_alt = nullptr;
+    uint32_t* vals = nullptr;
+    uint32_t* vals_alt = nullptr;
+    uint32_t* hist = nullptr;
+    int sort_tiles = 0;
+    int key_bits = 0;
+    bool sorted = false;     // particles gathered to the order of `keys`
+    // pages
+    int axis_bits = 0;       // bits per axis of the Morton page space
+    int64_t mask_words = 0;
+    uint64_t* pmask = nullptr;
+    uint64_t* nmask = nullptr;
+    int* wcount = nullptr;   // per-word popcounts, then their exclusive scan
+    int* scount = nullptr;
+    uint32_t* page_key = nullptr;
+    int* page2particle = nullptr;
+    int* adj = nullptr;
+    int* nvp = nullptr;
+    int* vpo = nullptr;
+    int* target = nullptr;
+    int64_t cap_p = 0, cap_tp = 0, cap_vp = 0;
+    // grid + PCG
+    float* grid = nullptr;
+    float* grid_p2g = nullptr;   // copy of the P2G result (diagnostics only)
+    bool keep_p2g = false;
+    float* cg_x = nullptr;
+    float* cg_r = nullptr;
+    float* cg_z = nullptr;
+    float* cg_s = nullptr;
+    float* cg_q = nullptr;
+    // counters
+    mpmx::DevCounts* dc = nullptr;
+    mpmx::DevCounts* hc = nullptr;   // pinned host mirror
+    // scans
+    int* scan_tmp = nullptr;
+    int64_t scan_tmp_cap = 0;
+    // misc
+    int num_cu = 256;
+    int substeps = 0;
+    int last_cg_iters = 0;
+    bool timing = false;
+    std::vector<mpmx::TimerEntry> timers;
+    std::string last_error;
+};
+
+namespace mpmx {
+
+// ---- device helpers -------------------------------------------------------
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t compact3(uint32_t v) {
+    v &= 0x09249249u;
+    v = (v ^ (v >> 2)) & 0x030C30C3u;
+    v = (v ^ (v >> 4)) & 0x0300F00Fu;
+    v = (v ^ (v >> 8)) & 0x030000FFu;
+    v = (v ^ (v >> 16)) & 0x000003FFu;
+    return v;
+}
+// Morton page key with the SPGrid bit order: z at bit 0, y at bit 1, x at bit 2
+// of every triple (SPGridMask.cpp:96-100)
+__device__ __forceinline__ uint32_t morton(uint32_t x, uint32_t y, uint32_t z) {
+    return spread3(z) | (spread3(y) << 1) | (spread3(x) << 2);
+}
+__device__ __forceinline__ void unmorton(uint32_t k, int& x, int& y, int& z) {
+    z = (int)compact3(k);
+    y = (int)compact3(k >> 1);
+    x = (int)compact3(k >> 2);
+}
+
+// base node floor(x*N + 1/2) - 1 evaluated exactly (the reference's
+// (int)(x * one_over_dx + 0.5f) - 1, Particle_kernal.cu:162), clamped to the
+// valid range so that a particle that left the domain still yields in-range
+// indices (the error itself is latched by the caller)
+__device__ __forceinline__ int base_node(float x, int N) {
+    double t = (double)x * (double)N + 0.5;
+    if (!(t == t)) t = 0.0;
+    int b = (int)floor(t) - 1;
+    b = b < 0 ? 0 : b;
+    b = b > N - 3 ? N - 3 : b;
+    return b;
+}
+__device__ __forceinline__ uint32_t key_of(int bi, int bj, int bk) {
+    return (morton((uint32_t)bi >> 2, (uint32_t)bj >> 2, (uint32_t)bk >> 2) << 6) |
+           (uint32_t)(((bi & 3) << 4) | ((bj & 3) << 2) | (bk & 3));
+}
+
+// quadratic B-spline weights of the 3 nodes base+0..2, d0 = x/dx - base in [0.5,1.5)
+// (P2GKernal.cu:74-82), and their derivatives in units of 1/dx
+__device__ __forceinline__ void bspline(float d0, float (&w)[3]) {
+    float z = 1.5f - d0;
+    w[0] = 0.5f * z * z;
+    const float d1 = d0 - 1.0f;
+    w[1] = 0.75f - d1 * d1;
+    z = 1.5f - (1.0f - d1);
+    w[2] = 0.5f * z * z;
+}
+__device__ __forceinline__ void bspline_grad(float d0, float (&w)[3], float (&g)[3]) {
+    float z = 1.5f - d0;
+    w[0] = 0.5f * z * z;
+    g[0] = -z;
+    const float d1 = d0 - 1.0f;
+    w[1] = 0.75f - d1 * d1;
+    g[1] = -d1 * 2.0f;
+    z = 1.5f - (1.0f - d1);
+    w[2] = 0.5f * z * z;
+    g[2] = z;
+}
+
+// order-preserving float <-> uint keys for atomicMax over signed floats
+__device__ __forceinline__ unsigned fkey(float f) {
+    const unsigned b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__host__ __device__ __forceinline__ float fkey_dec(unsigned k) {
+    const unsigned b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+#ifdef __HIP_DEVICE_COMPILE__
+    return __uint_as_float(b);
+#else
+    float f;
+    __builtin_memcpy(&f, &b, 4);
+    return f;
+#endif
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs.  Remap the hardware
+// block index so that consecutive logical blocks (neighbouring pages in Morton
+// order, which share grid pages and thus atomics) run on the same XCD/L2.
+__device__ __forceinline__ int xcd_logical_block() {
+    const int G = gridDim.x, b = blockIdx.x;
+    if (G % 8) return b;
+    return (b % 8) * (G / 8) + b / 8;
+}
+// contiguous [begin, end) chunk of `count` items for the calling logical block
+__device__ __forceinline__ void block_range(int count, int& begin, int& end) {
+    const int lb = xcd_logical_block();
+    const int G = gridDim.x;
+    const int per = (count + G - 1) / G;
+    begin = lb * per;
+    end = begin + per < count ? begin + per : count;
+}
+
+}  // namespace mpmx
+
+// ---- host-side launch helpers (defined in the .hip files) -------------------
+namespace mpmx {
+hipError_t scan_exclusive(mpmx_ctx* c, const int* in, int* out, int64_t n_cap, const int* n_dev,
+                          int64_t n_host, int* total_dev);
+hipError_t sort_and_gather(mpmx_ctx* c);
+hipError_t compute_keys(mpmx_ctx* c);
+hipError_t build_tables(mpmx_ctx* c);
+hipError_t launch_stress(mpmx_ctx* c);
+hipError_t launch_p2g(mpmx_ctx* c, float dt);
+hipError_t launch_grid_update(mpmx_ctx* c, float dt);
+hipError_t launch_pcg(mpmx_ctx* c, float dt);
+hipError_t launch_g2p(mpmx_ctx* c, float dt);
+hipError_t fill_zero(mpmx_ctx* c, void* p, int64_t bytes);
+hipError_t fill_zero_dev_count(mpmx_ctx* c, float* p, const int* count_dev, int64_t per_unit,
+                               int64_t cap_units);
+void timer_push(mpmx_ctx* c, const char* name, bool begin);
+int transfer_grid_blocks(mpmx_ctx* c);
+}  // namespace mpmx
+
+#define MPMX_TIMED(ctx, name, stmt)          \
+    do {                                     \
+        mpmx::timer_push(ctx, name, true);   \
+        stmt;                                \
+        mpmx::timer_push(ctx, name, false);  \
+    } while (0)

@@ -1,0 +1,186 @@
+// scene.cpp — seeded synthetic scenes (host only).
+//
+// The reference builds its scenes from an SDF of two dragons or an OBJ point
+// list with rand() rejection sampling (Simulator.cpp:145-267,
+// LoadSample.cpp:109-214); those assets are not part of the checkout.  These
+// generators produce the benchmark configurations from a SplitMix64 stream:
+//   0  cube: 20^3 cells at [54,74)^3 of a 128^3 grid, 8 particles per cell
+//   1  notched bar: 1,048,576 uniform particles in a 192x96x96-cell bar of a
+//      256^3 grid, V-notch at mid-length 1/4 of the height deep, grips 8 cells
+//      from each end pulled at +-0.5 m/s, psi+ gated at stretch 1.05
+//   2  notched bar, 8 particles per cell, 128x32x32 cells (1,048,576 particles)
+//   3  two-block impact: two 80^3-cell blocks x 8 ppc in a 512^3 grid, 8-cell
+//      gap along z, v_z = +-2.5 m/s
+//   4  sphere of radius 16 cells at the centre of a 128^3 grid, 8 ppc
+// Lattice scenes place 2x2x2 particles per cell at (0.25, 0.75) dx with a
+// uniform jitter of +-0.125 dx.  `scale` shrinks the geometry (lattice
+// scenes) or the particle count (scene 1).  Density follows
+// rho = max(1000, (lambda + 2 mu) (dt / (0.3 dx))^2) so the elastic wave
+// crosses at most 0.3 cells per substep.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/mpmx.h"
+
+namespace {
+
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    }
+    double u01() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+struct Scene {
+    mpmx_params p;
+    double dt;
+    std::vector<double> x, v;
+    double V0;
+};
+
+double density_rule(const mpmx_params& p, double dt) {
+    const double lambda = p.E * p.nu / ((1 + p.nu) * (1 - 2 * p.nu));
+    const double mu = p.E / (2 * (1 + p.nu));
+    const double dx = 1.0 / p.N;
+    const double r = dt / (0.3 * dx);
+    const double rho = (lambda + 2 * mu) * r * r;
+    return rho > 1000.0 ? rho : 1000.0;
+}
+
+// 8 jittered particles per cell for every cell accepted by `inside`
+template <typename F>
+void lattice(Scene& s, Rng& rng, int i0, int i1, int j0, int j1, int k0, int k1, F inside,
+             double vx = 0, double vy = 0, double vz = 0) {
+    const double dx = 1.0 / s.p.N;
+    for (int i = i0; i < i1; ++i)
+        for (int j = j0; j < j1; ++j)
+            for (int k = k0; k < k1; ++k)
+                for (int q = 0; q < 8; ++q) {
+                    double pos[3];
+                    const int cell[3] = {i, j, k};
+                    for (int a = 0; a < 3; ++a) {
+                        const double sub = ((q >> (2 - a)) & 1) ? 0.75 : 0.25;
+                        pos[a] = (cell[a] + sub + (rng.u01() - 0.5) * 0.25) * dx;
+                    }
+                    if (!inside(pos)) continue;
+                    s.x.insert(s.x.end(), pos, pos + 3);
+                    s.v.push_back(vx);
+                    s.v.push_back(vy);
+                    s.v.push_back(vz);
+                }
+}
+
+bool build(int scene, uint64_t seed, double scale, Scene& s) {
+    Rng rng{seed * 0x2545F4914F6CDD1Dull + 0x1234567ull};
+    mpmx_default_params(&s.p);
+    s.p.E = 1e5;
+    s.p.nu = 0.3;
+    s.p.yield_stress = 1e3;
+    s.dt = 1e-4;
+    if (scale <= 0) scale = 1;
+    auto all = [](const double*) { return true; };
+    switch (scene) {
+    case 0: {
+        s.p.N = 128;
+        int e = (int)std::lround(20 * scale);
+        if (e < 1) e = 1;
+        const int a = 64 - e / 2;   // [54,74) for e = 20
+        lattice(s, rng, a, a + e, a, a + e, a, a + e, all);
+        s.V0 = std::pow(1.0 / s.p.N, 3) / 8;
+        break;
+    }
+    case 1: {
+        s.p.N = 256;
+        const double x0 = 0.125, x1 = 0.875, y0 = 0.3125, y1 = 0.6875, z0 = 0.3125, z1 = 0.6875;
+        const double depth = 0.25 * (y1 - y0);
+        auto in_notch = [&](double x, double y) { return y > y1 - (depth - std::fabs(x - 0.5)); };
+        int64_t n = (int64_t)std::llround(1048576.0 * scale);
+        if (n < 1) n = 1;
+        s.x.reserve(3 * n);
+        while ((int64_t)s.x.size() < 3 * n) {
+            const double px = x0 + (x1 - x0) * rng.u01();
+            const double py = y0 + (y1 - y0) * rng.u01();
+            const double pz = z0 + (z1 - z0) * rng.u01();
+            if (in_notch(px, py)) continue;
+            s.x.push_back(px);
+            s.x.push_back(py);
+            s.x.push_back(pz);
+        }
+        s.v.assign(s.x.size(), 0.0);
+        const double vol = (x1 - x0) * (y1 - y0) * (z1 - z0) - depth * depth * (z1 - z0);
+        s.V0 = vol / (double)n;
+        s.p.grip_lo = 32 + 8;
+        s.p.grip_hi = 224 - 8;
+        s.p.grip_v = 0.5;
+        s.p.crit_stretch = 1.05;
+        break;
+    }
+    case 2: {
+        s.p.N = 256;
+        int L = (int)std::lround(128 * scale), H = (int)std::lround(32 * scale);
+        if (L < 8) L = 8;
+        if (H < 4) H = 4;
+        const int i0 = 128 - L / 2, j0 = 128 - H / 2, k0 = 128 - H / 2;
+        const double dx = 1.0 / s.p.N;
+        const double ytop = (j0 + H) * dx, depth = 0.25 * H * dx, xm = 0.5;
+        auto outside_notch = [&](const double* q) { return !(q[1] > ytop - (depth - std::fabs(q[0] - xm))); };
+        lattice(s, rng, i0, i0 + L, j0, j0 + H, k0, k0 + H, outside_notch);
+        s.V0 = std::pow(dx, 3) / 8;
+        const int grip = L >= 32 ? 8 : L / 4;
+        s.p.grip_lo = i0 + grip;
+        s.p.grip_hi = i0 + L - grip;
+        s.p.grip_v = 0.5;
+        s.p.crit_stretch = 1.05;
+        break;
+    }
+    case 3: {
+        s.p.N = 512;
+        int b = (int)std::lround(80 * scale);
+        if (b < 1) b = 1;
+        const int c0 = 256 - b / 2;
+        lattice(s, rng, c0, c0 + b, c0, c0 + b, 256 - 4 - b, 256 - 4, all, 0, 0, 2.5);
+        lattice(s, rng, c0, c0 + b, c0, c0 + b, 256 + 4, 256 + 4 + b, all, 0, 0, -2.5);
+        s.V0 = std::pow(1.0 / s.p.N, 3) / 8;
+        break;
+    }
+    case 4: {
+        s.p.N = 128;
+        const double r = 16 * scale / s.p.N;
+        auto ball = [&](const double* q) {
+            const double dx0 = q[0] - 0.5, dy = q[1] - 0.5, dz = q[2] - 0.5;
+            return dx0 * dx0 + dy * dy + dz * dz <= r * r;
+        };
+        const int lo = 64 - (int)std::ceil(16 * scale) - 1, hi = 64 + (int)std::ceil(16 * scale) + 1;
+        lattice(s, rng, lo, hi, lo, hi, lo, hi, ball);
+        s.V0 = std::pow(1.0 / s.p.N, 3) / 8;
+        break;
+    }
+    default:
+        return false;
+    }
+    s.p.V0 = s.V0;
+    return true;
+}
+
+}  // namespace
+
+extern "C" int64_t mpmx_scene(int scene, uint64_t seed, double scale, mpmx_params* p, double* x, double* v,
+                              double* mass, double* dt) {
+    Scene s;
+    if (!build(scene, seed, scale, s)) return -1;
+    const int64_t n = (int64_t)(s.x.size() / 3);
+    const double rho = density_rule(s.p, s.dt);
+    if (p) *p = s.p;
+    if (dt) *dt = s.dt;
+    if (x) std::memcpy(x, s.x.data(), sizeof(double) * 3 * n);
+    if (v) std::memcpy(v, s.v.data(), sizeof(double) * 3 * n);
+    if (mass)
+        for (int64_t i = 0; i < n; ++i) mass[i] = rho * s.V0;
+    return n;
+}

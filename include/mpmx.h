@@ -1,0 +1,228 @@
+/*
+ * mpmx.h — C ABI of libmpmx_hip.so, the MI355X (gfx950) implementation of the
+ * explicit MLS-MPM substep with von Mises plasticity and plastic phase-field
+ * fracture.
+ *
+ * The reference (KemengHuang/Explicit-GPUMPM-Ductile-Fracture) has no C ABI:
+ * its boundary is the C++ classes MPMSimulator, MPMTimeIntegrator /
+ * ExplicitTimeIntegrator, Particles, SPGrid and DomainTransformer with public
+ * device pointers.  Each entry point below names the reference method it
+ * replaces (paths relative to the reference's GPU_MPM/ directory); the C++
+ * classes of the same names in this repository are thin wrappers over these
+ * calls.
+ *
+ * Conventions
+ *  - POD structs only; no C++ exception crosses the ABI; every call returns an
+ *    mpmx_status.  mpmx_last_error() gives a message for the last failure.
+ *  - The context owns all device memory and one HIP stream.  Host buffers are
+ *    caller-owned and copied.
+ *  - Stage calls are asynchronous on the context stream; mpmx_get_*,
+ *    mpmx_sync and mpmx_destroy synchronise.  Device-side errors (particle left
+ *    the domain, non-finite state, table capacity exceeded, CG breakdown) are
+ *    latched in a device error word and reported at the next synchronising call.
+ *  - One context per host thread.
+ *  - Particle fields are returned in the caller's original particle order.
+ *    3x3 matrices are column-major per particle: M(r,c) = m[r + 3c].
+ */
+#ifndef MPMX_H
+#define MPMX_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mpmx_status {
+    MPMX_OK = 0,
+    MPMX_ERR_HIP = 1,            /* HIP runtime call failed                       */
+    MPMX_ERR_ARG = 2,            /* bad argument / call order                     */
+    MPMX_ERR_CAPACITY = 3,       /* page or virtual-page table capacity exceeded  */
+    MPMX_ERR_OUT_OF_DOMAIN = 4,  /* a particle left [2dx, 1-2dx]                  */
+    MPMX_ERR_NONFINITE = 5,      /* NaN/Inf in particle state                     */
+    MPMX_ERR_CG_BREAKDOWN = 6,   /* phase-field PCG hit rho_old == 0              */
+    MPMX_ERR_NO_DEVICE = 7       /* no usable HIP device                          */
+} mpmx_status;
+
+/* Scene constants.  The reference hard-codes these as compile-time constants
+ * and kernel literals (Setting.h:3-62, ExplicitTimeIntegrator.cu:188-335,
+ * Materials.cpp:4-14, TimeIntegrator.cu:100-165, G2PKernal.cu:134-142); here
+ * they are per-scene runtime values.  mpmx_default_params() fills the
+ * reference values. */
+typedef struct mpmx_params {
+    int    N;            /* cells per axis, dx = 1/N (Setting.h:13)               */
+    int    transfer;     /* 0 = MLS (P2GKernal.cu:6), 1 = APIC (P2GKernal.cu:512)  */
+    int    phase_field;  /* 1 = solve the damage field every substep              */
+    int    dot_mode;     /* 0 = reference "dot" = max over 128-entry block sums,
+                            1 = true inner product (ExplicitTimeIntegrator.cu:573-609) */
+    int    cg_max_iter;  /* 100 */
+    int    vpage;        /* particles per virtual page (Transformer_kernal.cu:119): 64, 128 or 256 */
+    int    wall_cells;   /* sticky wall width in cells, 4 */
+    int    grip_lo;      /* grid nodes with i <= grip_lo move at (-grip_v,0,0); < 0 disables */
+    int    grip_hi;      /* grid nodes with i >= grip_hi move at (+grip_v,0,0) */
+    double grip_v;
+    double E, nu;        /* Young's modulus, Poisson ratio */
+    double V0;           /* particle rest volume */
+    double yield_stress, hardening, alpha_c, dlambda_cap, eps_cap;
+    double Gc0, residual_c, ps, theta, M;
+    double gravity[3];
+    double vmax;         /* MLS grid-to-particle speed clamp */
+    double cg_tol;
+    double crit_stretch; /* psi+ history grows only while max principal stretch >= this (0: always) */
+} mpmx_params;
+
+typedef enum mpmx_precision {
+    MPMX_FP32 = 0,       /* FP32 particle state and grid                          */
+    MPMX_MIXED = 1       /* x FP32, F as FP16(F-I), v/C/c/alpha/psi/g BF16, FP32 grid,
+                            FP64 SVD when the singular-value gap < svd_gap         */
+} mpmx_precision;
+
+typedef struct mpmx_config {
+    int64_t max_particles;  /* capacity; 0 = size of the first mpmx_set_particles */
+    int64_t max_pages;      /* total (particle + neighbour) page capacity; 0 = auto */
+    int     device;         /* HIP device ordinal */
+    int     precision;      /* mpmx_precision */
+    double  svd_gap;        /* mixed precision: FP64 SVD when min singular-value gap < this (1e-4) */
+    int     cg_check_every; /* host reads the PCG convergence flag every this many iterations (1) */
+    int     p2g_variant;    /* 0 = auto (autotuned), 1 = particle-per-lane LDS atomics,
+                               2 = cell-gather (lanes own (cell, node column) pairs) */
+    int     block;          /* threads per transfer block; 0 = auto (= vpage) */
+} mpmx_config;
+
+typedef struct mpmx_ctx mpmx_ctx;
+
+typedef struct mpmx_stats {
+    int64_t num_particles;
+    int     num_page;          /* particle pages             (Transformer.cuh:39 _numPage)        */
+    int     num_total_page;    /* + neighbour-only pages     (_numTotalPage)                      */
+    int     num_virtual_page;  /* (_numVirtualPage)                                             */
+    int     cg_iters;          /* PCG iterations of the last substep                             */
+    int     cg_breakdown;
+    int     substeps;          /* substeps taken by this context                                 */
+    double  cg_norm;           /* final max|r| of the last PCG                                   */
+    double  mass;              /* FP64 sums over particles: mass and momentum                    */
+    double  momentum[3];
+    double  kinetic_energy;
+} mpmx_stats;
+
+/* per-kernel average durations of the last timed region (mpmx_timing_begin/end) */
+typedef struct mpmx_kernel_time {
+    char   name[32];
+    int    launches;
+    double total_ms;
+} mpmx_kernel_time;
+
+void        mpmx_default_params(mpmx_params* p);
+void        mpmx_default_config(mpmx_config* c);
+const char* mpmx_status_string(mpmx_status s);
+const char* mpmx_last_error(const mpmx_ctx* ctx);
+int         mpmx_device_count(void);
+
+/* MPMSimulator::MallocDeviceMEM + Particles::Particles_Malloc_MEM +
+ * SPGrid::SPGrid_MEM_Malloc + DomainTransformer::Transformer_Malloc_MEM
+ * (Simulator.cpp:85-100, Particles.cpp:54-94, SPGrid.cpp:10-55, Transformer.cpp:59-89) */
+mpmx_status mpmx_create(const mpmx_params* p, const mpmx_config* cfg, mpmx_ctx** out);
+/* the destructors (Simulator.cpp:102-111, Particles.cpp:96-125, SPGrid.cpp:63-79) */
+mpmx_status mpmx_destroy(mpmx_ctx* ctx);
+
+/* Particles::initialize / initialize_kernal (Particles.cpp:8-31, Particle_kernal.cu:441-481)
+ * and MPMSimulator::buildIntegrator's F = I, B = 0 (Simulator.cpp:136-142).
+ * x, v: n*3 doubles (xyz per particle); mass: n doubles.  Keys and the first
+ * canonical sort are computed here. */
+mpmx_status mpmx_set_particles(mpmx_ctx* ctx, int64_t n, const double* x, const double* v,
+                               const double* mass);
+
+/* ---- stages of one substep, in order (ExplicitTimeIntegrator.cu:881-995) ---- */
+/* Particles::reorder / sort_by_offsets (Particle_kernal.cu:484-573): canonical
+ * stable sort by cell key, then every particle field is gathered to the new order */
+mpmx_status mpmx_bin_sort(mpmx_ctx* ctx);
+/* DomainTransformer::rebuild (Transformer.cpp:125-136) */
+mpmx_status mpmx_rebuild(mpmx_ctx* ctx);
+/* ExplicitTimeIntegrator::computeForceCoefficient -> applyVonMises
+ * (ExplicitTimeIntegrator.cu:846-865, 132-370) */
+mpmx_status mpmx_stress(mpmx_ctx* ctx);
+/* MPMTimeIntegrator::transferP2G (TimeIntegrator.cu:299-372) */
+mpmx_status mpmx_p2g(mpmx_ctx* ctx, double dt);
+/* MPMTimeIntegrator::undateGrid + resolveCollision (TimeIntegrator.cu:378-403),
+ * also forms the phase-field right-hand side and Jacobi diagonal */
+mpmx_status mpmx_grid_update(mpmx_ctx* ctx, double dt);
+/* the damage solve of ExplicitTimeIntegrator::integrate (ExplicitTimeIntegrator.cu:909-968) */
+mpmx_status mpmx_phase_field(mpmx_ctx* ctx, double dt);
+/* MPMTimeIntegrator::transferG2P (TimeIntegrator.cu:411-474) */
+mpmx_status mpmx_g2p(mpmx_ctx* ctx, double dt);
+
+/* ExplicitTimeIntegrator::integrate (ExplicitTimeIntegrator.cu:881-995):
+ * bin_sort, rebuild, stress, p2g, grid_update, phase_field, g2p */
+mpmx_status mpmx_substep(mpmx_ctx* ctx, double dt);
+/* nsteps substeps with fixed dt (no host synchronisation in between unless the
+ * PCG needs a convergence check) */
+mpmx_status mpmx_run(mpmx_ctx* ctx, int nsteps, double dt);
+/* MPMSimulator::computeDt (Simulator.cpp:269-294): min(dt_default, cfl*dx/max|v|) */
+mpmx_status mpmx_compute_dt(mpmx_ctx* ctx, double dt_default, double cfl, double* dt_out);
+
+mpmx_status mpmx_sync(mpmx_ctx* ctx);
+
+/* field ids for mpmx_get_field / mpmx_set_field */
+typedef enum mpmx_field {
+    MPMX_FIELD_X = 0,       /* 3 per particle */
+    MPMX_FIELD_V = 1,       /* 3 */
+    MPMX_FIELD_F = 2,       /* 9, column-major */
+    MPMX_FIELD_B = 3,       /* 9, the affine matrix C (MLS/APIC B) */
+    MPMX_FIELD_C = 4,       /* damage phase c (1 = intact) */
+    MPMX_FIELD_ALPHA = 5,   /* equivalent plastic strain */
+    MPMX_FIELD_PSI = 6,     /* psi+ history maximum */
+    MPMX_FIELD_G = 7,       /* degradation g(c) */
+    MPMX_FIELD_MASS = 8,
+    MPMX_FIELD_FP = 9,      /* phase-field coefficient of the last stress update */
+    MPMX_FIELD_VOL = 10,    /* V0 det F of the last stress update */
+    MPMX_FIELD_SIGMA = 11   /* 9: V0 P F^T of the last stress update */
+} mpmx_field;
+/* Particles::Attribs_Back_Host (Particles.cpp:33-52): copies n*width doubles in
+ * original particle order */
+mpmx_status mpmx_get_field(mpmx_ctx* ctx, int field, double* host, size_t count);
+mpmx_status mpmx_set_field(mpmx_ctx* ctx, int field, const double* host, size_t count);
+
+/* current canonical order: order[i] = original id of the particle in sorted
+ * slot i; keys[i] = its 64-bit SPGrid offset (Particles::d_indices, cell_id) */
+mpmx_status mpmx_get_order(mpmx_ctx* ctx, int32_t* order, uint64_t* keys);
+
+/* DomainTransformer tables of the last rebuild (Transformer.cuh:24-50):
+ * page_key[num_total_page] (= d_pageOffset >> 12), page2particle[num_page+1],
+ * adj[7*num_page] (neighbour j of page p at adj[j*num_page+p]),
+ * vpo[num_page+1], target[num_virtual_page] (1-based, as d_targetPage),
+ * particle2cell[n] (1-based), cell2particle[num_cell+1]; counts[4] = numPage,
+ * numTotalPage, numVirtualPage, numCell.  Any pointer may be NULL. */
+mpmx_status mpmx_get_tables(mpmx_ctx* ctx, uint64_t* page_key, int32_t* page2particle, int32_t* adj,
+                            int32_t* vpo, int32_t* target, int32_t* particle2cell,
+                            int32_t* cell2particle, int32_t counts[4]);
+
+/* grid channel after the last grid update, page-major [num_total_page*64]:
+ * ch 0 mass, 1-3 velocity, 4 damage c (or its increment after the PCG),
+ * 5 sum of weights, 6 phase-field rhs, 7 Jacobi diagonal; 8 = PCG solution x.
+ * SPGrid::d_channels (SPGrid.cpp:34-49) with channels renumbered. */
+mpmx_status mpmx_get_grid(mpmx_ctx* ctx, int channel, float* host, size_t count);
+/* the grid channels right after P2G (before the update): 0 m, 1-3 momentum,
+ * 4 sum w c, 5 sum w, 6 sum V w, 7 sum V FP w */
+mpmx_status mpmx_get_grid_p2g(mpmx_ctx* ctx, int channel, float* host, size_t count);
+
+mpmx_status mpmx_get_stats(mpmx_ctx* ctx, mpmx_stats* out);
+
+/* kernel timing with HIP events on the context stream */
+mpmx_status mpmx_timing_begin(mpmx_ctx* ctx);
+mpmx_status mpmx_timing_end(mpmx_ctx* ctx, mpmx_kernel_time* out, int max_entries, int* n_entries);
+/* elapsed milliseconds of `nsteps` substeps measured with events on the
+ * context stream (inputs already resident) */
+mpmx_status mpmx_time_run(mpmx_ctx* ctx, int nsteps, double dt, double* ms);
+
+/* ---- synthetic scenes (host only, no GPU) ----
+ * scene ids: 0 cube (20^3 cells at [54,74)^3 of 128^3, 8 ppc jittered),
+ * 1 notched bar (1,048,576 uniform particles, 256^3), 2 notched bar 8 ppc,
+ * 3 two-block impact (2 x 80^3 cells x 8 ppc, 512^3), 4 sphere, 5 scaled cube.
+ * Query the particle count with x == NULL; returns particle count or -1. */
+int64_t mpmx_scene(int scene, uint64_t seed, double scale, mpmx_params* p, double* x, double* v,
+                   double* mass, double* dt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

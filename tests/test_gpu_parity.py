@@ -1,0 +1,156 @@
+"""GPU parity against the serial FP64 oracle, through the C ABI.
+
+Bars (stated per test):
+  - integer work (cell keys, canonical sort order, page tables): bit-exact;
+  - per-particle constitutive update on identical inputs: FP32 vs FP64,
+    relative to the field's scale;
+  - P2G grid sums on identical inputs: FP32 atomics vs FP64 serial sums;
+  - multi-substep trajectories: per-field tolerances from FP32 rounding growth.
+"""
+import numpy as np
+import pytest
+
+from conftest import max_rel_field, oracle_params
+
+pytestmark = pytest.mark.gpu
+
+
+def make(mpmx, scene_id, scale, seed=1, **over):
+    p, x, v, m, dt = mpmx.scene(scene_id, seed=seed, scale=scale)
+    for k, val in over.items():
+        setattr(p, k, val)
+    ctx = mpmx.Context(p)
+    ctx.set_particles(x, v, m)
+    return ctx, p, x, v, m, dt
+
+
+def oracle_tables_check(oracle, ctx, p, prev_order, x32):
+    """Keys of the GPU's own float32 positions, stably sorted from prev_order,
+    must give exactly the GPU's order, keys and tables."""
+    keys_prev = oracle.cell_keys(x32[prev_order], p.N)
+    perm, sk = oracle.sort_keys(keys_prev)
+    order, keys = ctx.order()
+    assert np.array_equal(order, prev_order[perm])
+    assert np.array_equal(keys, sk)
+    ot = oracle.page_tables(sk, p.vpage)
+    gt = ctx.tables()
+    for k in ("numPage", "numTotalPage", "numVirtualPage", "numCell"):
+        assert gt[k] == ot[k], k
+    for k in ("page_key", "page2particle", "adj", "vpo", "target", "particle2cell", "cell2particle"):
+        assert np.array_equal(np.asarray(gt[k]).astype(np.int64), np.asarray(ot[k]).astype(np.int64)), k
+
+
+@pytest.mark.parametrize("scene_id,scale", [(0, 0.5), (1, 0.02), (3, 0.1)])
+def test_keys_sort_tables_bitexact_initial(mpmx, oracle, scene_id, scale):
+    ctx, p, x, v, m, dt = make(mpmx, scene_id, scale)
+    ident = np.arange(ctx.n, dtype=np.int32)
+    oracle_tables_check(oracle, ctx, p, ident, x.astype(np.float32))
+
+
+@pytest.mark.parametrize("vpage", [64, 128, 256])
+def test_keys_sort_tables_bitexact_after_steps(mpmx, oracle, vpage):
+    ctx, p, x, v, m, dt = make(mpmx, 3, 0.1, vpage=vpage)
+    for _ in range(3):
+        prev, _k = ctx.order()
+        ctx.substep(dt)
+        ctx.bin_sort()
+        ctx.rebuild()
+        x32 = ctx.get("x").astype(np.float32)
+        oracle_tables_check(oracle, ctx, p, prev, x32)
+
+
+def test_stress_parity(mpmx, oracle):
+    ctx, p, x, v, m, dt = make(mpmx, 0, 0.5)
+    n = ctx.n
+    rng = np.random.default_rng(7)
+    F = np.eye(3)[None] + 0.04 * rng.standard_normal((n, 3, 3))
+    F[: n // 2] = np.eye(3)[None] + 0.3 * rng.standard_normal((n // 2, 3, 3))  # plastic half
+    F = np.where(np.linalg.det(F)[:, None, None] > 0.2, F, np.eye(3)[None])
+    c = rng.uniform(0.3, 1.0, n)
+    g = rng.uniform(0.1, 1.0, n)
+    alpha = np.where(rng.random(n) < 0.5, 0.0, rng.uniform(0.0, 0.9, n))
+    psi = np.where(rng.random(n) < 0.5, 0.0, 1e6 - 1.0)
+    F32 = F.astype(np.float32).astype(np.float64)
+    ins = {"F": F32, "c": c.astype(np.float32), "g": g.astype(np.float32),
+           "alpha": alpha.astype(np.float32), "psi": psi.astype(np.float32)}
+    for k, a in ins.items():
+        ctx.set(k, a)
+    ctx.stress()
+    gpu = {k: ctx.get(k) for k in ("F", "sigma", "g", "alpha", "psi", "FP", "vol")}
+    op = oracle_params(oracle, p)
+    ref = {k: [] for k in gpu}
+    for i in range(n):
+        r = oracle.von_mises(op, ins["F"][i], float(ins["c"][i]), float(ins["g"][i]),
+                             float(ins["alpha"][i]), float(ins["psi"][i]))
+        for k in ref:
+            ref[k].append(r[k])
+    ref = {k: np.array(a) for k, a in ref.items()}
+    # FP32 SVD + log/exp/pow against FP64: scale-relative bound 2e-5
+    for k in ("F", "sigma", "FP", "vol", "g"):
+        assert max_rel_field(gpu[k], ref[k]) < 2e-5, k
+    assert np.abs(gpu["alpha"] - ref["alpha"]).max() < 2e-5
+    assert max_rel_field(gpu["psi"], ref["psi"]) < 2e-5
+
+
+def test_p2g_parity(mpmx, oracle):
+    ctx, p, x, v, m, dt = make(mpmx, 3, 0.1)
+    # a few substeps so that B, sigma, c are non-trivial
+    ctx.run(40, dt)
+    ctx.bin_sort()
+    ctx.enable_p2g_copy()
+    ctx.rebuild()
+    ctx.stress()
+    ctx.p2g(dt)
+    ctx.grid_update(dt)
+    order, keys = ctx.order()
+    tab = ctx.tables()
+    f = {k: ctx.get(k)[order] for k in ("x", "v", "B", "sigma", "mass", "c", "vol", "FP")}
+    ref = oracle.p2g_mls(oracle_params(oracle, p), keys, f["x"].astype(np.float32).astype(np.float64),
+                         f["v"], f["B"], f["sigma"], f["mass"], f["c"], f["vol"], f["FP"], dt,
+                         tab["page_key"])
+    for ch in range(8):
+        gpu = ctx.grid(ch, p2g=True)
+        # FP32 sums of up to 8 x 27 contributions against FP64: 1e-5 of the channel scale
+        assert max_rel_field(gpu, ref[ch]) < 1e-5, ch
+
+
+def trajectory(mpmx, oracle, scene_id, scale, steps, **over):
+    ctx, p, x, v, m, dt = make(mpmx, scene_id, scale, **over)
+    sim = oracle.Sim(oracle_params(oracle, p), x.astype(np.float32), v, m.astype(np.float32))
+    for _ in range(steps):
+        ctx.substep(dt)
+        sim.substep(dt)
+    ctx.sync()
+    return ctx, sim
+
+
+def test_trajectory_cube(mpmx, oracle):
+    ctx, sim = trajectory(mpmx, oracle, 0, 0.5, 20)
+    assert max_rel_field(ctx.get("x"), sim.get("x")) < 1e-6
+    assert max_rel_field(ctx.get("v"), sim.get("v")) < 1e-4
+    assert max_rel_field(ctx.get("F"), sim.get("F")) < 1e-5
+    assert np.abs(ctx.get("c") - sim.get("c")).max() < 1e-4
+
+
+def test_trajectory_impact(mpmx, oracle):
+    ctx, sim = trajectory(mpmx, oracle, 3, 0.1, 60)
+    st = ctx.stats()
+    assert st["cg_breakdown"] == 0
+    dx = ctx.get("x") - sim.get("x")
+    assert np.abs(dx).max() < 1e-4 / 512 * 512 * 1e-2  # 1e-6 m
+    for k, tol in (("v", 1e-3), ("F", 1e-4)):
+        assert max_rel_field(ctx.get(k), sim.get(k)) < tol, k
+    assert np.abs(ctx.get("c") - sim.get("c")).max() < 1e-3
+
+
+def test_conservation_free_flight(mpmx):
+    """No walls touched, gravity only: FP64 momentum changes by m g t exactly."""
+    ctx, p, x, v, m, dt = make(mpmx, 0, 0.5)
+    s0 = ctx.stats()
+    ctx.run(10, dt)
+    s1 = ctx.stats()
+    assert abs(s1["mass"] - s0["mass"]) <= 1e-12 * s0["mass"]
+    expect = s0["momentum"][1] + s0["mass"] * (-10.0) * 10 * dt
+    assert abs(s1["momentum"][1] - expect) < 1e-5 * abs(expect)
+    assert abs(s1["momentum"][0]) < 1e-6 * abs(expect)
+    assert abs(s1["momentum"][2]) < 1e-6 * abs(expect)
