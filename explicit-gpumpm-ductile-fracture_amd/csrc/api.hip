@@ -595,8 +595,8 @@ mpmx_status mpmx_sync(mpmx_ctx* c) {
     return check_err(c);
 }
 
-mpmx_status mpmx_compute_dt(mpmx_ctx* c, double dt_default, double cfl, double* dt_out) {
-    if (!c || !dt_out || c->n == 0) return MPMX_ERR_ARG;
+mpmx_status mpmx_max_velocity_sq(mpmx_ctx* c, double* out) {
+    if (!c || !out || c->n == 0) return MPMX_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     unsigned* d = nullptr;
     HIPCHK(c, dalloc(&d, 1));
@@ -610,8 +610,17 @@ mpmx_status mpmx_compute_dt(mpmx_ctx* c, double dt_default, double cfl, double* 
     dfree(d);
     float s2;
     std::memcpy(&s2, &h, 4);
+    *out = s2;
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_compute_dt(mpmx_ctx* c, double dt_default, double cfl, double* dt_out) {
+    if (!c || !dt_out || c->n == 0) return MPMX_ERR_ARG;
+    double s2 = 0;
+    mpmx_status st = mpmx_max_velocity_sq(c, &s2);
+    if (st != MPMX_OK) return st;
     double dt = dt_default;
-    const double vmax = std::sqrt((double)s2);
+    const double vmax = std::sqrt(s2);
     // Simulator.cpp:282
     if (vmax > 0 && cfl / c->P.N / vmax < dt_default) dt = cfl / c->P.N / vmax;
     *dt_out = dt;
@@ -930,6 +939,27 @@ mpmx_status mpmx_get_stats(mpmx_ctx* c, mpmx_stats* out) {
     out->cg_breakdown = c->hc->cg_breakdown;
     out->cg_norm = __builtin_bit_cast(float, c->hc->norm_key);
     return check_err(c);
+}
+
+mpmx_status mpmx_get_device_pointers(mpmx_ctx* c, mpmx_device_ptrs* o) {
+    if (!c || !o) return MPMX_ERR_ARG;
+    std::memset(o, 0, sizeof(*o));
+    const bool mixed = c->cfg.precision == MPMX_MIXED;
+    if (mixed) {
+        const MState& s = c->ms[c->cur];
+        o->pos = s.x; o->vel = s.v; o->F = s.F; o->B = s.B; o->phase_c = s.c; o->alpha = s.alpha;
+        o->psi = s.psi; o->g = s.g; o->mass = s.mass; o->ids = s.id;
+    } else {
+        const PState& s = c->ps[c->cur];
+        o->pos = s.x; o->vel = s.v; o->F = s.F; o->B = s.B; o->phase_c = s.c; o->alpha = s.alpha;
+        o->psi = s.psi; o->g = s.g; o->mass = s.mass; o->ids = s.id;
+    }
+    o->sigma = c->sigma; o->FP = c->FP; o->vol = c->vol; o->keys = c->keys;
+    o->grid = c->grid; o->pcg_x = c->cg_x; o->pcg_r = c->cg_r; o->pcg_z = c->cg_z; o->pcg_s = c->cg_s;
+    o->pcg_q = c->cg_q; o->page_key = c->page_key; o->page2particle = c->page2particle; o->adj = c->adj;
+    o->vpo = c->vpo; o->target = c->target; o->cap_pages = c->cap_p; o->cap_total_pages = c->cap_tp;
+    o->precision = c->cfg.precision;
+    return MPMX_OK;
 }
 
 mpmx_status mpmx_timing_begin(mpmx_ctx* c) {

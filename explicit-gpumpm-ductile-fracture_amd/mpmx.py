@@ -26,7 +26,8 @@ EXPORTS = [
     "mpmx_rebuild", "mpmx_stress", "mpmx_p2g", "mpmx_grid_update", "mpmx_phase_field", "mpmx_g2p",
     "mpmx_substep", "mpmx_run", "mpmx_compute_dt", "mpmx_sync", "mpmx_get_field", "mpmx_set_field",
     "mpmx_get_order", "mpmx_get_tables", "mpmx_get_grid", "mpmx_get_grid_p2g", "mpmx_get_stats",
-    "mpmx_timing_begin", "mpmx_timing_end", "mpmx_time_run", "mpmx_scene",
+    "mpmx_timing_begin", "mpmx_timing_end", "mpmx_time_run", "mpmx_scene", "mpmx_get_device_pointers",
+    "mpmx_max_velocity_sq",
 ]
 
 
@@ -126,6 +127,8 @@ def lib():
         "mpmx_time_run": ([vp, C.c_int, C.c_double, C.POINTER(C.c_double)], C.c_int),
         "mpmx_scene": ([C.c_int, C.c_uint64, C.c_double, C.POINTER(Params), vp, vp, vp,
                         C.POINTER(C.c_double)], C.c_int64),
+        "mpmx_get_device_pointers": ([vp, vp], C.c_int),
+        "mpmx_max_velocity_sq": ([vp, C.POINTER(C.c_double)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -157,6 +157,8 @@ mpmx_status mpmx_substep(mpmx_ctx* ctx, double dt);
 /* nsteps substeps with fixed dt (no host synchronisation in between unless the
  * PCG needs a convergence check) */
 mpmx_status mpmx_run(mpmx_ctx* ctx, int nsteps, double dt);
+/* Particles::getMaxParticleVel_c (Particle_kernal.cu:421-440): max |v|^2 over particles */
+mpmx_status mpmx_max_velocity_sq(mpmx_ctx* ctx, double* out);
 /* MPMSimulator::computeDt (Simulator.cpp:269-294): min(dt_default, cfl*dx/max|v|) */
 mpmx_status mpmx_compute_dt(mpmx_ctx* ctx, double dt_default, double cfl, double* dt_out);
 
@@ -206,6 +208,48 @@ mpmx_status mpmx_get_grid(mpmx_ctx* ctx, int channel, float* host, size_t count)
 mpmx_status mpmx_get_grid_p2g(mpmx_ctx* ctx, int channel, float* host, size_t count);
 
 mpmx_status mpmx_get_stats(mpmx_ctx* ctx, mpmx_stats* out);
+
+/* Raw device pointers of the current buffers, for callers that work on the
+ * device data directly as the reference's classes expose it (public members of
+ * Particles, SPGrid and DomainTransformer: Particles.cuh:24-74, SPGrid.h:15-33,
+ * Transformer.cuh:24-50).  Particle buffers are double-buffered and swap on
+ * every sort, so re-read these after mpmx_bin_sort / mpmx_substep.
+ * Layouts: pos, vel float[3n] xyz; F, B, sigma float[9n] component-major
+ * (entry (r,c) of particle i at [(r+3c)*n + i]); scalars float[n]; keys
+ * uint32[n] = Morton page key << 6 | cell; grid float[pages][8][64];
+ * PCG vectors float[pages*64]; page_key uint32; adj int[7][cap_pages];
+ * target int (0-based).  Fields absent in the active precision are NULL. */
+typedef struct mpmx_device_ptrs {
+    void* pos;
+    void* vel;
+    void* F;
+    void* B;
+    void* phase_c;
+    void* alpha;
+    void* psi;
+    void* g;
+    void* mass;
+    int32_t* ids;
+    float* sigma;
+    float* FP;
+    float* vol;
+    uint32_t* keys;
+    float* grid;
+    float* pcg_x;
+    float* pcg_r;
+    float* pcg_z;
+    float* pcg_s;
+    float* pcg_q;
+    uint32_t* page_key;
+    int32_t* page2particle;
+    int32_t* adj;
+    int32_t* vpo;
+    int32_t* target;
+    int64_t cap_pages;      /* stride of adj */
+    int64_t cap_total_pages;
+    int     precision;
+} mpmx_device_ptrs;
+mpmx_status mpmx_get_device_pointers(mpmx_ctx* ctx, mpmx_device_ptrs* out);
 
 /* kernel timing with HIP events on the context stream */
 mpmx_status mpmx_timing_begin(mpmx_ctx* ctx);
