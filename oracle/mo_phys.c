@@ -329,6 +329,14 @@ static int node_index(const mo_sim* s, int i, int j, int k) {
 
 static double* CH(mo_sim* s, int ch) { return s->grid + (size_t)ch * s->gcap * 64; }
 
+/* A node takes part in the grid update and the damage solve when some
+ * particle gives it a positive weight.  The reference tests its tag channel
+ * (0.0001 per contribution, TimeIntegrator.cu:67-69), which also marks nodes
+ * reached only with weight exactly 0 (a particle coordinate exactly on a cell
+ * centre) and then divides by their zero mass; sum w > 0 is the same set
+ * without those nodes. */
+static int ACTIVE(mo_sim* s, int ni) { return CH(s, CH_W)[ni] > 0.0; }
+
 /* quadratic B-spline weights and gradients (P2GKernal.cu:74-82, 237-258) */
 static void weights(const double xp[3], double inv_dx, double w[3][3], double wg[3][3]) {
     for (int a = 0; a < 3; ++a) {
@@ -434,7 +442,7 @@ static void p2g(mo_sim* s, double dt) {
 static void grid_update(mo_sim* s, double dt) {
     const int nn = s->counts[1] * 64;
     for (int ni = 0; ni < nn; ++ni) {
-        if (!(CH(s, CH_TAG)[ni] > (double)0.000001f)) continue;
+        if (!ACTIVE(s, ni)) continue;
         double mass = CH(s, CH_M)[ni];
         mass = 1.0 / mass;
         CH(s, CH_C)[ni] = CH(s, CH_C)[ni] / CH(s, CH_W)[ni];
@@ -518,7 +526,7 @@ static void precondition(mo_sim* s, double dt) {
     }
     const int nn = s->counts[1] * 64;
     for (int ni = 0; ni < nn; ++ni) {
-        if (!(CH(s, CH_TAG)[ni] > 0.0)) continue;
+        if (!ACTIVE(s, ni)) continue;
         const double mr = CH(s, CH_DIAG)[ni];
         CH(s, CH_DIAG)[ni] = 0.0;
         if (fabs(mr) < 1e-30) continue;
@@ -596,7 +604,7 @@ static int phase_field(mo_sim* s, double dt) {
     }
     for (int i = 0; i < nn; ++i) { s->r[i] = 0; s->xx[i] = 0; s->mr[i] = 0; s->q[i] = 0; }
     for (int i = 0; i < nn; ++i)
-        if (CH(s, CH_TAG)[i] > 0.0) s->r[i] = CH(s, CH_RHS)[i];
+        if (ACTIVE(s, i)) s->r[i] = CH(s, CH_RHS)[i];
     double norm = vnorm(s, s->r);
     precondition(s, dt);
     memcpy(s->s, s->mr, sizeof(double) * (size_t)nn);
@@ -609,7 +617,7 @@ static int phase_field(mo_sim* s, double dt) {
         const double sq = vdot(s, s->s, s->q);
         const double alpha = rho_old / sq;
         for (int i = 0; i < nn; ++i)
-            if (CH(s, CH_TAG)[i] > 0.0) {
+            if (ACTIVE(s, i)) {
                 s->xx[i] += s->s[i] * alpha;
                 s->r[i] = s->r[i] - s->q[i] * alpha;
             }
@@ -626,7 +634,7 @@ static int phase_field(mo_sim* s, double dt) {
         const double beta = rho / rho_old;
         rho_old = rho;
         for (int i = 0; i < nn; ++i)
-            if (CH(s, CH_TAG)[i] > 0.0) s->s[i] = s->s[i] * beta + s->mr[i];
+            if (ACTIVE(s, i)) s->s[i] = s->s[i] * beta + s->mr[i];
     }
     free(col);
     s->cg_iters = it;
@@ -634,7 +642,7 @@ static int phase_field(mo_sim* s, double dt) {
     s->cg_norm = norm;
     /* update_grid_phase (:632-642): channel 7 becomes the damage increment */
     for (int i = 0; i < nn; ++i)
-        if (CH(s, CH_TAG)[i] > 0.0) CH(s, CH_C)[i] = s->xx[i] - CH(s, CH_C)[i];
+        if (ACTIVE(s, i)) CH(s, CH_C)[i] = s->xx[i] - CH(s, CH_C)[i];
     return breakdown ? -3 : 0;
 }
 

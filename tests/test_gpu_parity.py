@@ -85,11 +85,14 @@ def test_stress_parity(mpmx, oracle):
         for k in ref:
             ref[k].append(r[k])
     ref = {k: np.array(a) for k, a in ref.items()}
-    # FP32 SVD + log/exp/pow against FP64: scale-relative bound 2e-5
-    for k in ("F", "sigma", "FP", "vol", "g"):
-        assert max_rel_field(gpu[k], ref[k]) < 2e-5, k
-    assert np.abs(gpu["alpha"] - ref["alpha"]).max() < 2e-5
-    assert max_rel_field(gpu["psi"], ref["psi"]) < 2e-5
+    # FP32 SVD + log/exp/pow against FP64 on strains up to ~0.5: the
+    # singular values carry ~1e-6 relative error, amplified by the return
+    # mapping's exp(eps) recomposition; scale-relative bound 1e-4
+    errs = {k: max_rel_field(gpu[k], ref[k]) for k in ("F", "sigma", "FP", "vol", "g", "psi")}
+    errs["alpha"] = np.abs(gpu["alpha"] - ref["alpha"]).max()
+    print("stress parity:", {k: "%.1e" % e for k, e in errs.items()})
+    for k, e in errs.items():
+        assert e < 1e-4, k
 
 
 def test_p2g_parity(mpmx, oracle):
@@ -137,11 +140,67 @@ def test_trajectory_impact(mpmx, oracle):
     ctx, sim = trajectory(mpmx, oracle, 3, 0.1, 60)
     st = ctx.stats()
     assert st["cg_breakdown"] == 0
-    dx = ctx.get("x") - sim.get("x")
-    assert np.abs(dx).max() < 1e-4 / 512 * 512 * 1e-2  # 1e-6 m
+    # x and F: 1e-4 of the field scale, the bar stated for the reduced-precision
+    # forms; FP32 against FP64 stays well inside it through the impact
+    assert max_rel_field(ctx.get("x"), sim.get("x")) < 1e-4
     for k, tol in (("v", 1e-3), ("F", 1e-4)):
         assert max_rel_field(ctx.get(k), sim.get(k)) < tol, k
     assert np.abs(ctx.get("c") - sim.get("c")).max() < 1e-3
+
+
+def test_trajectory_impact_apic(mpmx, oracle):
+    """The reference's default transfer (P2G_APIC / G2P_APIC): stress as a grid
+    force, F updated with sum v grad w^T, no velocity clamp."""
+    ctx, sim = trajectory(mpmx, oracle, 3, 0.1, 60, transfer=1)
+    assert max_rel_field(ctx.get("x"), sim.get("x")) < 1e-4
+    for k, tol in (("v", 1e-3), ("F", 1e-4)):
+        assert max_rel_field(ctx.get(k), sim.get(k)) < tol, k
+    assert np.abs(ctx.get("c") - sim.get("c")).max() < 1e-3
+
+
+def test_true_dot_mode(mpmx, oracle):
+    """dot_mode 1 (true inner products in the PCG) on the impact scene."""
+    ctx, sim = trajectory(mpmx, oracle, 3, 0.1, 60, dot_mode=1)
+    assert max_rel_field(ctx.get("x"), sim.get("x")) < 1e-4
+    assert max_rel_field(ctx.get("F"), sim.get("F")) < 1e-4
+    assert np.abs(ctx.get("c") - sim.get("c")).max() < 1e-3
+
+
+def test_zero_weight_node_free_flight(mpmx):
+    """A lone particle with two coordinates exactly on cell centres reaches
+    nodes with weight 0; they stay inactive and the particle moves as in free
+    flight, v = v0 + g dt per substep (closed form, FP32 rounding only)."""
+    p = mpmx.default_params()
+    p.N = 64
+    p.V0 = (1.0 / 64) ** 3 / 8
+    x = np.array([[20.5 / 64, 30.5 / 64, 40.25 / 64]])
+    v0 = np.array([[0.2, -0.1, 0.05]])
+    ctx = mpmx.Context(p)
+    ctx.set_particles(x, v0, np.array([1e-3]))
+    dt = 1e-4
+    ctx.run(3, dt)
+    v = ctx.get("v")
+    assert np.isfinite(v).all() and np.isfinite(ctx.get("c")).all()
+    assert np.abs(v - (v0 + np.array([0, -10.0, 0]) * 3 * dt)).max() < 1e-6
+    assert ctx.get("c")[0] == pytest.approx(1.0, abs=1e-6)
+
+
+def test_error_codes(mpmx):
+    """Out-of-domain input is refused on the host; a grid capacity too small
+    for the scene is reported as MPMX_ERR_CAPACITY, not a fault."""
+    p, x, v, m, dt = mpmx.scene(0, seed=1, scale=0.5)
+    ctx = mpmx.Context(p)
+    bad = x.copy()
+    bad[0, 0] = 1.5 / p.N
+    with pytest.raises(mpmx.MpmxError) as e:
+        ctx.set_particles(bad, v, m)
+    assert e.value.status == mpmx.ERR_OUT_OF_DOMAIN
+    ctx2 = mpmx.Context(p, mpmx.default_config(max_pages=8))
+    ctx2.set_particles(x, v, m)
+    with pytest.raises(mpmx.MpmxError) as e:
+        ctx2.run(2, dt)
+        ctx2.sync()
+    assert e.value.status == mpmx.ERR_CAPACITY
 
 
 def test_conservation_free_flight(mpmx):

@@ -377,6 +377,29 @@ def test_substep_free_flight_conservation(oracle):
     assert sim.counts()["cg_iters"] == 1
 
 
+def test_zero_weight_node_stays_inactive(oracle):
+    """A particle with a coordinate exactly on a cell centre (x N = k + 1/2)
+    gives its third node along that axis the weight 0.  Alone, that node has
+    no mass and must stay out of the grid update and the damage solve: the
+    particle then moves exactly as in free flight, v = v0 + g dt."""
+    N = 64
+    x = np.array([[20.5 / N, 30.5 / N, 40.25 / N]])
+    v0 = np.array([[0.2, -0.1, 0.05]])
+    p = oracle.default_params(N=N, V0=(1 / N) ** 3 / 8)
+    sim = oracle.Sim(p, x, v0, np.array([1e-3]))
+    dt = 1e-4
+    sim.substep(dt)
+    assert np.isfinite(sim.get("v")).all() and np.isfinite(sim.get("c")).all()
+    assert np.abs(sim.get("v") - (v0 + np.array([0, -10.0, 0]) * dt)).max() < 1e-13
+    w = sim.grid(8, stage=1)
+    m = sim.grid(0, stage=1)
+    touched = sim.grid(9, stage=1) > 0
+    # 27 contributions; weight 0 on the third node of both cell-centre axes: 2 x 2 x 3 remain
+    assert touched.sum() == 27 and (w > 0).sum() == 12
+    assert np.all(m[touched & (w == 0)] == 0)
+    assert sim.get("c")[0] == pytest.approx(1.0, abs=1e-12)
+
+
 def test_pcg_matches_dense_solve(oracle):
     """Phase-field system (D + M K) x = b assembled densely from the particles
     with textbook B-spline weights; the oracle's PCG (true dot products, tight
@@ -400,7 +423,7 @@ def test_pcg_matches_dense_solve(oracle):
     c_grid = sim.grid(7, stage=1)
     xsol = sim.grid(7, stage=2) + c_grid
     rhs_o = sim.grid(10, stage=2)
-    tag = sim.grid(9, stage=1) > 0
+    tag = sim.grid(8, stage=1) > 0  # active nodes: sum w > 0
     pages = sim.pages()
 
     def coords(pk):
