@@ -272,16 +272,22 @@ static void free_particles(mpmx_ctx* c) {
     dfree(c->sigma); dfree(c->FP); dfree(c->vol);
     dfree(c->keys); dfree(c->keys_alt); dfree(c->vals); dfree(c->vals_alt); dfree(c->hist);
     dfree(c->pmask); dfree(c->nmask); dfree(c->wcount); dfree(c->scount);
-    dfree(c->page_key); dfree(c->page2particle); dfree(c->adj); dfree(c->nvp); dfree(c->vpo); dfree(c->target);
+    dfree(c->page_key); dfree(c->page2particle); dfree(c->adj); dfree(c->nvp); dfree(c->vpo); dfree(c->target); dfree(c->block_key); dfree(c->bcount);
     dfree(c->grid); dfree(c->grid_p2g);
-    dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q);
+    dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q); dfree(c->cg_part);
     dfree(c->scan_tmp);
 }
 
 static mpmx_status alloc_grid(mpmx_ctx* c, int64_t cap_tp) {
     dfree(c->grid); dfree(c->grid_p2g);
-    dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q);
+    dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q); dfree(c->cg_part);
     c->cap_tp = cap_tp;
+    // workgroups of the node-space kernels: about four 128-node blocks each at
+    // capacity, at most 4 per CU, a multiple of 8 (one share per XCD)
+    int64_t nb = ((cap_tp + 1) / 2 + 3) / 4;
+    nb = std::min<int64_t>(nb, (int64_t)c->num_cu * 4);
+    c->cg_nb = (int)std::max<int64_t>(8, (nb + 7) / 8 * 8);
+    HIPCHK(c, dalloc(&c->cg_part, 4 * (int64_t)c->cg_nb));
     HIPCHK(c, dalloc(&c->grid, cap_tp * kPageFloats));
     if (c->keep_p2g) HIPCHK(c, dalloc(&c->grid_p2g, cap_tp * kPageFloats));
     HIPCHK(c, dalloc(&c->cg_x, cap_tp * 64));
@@ -335,6 +341,7 @@ void mpmx_default_config(mpmx_config* c) {
     c->precision = MPMX_FP32;
     c->svd_gap = 1e-4;
     c->cg_check_every = 1;
+    c->stress_fp64 = 1;
 }
 
 const char* mpmx_status_string(mpmx_status s) {
@@ -370,7 +377,6 @@ mpmx_status mpmx_create(const mpmx_params* p, const mpmx_config* cfg, mpmx_ctx**
     c->P = *p;
     if (cfg) c->cfg = *cfg;
     else mpmx_default_config(&c->cfg);
-    if (c->cfg.svd_gap <= 0) c->cfg.svd_gap = 1e-4;
     c->device = c->cfg.device;
     if (hipSetDevice(c->device) != hipSuccess) {
         delete c;
@@ -474,6 +480,9 @@ mpmx_status mpmx_set_particles(mpmx_ctx* c, int64_t n, const double* x, const do
     const int64_t tp_bound = std::min<int64_t>(8 * c->cap_p, pages_nb);
     c->cap_vp = n / c->P.vpage + c->cap_p + 1;
     HIPCHK(c, dalloc(&c->page_key, tp_bound));
+    c->tp_bound = tp_bound;
+    HIPCHK(c, dalloc(&c->block_key, tp_bound));
+    HIPCHK(c, dalloc(&c->bcount, c->mask_words));
     HIPCHK(c, dalloc(&c->page2particle, c->cap_p + 1));
     HIPCHK(c, dalloc(&c->adj, 7 * c->cap_p));
     HIPCHK(c, dalloc(&c->nvp, c->cap_p + 1));
@@ -937,7 +946,7 @@ mpmx_status mpmx_get_stats(mpmx_ctx* c, mpmx_stats* out) {
     out->num_virtual_page = c->hc->num_vp;
     out->cg_iters = c->hc->cg_iters;
     out->cg_breakdown = c->hc->cg_breakdown;
-    out->cg_norm = __builtin_bit_cast(float, c->hc->norm_key);
+    out->cg_norm = c->hc->cg_norm;
     return check_err(c);
 }
 

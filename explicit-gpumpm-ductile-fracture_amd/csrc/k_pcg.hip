@@ -14,15 +14,37 @@
 //    applied per node instead of per particle;
 //  - D is computed once per substep (the reference rebuilds it every
 //    iteration, preConditionP2G_APIC);
-//  - the convergence test, the breakdown test and the scalars live on the
-//    device; every kernel returns at once after convergence, so a fixed number
-//    of launches can be queued (or captured) without host round trips.
+//  - scalars never leave the device: each node kernel writes one partial per
+//    workgroup (max or sum of its block sums, max|r|) and the next kernel
+//    reduces the partials in a fixed order, so no same-address atomics and a
+//    bit-reproducible solve; the convergence test is evaluated identically by
+//    every workgroup, and every kernel returns at once after convergence, so a
+//    fixed number of iterations can be queued without host round trips;
+//  - a breakdown (rho = 0 before convergence) sets x = 1e20 for G2P as the
+//    reference does (:959-962), which leaves c unchanged that substep; it is
+//    counted in the statistics, not raised as an error.
+//
+// Partials: cg_part[0] s.q, cg_part[1 + (k & 1)] r.z entering iteration k,
+// cg_part[3] max|r|, each cg_nb floats.
 #include "access.h"
 
 namespace mpmx {
 
-__device__ __forceinline__ float dot_value(const DevParams& dp, unsigned key, float sum) {
-    return dp.dot_mode == 0 ? fkey_dec(key) : sum;
+// the PCG has stopped (converged, broken down): read once per workgroup so
+// that every thread of it takes the same branch
+__device__ __forceinline__ bool cg_stopped(const DevCounts* dc) {
+    __shared__ int sdone;
+    if (threadIdx.x == 0) sdone = dc->cg_done;
+    __syncthreads();
+    return sdone != 0;
+}
+
+// "dot" of the partials: max of block sums (dot_mode 0) or their sum
+__device__ __forceinline__ float reduce_dot(const DevParams& dp, const float* parts, int cnt, float* lds) {
+    return dp.dot_mode == 0 ? reduce_parts<true>(parts, cnt, lds) : reduce_parts<false>(parts, cnt, lds);
+}
+__device__ __forceinline__ float combine_dot(const DevParams& dp, float v, float* lds) {
+    return dp.dot_mode == 0 ? combine_waves<true>(v, lds) : combine_waves<false>(v, lds);
 }
 
 // q += M grad w . col for every particle, col = V dx^2 sum_j grad w_j s_j
@@ -30,13 +52,7 @@ template <int BLOCK, typename ST>
 __global__ __launch_bounds__(BLOCK) void k_cg_apply(DevParams dp, ST s, const float* __restrict__ vol, Tables t,
                                                     const float* __restrict__ cgs, float* __restrict__ cgq,
                                                     int n, DevCounts* dc) {
-    if (dc->cg_done) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        dc->rho_old = dot_value(dp, dc->rho_key, dc->rho_sum);
-        dc->sq_key = 0u;
-        dc->sq_sum = 0.f;
-        dc->cg_iters += 1;
-    }
+    if (cg_stopped(dc)) return;
     __shared__ float ts[kTileN];
     __shared__ float tq[kTileN];
     int nvp = dc->num_vp;
@@ -116,18 +132,17 @@ __global__ __launch_bounds__(BLOCK) void k_cg_apply(DevParams dp, ST s, const fl
     const int nblk = ((nt) + 1) / 2;                                         \
     for (int blk = wave; blk < nblk; blk += nwaves)
 
-// q = q + s D; sq = "dot"(s, q)
+// q = q + s D; partial of "dot"(s, q)
 __global__ __launch_bounds__(256) void k_cg_dot(DevParams dp, const float* __restrict__ grid,
                                                 const float* __restrict__ cgs, float* __restrict__ cgq,
-                                                int64_t cap_tp, DevCounts* dc) {
-    if (dc->cg_done) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        dc->norm_key = 0u;
-        dc->rho_key = 0u;
-        dc->rho_sum = 0.f;
-    }
+                                                float* __restrict__ part_sq, int64_t cap_tp,
+                                                const DevCounts* __restrict__ dc) {
+    __shared__ float lds[4];
+    if (cg_stopped(dc)) return;
     int nt = dc->num_total;
     nt = nt < cap_tp ? nt : (int)cap_tp;
+    const bool maxdot = dp.dot_mode == 0;
+    float wacc = maxdot ? -INFINITY : 0.f;
     FOR_128_BLOCKS(nt) {
         const int n0 = blk * 128 + lane * 2;
         float part = 0.f;
@@ -141,27 +156,32 @@ __global__ __launch_bounds__(256) void k_cg_dot(DevParams dp, const float* __res
             part = sv.x * qv.x + sv.y * qv.y;
         }
         const float bs = wave_sum(part);
-        if (lane == 0) {
-            if (dp.dot_mode == 0) atomicMax(&dc->sq_key, fkey(bs));
-            else atomicAdd(&dc->sq_sum, bs);
-        }
+        wacc = maxdot ? fmaxf(wacc, bs) : wacc + bs;
     }
+    const float r = combine_dot(dp, wacc, lds);
+    if (threadIdx.x == 0) part_sq[blockIdx.x] = r;
 }
 
-// alpha = rho_old / sq; x += alpha s; r -= alpha q; q = 0; norm = max|r|;
-// z = r / D where |D| >= 1e-30 (else z keeps its value, as precondition_subd);
-// rho = "dot"(r, z).  Only nodes touched by particles (sum w > 0) change.
+// alpha = rho_old / sq; x += alpha s; r -= alpha q; q = 0; z = r / D where
+// |D| >= 1e-30 (else z keeps its value, as precondition_subd); partials of
+// max|r| and "dot"(r, z).  Only nodes touched by particles (sum w > 0) change.
 __global__ __launch_bounds__(256) void k_cg_update(DevParams dp, const float* __restrict__ grid,
                                                    float* __restrict__ cgx, float* __restrict__ cgr,
                                                    float* __restrict__ cgz, const float* __restrict__ cgs,
-                                                   float* __restrict__ cgq, int64_t cap_tp, DevCounts* dc) {
-    if (dc->cg_done) return;
-    const float alpha = dc->rho_old / dot_value(dp, dc->sq_key, dc->sq_sum);
+                                                   float* __restrict__ cgq, float* __restrict__ part, int nb, int k,
+                                                   int64_t cap_tp, const DevCounts* __restrict__ dc) {
+    __shared__ float lds[4];
+    if (cg_stopped(dc)) return;
+    const float sq = reduce_dot(dp, part, nb, lds);
+    const float rho_old = reduce_dot(dp, part + (1 + (k & 1)) * nb, nb, lds);
+    const float alpha = rho_old / sq;
     int nt = dc->num_total;
     nt = nt < cap_tp ? nt : (int)cap_tp;
+    const bool maxdot = dp.dot_mode == 0;
+    float wacc = maxdot ? -INFINITY : 0.f, wmax = 0.f;
     FOR_128_BLOCKS(nt) {
         const int n0 = blk * 128 + lane * 2;
-        float part = 0.f, rmax = 0.f;
+        float rz = 0.f, rmax = 0.f;
         if ((n0 >> 6) < nt) {
             const float* gp = grid + (int64_t)(n0 >> 6) * kPageFloats + (n0 & 63);
 #pragma unroll
@@ -178,40 +198,45 @@ __global__ __launch_bounds__(256) void k_cg_update(DevParams dp, const float* __
                         cgz[nn] = z;
                     }
                     rmax = fmaxf(rmax, fabsf(r));
-                    part += r * z;
+                    rz += r * z;
                 }
                 cgq[nn] = 0.f;
             }
         }
-        const float bs = wave_sum(part);
-        const float bm = wave_max(rmax);
-        if (lane == 0) {
-            atomicMax(&dc->norm_key, __float_as_uint(bm));
-            if (dp.dot_mode == 0) atomicMax(&dc->rho_key, fkey(bs));
-            else atomicAdd(&dc->rho_sum, bs);
-        }
+        const float bs = wave_sum(rz);
+        wacc = maxdot ? fmaxf(wacc, bs) : wacc + bs;
+        wmax = fmaxf(wmax, wave_max(rmax));
+    }
+    const float rho = combine_dot(dp, wacc, lds);
+    const float norm = combine_waves<true>(wmax, lds);
+    if (threadIdx.x == 0) {
+        part[(1 + ((k + 1) & 1)) * nb + blockIdx.x] = rho;
+        part[3 * nb + blockIdx.x] = norm;
     }
 }
 
 // convergence / breakdown test, then s = s beta + z
 __global__ __launch_bounds__(256) void k_cg_direction(DevParams dp, const float* __restrict__ grid,
                                                       const float* __restrict__ cgz, float* __restrict__ cgs,
+                                                      const float* __restrict__ part, int nb, int k,
                                                       int64_t cap_tp, DevCounts* dc) {
-    if (dc->cg_done) return;
-    const float norm = __uint_as_float(dc->norm_key);
+    __shared__ float lds[4];
+    if (cg_stopped(dc)) return;
+    const float norm = reduce_parts<true>(part + 3 * nb, nb, lds);
+    const float rho_old = reduce_dot(dp, part + (1 + (k & 1)) * nb, nb, lds);
+    const float rho = reduce_dot(dp, part + (1 + ((k + 1) & 1)) * nb, nb, lds);
     const bool converged = norm < dp.cg_tol;
-    const float rho_old = dc->rho_old;
-    if (converged || rho_old == 0.f) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const bool stop = converged || rho_old == 0.f;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        dc->cg_iters = k + 1;
+        dc->cg_norm = norm;
+        if (stop) {
             dc->cg_done = 1;
-            if (!converged) {
-                dc->cg_breakdown = 1;
-                atomicOr(&dc->err, kErrBreakdown);
-            }
+            if (!converged) dc->cg_breakdown = 1;
         }
-        return;
     }
-    const float beta = dot_value(dp, dc->rho_key, dc->rho_sum) / rho_old;
+    if (stop) return;
+    const float beta = rho / rho_old;
     int nt = dc->num_total;
     nt = nt < cap_tp ? nt : (int)cap_tp;
     FOR_128_BLOCKS(nt) {
@@ -240,22 +265,18 @@ static void apply_block(mpmx_ctx* c, ST st) {
     }
 }
 
-static int node_blocks(mpmx_ctx* c) {
-    int64_t blocks = (c->cap_tp * 64 / 128 + 3) / 4;
-    blocks = blocks < (int64_t)c->num_cu * 16 ? blocks : (int64_t)c->num_cu * 16;
-    return blocks < 1 ? 1 : (int)blocks;
-}
-
-// one PCG iteration: four launches, all no-ops once the device flag says done
-static hipError_t pcg_iteration(mpmx_ctx* c) {
+// one PCG iteration k: four launches, all no-ops once the device flag says done
+static hipError_t pcg_iteration(mpmx_ctx* c, int k) {
     MPMX_TIMED(c, "cg_apply", (c->cfg.precision == MPMX_MIXED) ? apply_block(c, c->ms[c->cur])
                                                                : apply_block(c, c->ps[c->cur]));
-    const int nb = node_blocks(c);
+    const int nb = c->cg_nb;
     MPMX_TIMED(c, "cg_vector", ({
-                   k_cg_dot<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_s, c->cg_q, c->cap_tp, c->dc);
+                   k_cg_dot<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_s, c->cg_q, c->cg_part, c->cap_tp,
+                                                       c->dc);
                    k_cg_update<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_x, c->cg_r, c->cg_z, c->cg_s,
-                                                          c->cg_q, c->cap_tp, c->dc);
-                   k_cg_direction<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_z, c->cg_s, c->cap_tp, c->dc);
+                                                          c->cg_q, c->cg_part, nb, k, c->cap_tp, c->dc);
+                   k_cg_direction<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_z, c->cg_s, c->cg_part, nb, k,
+                                                             c->cap_tp, c->dc);
                }));
     return hipGetLastError();
 }
@@ -268,7 +289,7 @@ hipError_t launch_pcg(mpmx_ctx* c, float dt) {
     while (it < c->P.cg_max_iter) {
         const int chunk = (c->P.cg_max_iter - it) < every ? (c->P.cg_max_iter - it) : every;
         for (int k = 0; k < chunk; ++k) {
-            hipError_t e = pcg_iteration(c);
+            hipError_t e = pcg_iteration(c, it + k);
             if (e != hipSuccess) return e;
         }
         it += chunk;

@@ -79,7 +79,10 @@ __device__ __forceinline__ void von_mises(const DevParams& dp, const StressIn& i
         }
         if (alpha > T(dp.alpha_c)) out.alpha = dp.alpha_c;
         const T tdn = sqrt(tau_dev_tr[0] * tau_dev_tr[0] + tau_dev_tr[1] * tau_dev_tr[1] + tau_dev_tr[2] * tau_dev_tr[2]);
-        const T taudiv = (T(1) / tdn) * p23 * g * tau_tr_sum * dlambda;
+        // no deviatoric trial stress (equal singular values, which FP32
+        // rounding produces near F = s R): no flow direction, no flow; the
+        // reference's (1/0) * 0 would give NaN
+        const T taudiv = tdn > T(0) ? (T(1) / tdn) * p23 * g * tau_tr_sum * dlambda : T(0);
         const T third = (T(1) / T(3)) * tau_tr_sum;
         T tau_nn[3];
 #pragma unroll
@@ -147,9 +150,23 @@ __device__ __forceinline__ void von_mises(const DevParams& dp, const StressIn& i
     out.vol = (float)(T(dp.V0) * det);
 }
 
-__device__ __forceinline__ void stress_one(const DevParams& dp, const StressIn& in, StressOut& out,
-                                           bool allow_fp64, int* fallbacks) {
+// ALL64: SVD and return mapping in FP64 for every particle.  Otherwise FP32,
+// redone in FP64 where the smallest singular-value gap is below svd_gap (the
+// directions of near-equal singular values are ill-conditioned in FP32).
+// Returns whether the FP64 redo ran.
+template <bool ALL64>
+__device__ __forceinline__ bool stress_one(const DevParams& dp, const StressIn& in, StressOut& out) {
     out.alpha = in.alpha;
+    if (ALL64) {
+        double Ad[3][3], Ud[3][3], Sd[3], Vd[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) Ad[r][cc] = in.F[r][cc];
+        svd3<double>(Ad, Ud, Sd, Vd);
+        von_mises<double>(dp, in, out, Ud, Sd, Vd);
+        return false;
+    }
     float A[3][3], U[3][3], S[3], V[3][3];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -157,9 +174,7 @@ __device__ __forceinline__ void stress_one(const DevParams& dp, const StressIn& 
         for (int cc = 0; cc < 3; ++cc) A[r][cc] = in.F[r][cc];
     svd3<float>(A, U, S, V);
     const float gap = fminf(fabsf(S[0] - S[1]), fabsf(S[1] - S[2]));
-    if (allow_fp64 && gap < dp.svd_gap) {
-        // near-degenerate singular values: redo the decomposition and the
-        // return mapping in FP64
+    if (gap < dp.svd_gap) {
         double Ad[3][3], Ud[3][3], Sd[3], Vd[3][3];
 #pragma unroll
         for (int r = 0; r < 3; ++r)
@@ -167,12 +182,20 @@ __device__ __forceinline__ void stress_one(const DevParams& dp, const StressIn& 
             for (int cc = 0; cc < 3; ++cc) Ad[r][cc] = A[r][cc];
         svd3<double>(Ad, Ud, Sd, Vd);
         von_mises<double>(dp, in, out, Ud, Sd, Vd);
-        atomicAdd(fallbacks, 1);
-        return;
+        return true;
     }
     von_mises<float>(dp, in, out, U, S, V);
+    return false;
 }
 
+// one atomic per wave for the FP64-redo counter
+__device__ __forceinline__ void count_fallbacks(bool fb, int* counter) {
+    const unsigned long long m = __ballot(fb);
+    const unsigned long long act = __ballot(1);
+    if (m && (int)(threadIdx.x & 63) == __ffsll((long long)act) - 1) atomicAdd(counter, (int)__popcll(m));
+}
+
+template <bool ALL64>
 __global__ __launch_bounds__(256) void k_stress_fp32(DevParams dp, PState s, float* __restrict__ sigma,
                                                      float* __restrict__ FP, float* __restrict__ vol, int n,
                                                      DevCounts* dc) {
@@ -188,7 +211,7 @@ __global__ __launch_bounds__(256) void k_stress_fp32(DevParams dp, PState s, flo
     in.alpha = s.alpha[i];
     in.psi = s.psi[i];
     StressOut out;
-    stress_one(dp, in, out, false, &dc->svd_fallbacks);
+    count_fallbacks(stress_one<ALL64>(dp, in, out), &dc->svd_fallbacks);
     bool finite = true;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -206,6 +229,7 @@ __global__ __launch_bounds__(256) void k_stress_fp32(DevParams dp, PState s, flo
     if (!finite) atomicOr(&dc->err, kErrNonFinite);
 }
 
+template <bool ALL64>
 __global__ __launch_bounds__(256) void k_stress_mixed(DevParams dp, MState s, float* __restrict__ sigma,
                                                       float* __restrict__ FP, float* __restrict__ vol, int n,
                                                       DevCounts* dc) {
@@ -222,7 +246,7 @@ __global__ __launch_bounds__(256) void k_stress_mixed(DevParams dp, MState s, fl
     in.alpha = bf2f(s.alpha[i]);
     in.psi = bf2f(s.psi[i]);
     StressOut out;
-    stress_one(dp, in, out, true, &dc->svd_fallbacks);
+    count_fallbacks(stress_one<ALL64>(dp, in, out), &dc->svd_fallbacks);
     bool finite = true;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -240,14 +264,20 @@ __global__ __launch_bounds__(256) void k_stress_mixed(DevParams dp, MState s, fl
     if (!finite) atomicOr(&dc->err, kErrNonFinite);
 }
 
-hipError_t launch_stress(mpmx_ctx* c) {
+template <bool ALL64>
+static void stress_dispatch(mpmx_ctx* c) {
     const int n = (int)c->n;
     if (c->cfg.precision == MPMX_MIXED)
-        k_stress_mixed<<<(n + 255) / 256, 256, 0, c->stream>>>(c->dp, c->ms[c->cur], c->sigma, c->FP, c->vol,
-                                                               n, c->dc);
+        k_stress_mixed<ALL64><<<(n + 255) / 256, 256, 0, c->stream>>>(c->dp, c->ms[c->cur], c->sigma, c->FP,
+                                                                      c->vol, n, c->dc);
     else
-        k_stress_fp32<<<(n + 255) / 256, 256, 0, c->stream>>>(c->dp, c->ps[c->cur], c->sigma, c->FP, c->vol,
-                                                              n, c->dc);
+        k_stress_fp32<ALL64><<<(n + 255) / 256, 256, 0, c->stream>>>(c->dp, c->ps[c->cur], c->sigma, c->FP,
+                                                                     c->vol, n, c->dc);
+}
+
+hipError_t launch_stress(mpmx_ctx* c) {
+    if (c->cfg.stress_fp64) stress_dispatch<true>(c);
+    else stress_dispatch<false>(c);
     return hipGetLastError();
 }
 

@@ -22,18 +22,6 @@ __device__ __forceinline__ uint32_t neighbor_key(uint32_t pk, int q) {
     return morton((uint32_t)(x + ((o >> 2) & 1)), (uint32_t)(y + ((o >> 1) & 1)), (uint32_t)(z + (o & 1)));
 }
 
-__device__ __forceinline__ int page_rank(uint32_t k, const uint64_t* __restrict__ pmask,
-                                         const uint64_t* __restrict__ nmask, const int* __restrict__ wpre,
-                                         const int* __restrict__ spre, int num_page) {
-    const uint32_t w = k >> 6, b = k & 63u;
-    const uint64_t below = b ? ((~0ull) >> (64 - b)) : 0ull;
-    const uint64_t P = pmask[w];
-    if ((P >> b) & 1ull) return wpre[w] + __popcll(P & below);
-    const uint64_t S = nmask[w] & ~P;
-    if ((S >> b) & 1ull) return num_page + spre[w] + __popcll(S & below);
-    return -1;
-}
-
 // markPageBoundary + buildHashMapFromPage + supplementAdjacentPages, as bits
 __global__ void k_mark_pages(const uint32_t* __restrict__ keys, int n, uint64_t* pmask, uint64_t* nmask) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -135,6 +123,35 @@ __global__ void k_vtargets(int* vpo, int* target, int64_t cap_p, int64_t cap_vp,
     for (int v = b; v < e && v < cap_vp; ++v) target[v] = p;
 }
 
+// 2x2x2-page blocks for the gather transfer.  The 8 pages of block b are the
+// Morton codes 8b .. 8b+7, i.e. one byte of the page bitmaps, so a block is
+// occupied when that byte of pmask | nmask is non-zero; blocks are numbered in
+// ascending Morton order by a scan of the per-word counts.
+__global__ void k_block_counts(const uint64_t* __restrict__ pmask, const uint64_t* __restrict__ nmask,
+                               int* __restrict__ bcount, int64_t words) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    const uint64_t m = pmask[w] | nmask[w];
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c += ((m >> (8 * k)) & 0xffull) != 0ull;
+    bcount[w] = c;
+}
+__global__ void k_block_keys(const uint64_t* __restrict__ pmask, const uint64_t* __restrict__ nmask,
+                             const int* __restrict__ bpre, int64_t words, uint32_t* __restrict__ block_key,
+                             int64_t cap) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    const uint64_t m = pmask[w] | nmask[w];
+    int id = bpre[w];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if ((m >> (8 * k)) & 0xffull) {
+            if (id < cap) block_key[id] = (uint32_t)(w * 8 + k);
+            ++id;
+        }
+}
+
 hipError_t build_tables(mpmx_ctx* c) {
     const int n = (int)c->n;
     const int64_t words = c->mask_words;
@@ -160,6 +177,11 @@ hipError_t build_tables(mpmx_ctx* c) {
     if ((e = scan_exclusive(c, c->nvp, c->vpo, c->cap_p, &c->dc->num_page, 0, &c->dc->num_vp)) != hipSuccess)
         return e;
     k_vtargets<<<pblocks, 256, 0, c->stream>>>(c->vpo, c->target, c->cap_p, c->cap_vp, c->dc);
+    const int wb = (int)((words + 255) / 256);
+    k_block_counts<<<wb, 256, 0, c->stream>>>(c->pmask, c->nmask, c->bcount, words);
+    if ((e = scan_exclusive(c, c->bcount, c->bcount, words, nullptr, words, &c->dc->num_blocks)) != hipSuccess)
+        return e;
+    k_block_keys<<<wb, 256, 0, c->stream>>>(c->pmask, c->nmask, c->bcount, words, c->block_key, c->tp_bound);
     return hipGetLastError();
 }
 

@@ -1,22 +1,21 @@
-// k_transfer.hip — particle-to-grid transfer, grid update with boundaries,
-// and grid-to-particle transfer (MLS and APIC).
+// k_transfer.hip — scatter particle-to-grid transfer and grid-to-particle
+// transfer (MLS and APIC).
 //
 //   P2G  replaces P2G_MLS / P2G_APIC (P2GKernal.cu:6-155, 512-702) and the two
 //        phase-field particle passes volP2G_APIC / preConditionP2G_APIC
 //        (P2GKernal.cu:897-1157): their per-node sums are sum(V w) and
 //        sum(V FP w) times node constants, so they ride along as two more
 //        channels of the same pass.
-//   grid undateGrid_kernal + collideWithGround / PullGrid (TimeIntegrator.cu:63-261)
-//        plus the PCG set-up of ExplicitTimeIntegrator.cu:914-935.
 //   G2P  replaces G2P_MLS / G2P_APIC (G2PKernal.cu:32-441) and update_grid_phase
 //        (ExplicitTimeIntegrator.cu:632-642), whose increment x - c is formed
 //        while the tile is staged.
 //
 // One workgroup works through a contiguous run of virtual pages (<= vpage
 // particles of one 4x4x4-cell page, sorted by cell).  The 6x6x6 node tile that
-// the page's particles touch is staged in LDS; P2G accumulates it with LDS
-// float atomics and flushes only non-zero entries to HBM with global float
-// atomics (8 pages: the page and its 7 +x/+y/+z neighbours).
+// the page's particles touch is staged in LDS.  The scatter P2G here (variant
+// 1) accumulates that tile with LDS float atomics and flushes non-zero entries
+// to HBM with global float atomics (8 pages: the page and its 7 +x/+y/+z
+// neighbours); the default gather P2G is in k_grid.hip.
 #include "access.h"
 
 namespace mpmx {
@@ -117,100 +116,6 @@ __global__ __launch_bounds__(BLOCK) void k_p2g(DevParams dp, ST s, const float* 
             }
         }
         __syncthreads();
-    }
-}
-
-// PCG scalars start from zero before the grid update accumulates rho into them
-__global__ void k_cg_reset(DevParams dp, DevCounts* dc) {
-    if (threadIdx.x != 0) return;
-    dc->cg_done = dp.phase_field ? 0 : 1;
-    dc->cg_iters = 0;
-    dc->cg_breakdown = 0;
-    dc->sq_key = 0u;
-    dc->rho_key = 0u;
-    dc->norm_key = 0u;
-    dc->sq_sum = 0.f;
-    dc->rho_sum = 0.f;
-    dc->rho_old = 0.f;
-}
-
-// undateGrid_kernal + collideWithGround/PullGrid + PCG initialisation.
-// One wavefront per 128 consecutive nodes (= two pages), two nodes per lane;
-// the 128-node blocks are exactly the reference's 128-entry dot-product blocks
-// (ExplicitTimeIntegrator.cu:573-609, 731-747).
-__global__ __launch_bounds__(256) void k_grid_update(DevParams dp, const uint32_t* __restrict__ page_key,
-                                                     float* __restrict__ grid, float* __restrict__ cgx,
-                                                     float* __restrict__ cgr, float* __restrict__ cgz,
-                                                     float* __restrict__ cgs, float* __restrict__ cgq,
-                                                     float dt, int64_t cap_tp, DevCounts* dc) {
-    int nt = dc->num_total;
-    nt = nt < cap_tp ? nt : (int)cap_tp;
-    const int nblk = (nt + 1) / 2;
-    const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (int blk = wave; blk < nblk; blk += nwaves) {
-        float rz = 0.f;
-        const int n0 = blk * 128 + lane * 2;
-        const int pg = n0 >> 6;
-        if (pg < nt) {
-            int px, py, pz;
-            unmorton(page_key[pg], px, py, pz);
-            float* gp = grid + (int64_t)pg * kPageFloats;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int cell = (n0 & 63) + h;
-                const int64_t nn = (int64_t)pg * 64 + cell;
-                const float W = gp[5 * 64 + cell];
-                float r = 0.f, z = 0.f;
-                if (W > 0.f) {
-                    const float m = gp[cell];
-                    const float inv = 1.0f / m;
-                    const float c = gp[4 * 64 + cell] / W;
-                    float v0 = gp[1 * 64 + cell] * inv + dp.gravity[0] * dt;
-                    float v1 = gp[2 * 64 + cell] * inv + dp.gravity[1] * dt;
-                    float v2 = gp[3 * 64 + cell] * inv + dp.gravity[2] * dt;
-                    const int gi = px * 4 + (cell >> 4), gj = py * 4 + ((cell >> 2) & 3), gk = pz * 4 + (cell & 3);
-                    if (dp.grip_lo >= 0) {
-                        if (gi <= dp.grip_lo) { v0 = -dp.grip_v; v1 = 0.f; v2 = 0.f; }
-                        if (gi >= dp.grip_hi) { v0 = dp.grip_v; v1 = 0.f; v2 = 0.f; }
-                    }
-                    const int lo = dp.wall, hi = dp.N - dp.wall;
-                    if (gi <= lo && v0 < 0.f) { v0 = 0.f; v1 = 0.f; v2 = 0.f; }
-                    if (gi > hi && v0 > 0.f) { v0 = 0.f; v1 = 0.f; v2 = 0.f; }
-                    if (gj <= lo && v1 < 0.f) { v0 = 0.f; v1 = 0.f; v2 = 0.f; }
-                    if (gj > hi && v1 > 0.f) { v0 = 0.f; v1 = 0.f; v2 = 0.f; }
-                    if (gk <= lo && v2 < 0.f) { v0 = 0.f; v1 = 0.f; v2 = 0.f; }
-                    if (gk > hi && v2 > 0.f) { v0 = 0.f; v1 = 0.f; v2 = 0.f; }
-                    gp[1 * 64 + cell] = v0;
-                    gp[2 * 64 + cell] = v1;
-                    gp[3 * 64 + cell] = v2;
-                    gp[4 * 64 + cell] = c;
-                    const float VW = gp[6 * 64 + cell], VFW = gp[7 * 64 + cell];
-                    const float rhs = VW * (c / dt + dp.M);
-                    const float D = dp.M * VFW + VW / dt;
-                    gp[6 * 64 + cell] = rhs;
-                    gp[7 * 64 + cell] = D;
-                    r = rhs;
-                    z = (fabsf(D) < 1e-30f) ? 0.f : r / D;
-                }
-                if (dp.phase_field) {
-                    cgx[nn] = 0.f;
-                    cgr[nn] = r;
-                    cgz[nn] = z;
-                    cgs[nn] = z;
-                    cgq[nn] = 0.f;
-                }
-                rz += r * z;
-            }
-        }
-        if (dp.phase_field) {
-            const float bs = wave_sum(rz);
-            if (lane == 0) {
-                if (dp.dot_mode == 0) atomicMax(&dc->rho_key, fkey(bs));
-                else atomicAdd(&dc->rho_sum, bs);
-            }
-        }
     }
 }
 
@@ -374,26 +279,13 @@ static void p2g_block(mpmx_ctx* c, ST st, float dt) {
     }
 }
 
-hipError_t launch_p2g(mpmx_ctx* c, float dt) {
+// scatter variant (p2g_variant 1): zero the grid, then every workgroup adds its
+// pages' tiles with global atomics; the grid update runs as a separate pass
+hipError_t launch_p2g_scatter(mpmx_ctx* c, float dt) {
     hipError_t e = fill_zero_dev_count(c, c->grid, &c->dc->num_total, kPageFloats, c->cap_tp);
     if (e != hipSuccess) return e;
     if (c->cfg.precision == MPMX_MIXED) p2g_block(c, c->ms[c->cur], dt);
     else p2g_block(c, c->ps[c->cur], dt);
-    return hipGetLastError();
-}
-
-hipError_t launch_grid_update(mpmx_ctx* c, float dt) {
-    if (c->keep_p2g && c->grid_p2g) {
-        hipError_t e = hipMemcpyAsync(c->grid_p2g, c->grid, sizeof(float) * c->cap_tp * kPageFloats,
-                                      hipMemcpyDeviceToDevice, c->stream);
-        if (e != hipSuccess) return e;
-    }
-    int64_t blocks = (c->cap_tp * 64 / 128 + 3) / 4;
-    blocks = blocks < (int64_t)c->num_cu * 16 ? blocks : (int64_t)c->num_cu * 16;
-    if (blocks < 1) blocks = 1;
-    k_cg_reset<<<1, 64, 0, c->stream>>>(c->dp, c->dc);
-    k_grid_update<<<(int)blocks, 256, 0, c->stream>>>(c->dp, c->page_key, c->grid, c->cg_x, c->cg_r, c->cg_z,
-                                                       c->cg_s, c->cg_q, dt, c->cap_tp, c->dc);
     return hipGetLastError();
 }
 

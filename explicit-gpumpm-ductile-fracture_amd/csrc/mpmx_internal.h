@@ -16,7 +16,9 @@
 //  grid       float[cap_tp][8][64]: per page, 8 channels of 64 cells (256 B each)
 //             0 m | 1-3 momentum -> velocity | 4 sum w c -> c | 5 sum w |
 //             6 sum V w -> PCG rhs | 7 sum V FP w -> Jacobi diagonal
-//  PCG        x, r, z, s, q float[cap_tp*64] page-major (the reference's d_grid_*)
+//  PCG        x, r, z, s, q float[cap_tp*64] page-major (the reference's d_grid_*);
+//             partials float[4][cg_nb]: s.q, r.z (two, by iteration parity), max|r|
+//  blocks     block_key uint32[tp_bound]: occupied 2x2x2-page blocks (page key >> 3)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -58,10 +60,13 @@ struct DevParams {
     float vmax, cg_tol, crit_stretch;
     float L0;
     float sq23f;   // sqrt(2/3) as float
-    float svd_gap; // mixed precision FP64 SVD threshold
+    float svd_gap; // FP32 stress arithmetic: FP64 redo below this singular-value gap
 };
 
-// device-resident counters and PCG scalars
+// device-resident counters and PCG state.  The PCG scalars themselves are not
+// here: every node kernel writes one partial per workgroup (cg_part) and the
+// kernels that need a scalar reduce those partials in a fixed order, so the
+// solve has no same-address atomics and is bit-reproducible.
 struct DevCounts {
     int num_page;
     int num_total;
@@ -71,15 +76,11 @@ struct DevCounts {
     int cg_done;
     int cg_iters;
     int cg_breakdown;
-    unsigned sq_key;    // ordered-float max accumulators
-    unsigned rho_key;
-    unsigned norm_key;
-    float rho_old;
-    float sq_sum;       // true-dot accumulators
-    float rho_sum;
+    float cg_norm;      // max|r| at the last convergence test
     int num_cell;
     int svd_fallbacks;
-    int pad[16];
+    int num_blocks;     // occupied 2x2x2-page blocks
+    int pad[20];
 };
 
 struct PState {
@@ -152,6 +153,9 @@ struct mpmx_ctx {
     int* vpo = nullptr;
     int* target = nullptr;
     int64_t cap_p = 0, cap_tp = 0, cap_vp = 0;
+    int64_t tp_bound = 0;    // total pages the domain can hold
+    uint32_t* block_key = nullptr;   // [tp_bound] occupied 2x2x2-page blocks, Morton order
+    int* bcount = nullptr;           // [mask_words] per-word block counts, then their scan
     // grid + PCG
     float* grid = nullptr;
     float* grid_p2g = nullptr;   // copy of the P2G result (diagnostics only)
@@ -161,6 +165,9 @@ struct mpmx_ctx {
     float* cg_z = nullptr;
     float* cg_s = nullptr;
     float* cg_q = nullptr;
+    float* cg_part = nullptr;    // [4][cg_nb] per-workgroup partials
+    int cg_nb = 0;               // workgroups of every node-space kernel
+    bool grid_fused = false;     // last P2G also ran the grid update (gather variant)
     // counters
     mpmx::DevCounts* dc = nullptr;
     mpmx::DevCounts* hc = nullptr;   // pinned host mirror
@@ -170,7 +177,6 @@ struct mpmx_ctx {
     // misc
     int num_cu = 256;
     int substeps = 0;
-    int last_cg_iters = 0;
     bool timing = false;
     std::vector<mpmx::TimerEntry> timers;
     std::string last_error;
@@ -204,6 +210,20 @@ __device__ __forceinline__ void unmorton(uint32_t k, int& x, int& y, int& z) {
     z = (int)compact3(k);
     y = (int)compact3(k >> 1);
     x = (int)compact3(k >> 2);
+}
+
+// id of page key k: rank among particle pages, or num_page + rank among the
+// neighbour-only pages, or -1 (popcount rank in the page bitmaps)
+__device__ __forceinline__ int page_rank(uint32_t k, const uint64_t* __restrict__ pmask,
+                                         const uint64_t* __restrict__ nmask, const int* __restrict__ wpre,
+                                         const int* __restrict__ spre, int num_page) {
+    const uint32_t w = k >> 6, b = k & 63u;
+    const uint64_t below = b ? ((~0ull) >> (64 - b)) : 0ull;
+    const uint64_t P = pmask[w];
+    if ((P >> b) & 1ull) return wpre[w] + __popcll(P & below);
+    const uint64_t S = nmask[w] & ~P;
+    if ((S >> b) & 1ull) return num_page + spre[w] + __popcll(S & below);
+    return -1;
 }
 
 // base node floor(x*N + 1/2) - 1 evaluated exactly (the reference's
@@ -272,6 +292,32 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// max (MAX) or sum of parts[0..cnt) for every thread of the workgroup, in a
+// fixed order (thread-strided, wave butterfly, waves in order); lds holds one
+// float per wave
+template <bool MAX>
+__device__ __forceinline__ float reduce_parts(const float* __restrict__ parts, int cnt, float* lds) {
+    float v = MAX ? -INFINITY : 0.f;
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) v = MAX ? fmaxf(v, parts[k]) : v + parts[k];
+    v = MAX ? wave_max(v) : wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = lds[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = MAX ? fmaxf(r, lds[w]) : r + lds[w];
+    return r;
+}
+// combine one value per wave into thread 0 of the workgroup, waves in order
+template <bool MAX>
+__device__ __forceinline__ float combine_waves(float v, float* lds) {
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = lds[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = MAX ? fmaxf(r, lds[w]) : r + lds[w];
+    return r;
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs.  Remap the hardware
 // block index so that consecutive logical blocks (neighbouring pages in Morton
 // order, which share grid pages and thus atomics) run on the same XCD/L2.
@@ -303,6 +349,7 @@ hipError_t launch_p2g(mpmx_ctx* c, float dt);
 hipError_t launch_grid_update(mpmx_ctx* c, float dt);
 hipError_t launch_pcg(mpmx_ctx* c, float dt);
 hipError_t launch_g2p(mpmx_ctx* c, float dt);
+hipError_t launch_cg_reset(mpmx_ctx* c);
 hipError_t fill_zero(mpmx_ctx* c, void* p, int64_t bytes);
 hipError_t fill_zero_dev_count(mpmx_ctx* c, float* p, const int* count_dev, int64_t per_unit,
                                int64_t cap_units);

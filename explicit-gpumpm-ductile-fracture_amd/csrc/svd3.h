@@ -10,8 +10,9 @@
 // Written for registers: matrices are plain T[3][3] arrays fully unrolled by
 // the compiler; a Givens rotation is (i, k, c, s).  The sweep loop is bounded
 // (kMaxSweeps) so that a non-converging input can never stall a wavefront; the
-// reference loop is unbounded.  Templated on float (plain path) and double
-// (near-degenerate fallback of the mixed-precision path).
+// reference loop is unbounded.  Templated on float and double (the stress
+// update runs it in FP64 by default: FP64 is half the FP32 rate on CDNA4 and
+// the update is bound by particle-state traffic).
 #pragma once
 #include <hip/hip_runtime.h>
 

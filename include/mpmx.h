@@ -73,8 +73,8 @@ typedef struct mpmx_params {
 
 typedef enum mpmx_precision {
     MPMX_FP32 = 0,       /* FP32 particle state and grid                          */
-    MPMX_MIXED = 1       /* x FP32, F as FP16(F-I), v/C/c/alpha/psi/g BF16, FP32 grid,
-                            FP64 SVD when the singular-value gap < svd_gap         */
+    MPMX_MIXED = 1       /* x FP32, F as FP16(F-I), v/C/c/alpha/psi/g BF16, FP32 grid;
+                            stress arithmetic as mpmx_config.stress_fp64 says       */
 } mpmx_precision;
 
 typedef struct mpmx_config {
@@ -82,11 +82,15 @@ typedef struct mpmx_config {
     int64_t max_pages;      /* total (particle + neighbour) page capacity; 0 = auto */
     int     device;         /* HIP device ordinal */
     int     precision;      /* mpmx_precision */
-    double  svd_gap;        /* mixed precision: FP64 SVD when min singular-value gap < this (1e-4) */
+    double  svd_gap;        /* FP32 stress arithmetic: FP64 redo when min singular-value gap < this (1e-4) */
     int     cg_check_every; /* host reads the PCG convergence flag every this many iterations (1) */
     int     p2g_variant;    /* 0 = auto (autotuned), 1 = particle-per-lane LDS atomics,
                                2 = cell-gather (lanes own (cell, node column) pairs) */
     int     block;          /* threads per transfer block; 0 = auto (= vpage) */
+    int     stress_fp64;    /* arithmetic of the SVD and return mapping: 1 = FP64 for every
+                               particle (default; state stays in its storage precision),
+                               0 = FP32, redone in FP64 where the singular-value gap is
+                               below svd_gap (0 = never)                              */
 } mpmx_config;
 
 typedef struct mpmx_ctx mpmx_ctx;

@@ -144,7 +144,12 @@ void mo_von_mises(const mo_params* p, double F[9], double phase, double* g_io, d
 
         const double tau_dev_tr_norm = sqrt(tau_dev_tr[0] * tau_dev_tr[0] + tau_dev_tr[1] * tau_dev_tr[1] +
                                             tau_dev_tr[2] * tau_dev_tr[2]);
-        const double taudiv = (1.0 / tau_dev_tr_norm) * parameter1 * g * tau_tr_sum * dlambda;
+        /* A trial stress with no deviatoric part (equal singular values) has
+         * no flow direction: the reference evaluates (1/0) * ... and, with
+         * dlambda = 0 under compression, inf * 0 = NaN.  Here it does not flow. */
+        const double taudiv = tau_dev_tr_norm > 0.0
+                                  ? (1.0 / tau_dev_tr_norm) * parameter1 * g * tau_tr_sum * dlambda
+                                  : 0.0;
         double tau_nn[3];
         const double parameter2 = (1.0 / 3.0) * tau_tr_sum;
         for (int a = 0; a < 3; ++a) tau_nn[a] = (tau_dev_tr[a] - tau_dev_tr[a] * taudiv) + parameter2;

@@ -242,6 +242,28 @@ def test_von_mises_rest_and_rotation(oracle):
     assert np.abs(r["sigma"]).max() < 1e-8 and np.allclose(r["F"], R)
 
 
+def test_von_mises_isotropic_state_is_a_fixed_point(oracle):
+    """F = s R past the yield surface: the yield test on the full trial norm
+    fires, but the trial stress has no deviatoric part, so the return mapping
+    leaves F alone (the reference divides by the zero deviatoric norm and
+    returns NaN).  The stress is the closed form V0 s tau_vol I, with
+    tau_vol = kappa/2 (J^2 - 1) and J = s^3, times g under tension."""
+    p = oracle.default_params(E=1e5, nu=0.3, V0=1.0, yield_stress=1000.0)
+    kappa = 1e5 / (3 * (1 - 2 * 0.3))
+    ang = 0.4
+    R = np.array([[np.cos(ang), 0, -np.sin(ang)], [0, 1, 0], [np.sin(ang), 0, np.cos(ang)]])
+    for s, rot in ((0.99, R), (0.99, np.eye(3)), (1.01, np.eye(3))):
+        F = s * rot
+        c = 0.9
+        r = oracle.von_mises(p, F, c, 1.0, 0.0, 0.0)
+        assert np.isfinite(r["F"]).all() and np.isfinite(r["sigma"]).all()
+        assert np.abs(r["F"] - F).max() < 1e-12
+        J = s ** 3
+        g = c * c * (1 - p.theta) + p.theta
+        tv = kappa / 2 * (J * J - 1) * (g if J >= 1 else 1.0)
+        assert np.abs(r["sigma"] - s * tv * np.eye(3)).max() < 1e-9 * abs(s * tv)
+
+
 def test_von_mises_small_strain_is_linear_elasticity(oracle):
     """For F = I + eps (symmetric, small) and an intact particle, the Kirchhoff
     stress is V0 (lambda tr(eps) I + 2 mu eps) to first order (kappa = lambda + 2mu/3)."""
