@@ -89,6 +89,8 @@ def lib():
         L.mo_sim_set.restype = C.c_int
         L.mo_sim_get_order.argtypes = [C.c_void_p, i32p, u64p]
         L.mo_sim_counts.argtypes = [C.c_void_p, i32p]
+        L.mo_sim_round_f32.argtypes = [C.c_void_p]
+        L.mo_sim_round_f32.restype = None
         L.mo_sim_cg_norm.argtypes = [C.c_void_p]
         L.mo_sim_cg_norm.restype = C.c_double
         L.mo_sim_grid.argtypes = [C.c_void_p, C.c_int, C.c_int, f64p]
@@ -260,6 +262,10 @@ class Sim:
         k = np.zeros(self.n, np.uint64)
         lib().mo_sim_get_order(self.h, o, k)
         return o, k
+
+    def round_f32(self):
+        """Round the particle state to float32 (an FP32 store)."""
+        lib().mo_sim_round_f32(self.h)
 
     def counts(self):
         c = np.zeros(6, np.int32)

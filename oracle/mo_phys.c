@@ -772,6 +772,20 @@ int mo_sim_set(mo_sim* s, int f, const double* in) {
     if (f == MO_X) resort(s);
     return 0;
 }
+/* Round the particle state to float32, as a store in FP32 would: used to
+ * measure how far storage rounding alone moves a trajectory. */
+void mo_sim_round_f32(mo_sim* s) {
+    double* f3[] = {s->x, s->v};
+    double* f9[] = {s->F, s->B};
+    double* f1[] = {s->c, s->g, s->alpha, s->psi};
+    for (int k = 0; k < 2; ++k)
+        for (size_t i = 0; i < (size_t)s->n * 3; ++i) f3[k][i] = (double)(float)f3[k][i];
+    for (int k = 0; k < 2; ++k)
+        for (size_t i = 0; i < (size_t)s->n * 9; ++i) f9[k][i] = (double)(float)f9[k][i];
+    for (int k = 0; k < 4; ++k)
+        for (size_t i = 0; i < (size_t)s->n; ++i) f1[k][i] = (double)(float)f1[k][i];
+}
+
 int mo_sim_get_order(const mo_sim* s, int* order, uint64_t* keys) {
     if (order) memcpy(order, s->order, sizeof(int) * (size_t)s->n);
     if (keys) memcpy(keys, s->keys, sizeof(uint64_t) * (size_t)s->n);

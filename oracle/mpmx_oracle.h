@@ -113,6 +113,8 @@ enum { MO_X = 0, MO_V = 1, MO_F = 2, MO_B = 3, MO_C = 4, MO_ALPHA = 5, MO_PSI = 
  * column-major per particle; scalars: 1) */
 int     mo_sim_get(const mo_sim* s, int field, double* out);
 int     mo_sim_set(mo_sim* s, int field, const double* in);
+/* round the particle state (x, v, F, B, c, g, alpha, psi) to float32 */
+void    mo_sim_round_f32(mo_sim* s);
 /* current sorted order: order[new] = original id; keys sorted */
 int     mo_sim_get_order(const mo_sim* s, int* order, uint64_t* keys);
 /* counts[0..5] = numPage, numTotalPage, numVirtualPage, numCell, cg_iters, cg_breakdown */

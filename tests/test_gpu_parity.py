@@ -5,7 +5,11 @@ Bars (stated per test):
   - per-particle constitutive update on identical inputs: FP32 vs FP64,
     relative to the field's scale;
   - P2G grid sums on identical inputs: FP32 atomics vs FP64 serial sums;
-  - multi-substep trajectories: per-field tolerances from FP32 rounding growth.
+  - one substep from a yielded, damaged GPU state loaded into the oracle:
+    per-field bounds from FP32 rounding (the strict test of every kernel);
+  - multi-substep trajectories: x within 1e-4 of its scale; v and F within
+    3x of what FP32 storage alone does to the oracle's trajectory, since the
+    model amplifies rounding in yielded material.
 """
 import numpy as np
 import pytest
@@ -158,12 +162,13 @@ def test_one_step_from_gpu_state(mpmx, oracle, transfer):
 
 
 def sensitivity(oracle, p, x, v, m, dt, steps, ref):
-    """How far the oracle itself moves when every initial coordinate is moved
-    by one float32 ulp: the conditioning of the trajectory."""
-    xp = np.nextafter(x.astype(np.float32), np.float32(1)).astype(np.float64)
-    sim = oracle.Sim(oracle_params(oracle, p), xp, v, m.astype(np.float32))
+    """How far the oracle itself moves when its particle state is rounded to
+    float32 after every substep (FP32 storage, FP64 arithmetic): the
+    conditioning of the trajectory under the rounding an FP32 build must do."""
+    sim = oracle.Sim(oracle_params(oracle, p), x.astype(np.float32), v, m.astype(np.float32))
     for _ in range(steps):
         sim.substep(dt)
+        sim.round_f32()
     return {k: max_rel_field(sim.get(k), ref.get(k)) for k in ("x", "v", "F")}
 
 
@@ -188,8 +193,9 @@ def test_trajectory_cube(mpmx, oracle):
 
 def trajectory_check(mpmx, oracle, steps, **over):
     """60 substeps of the impact: x within 1e-4 of its scale; v and F, which
-    the model amplifies in yielded material, within 10x the oracle's own
-    one-ulp sensitivity (or 1e-4, whichever is larger)."""
+    the model amplifies in yielded material, within 3x the distance between
+    the oracle and the oracle with FP32 state storage (or 1e-4, whichever is
+    larger)."""
     ctx, sim = trajectory(mpmx, oracle, 3, 0.1, steps, **over)
     p, x, v, m, dt = mpmx.scene(3, seed=1, scale=0.1)
     for k, val in over.items():
@@ -197,12 +203,12 @@ def trajectory_check(mpmx, oracle, steps, **over):
     env = sensitivity(oracle, p, x, v, m, dt, steps, sim)
     errs = {k: max_rel_field(ctx.get(k), sim.get(k)) for k in ("x", "v", "F")}
     dc = np.abs(ctx.get("c") - sim.get("c")).max()
-    print("trajectory %s: gpu-oracle %s  oracle 1-ulp envelope %s  dc %.1e" % (
+    print("trajectory %s: gpu-oracle %s  oracle fp32-storage envelope %s  dc %.1e" % (
         over, {k: "%.1e" % e for k, e in errs.items()}, {k: "%.1e" % e for k, e in env.items()}, dc))
     assert ctx.stats()["cg_breakdown"] == 0
     assert errs["x"] < 1e-4
     for k in ("v", "F"):
-        assert errs[k] < max(1e-4, 10 * env[k]), k
+        assert errs[k] < max(1e-4, 3 * env[k]), k
     assert dc < 1e-2
 
 
