@@ -10,7 +10,7 @@ region (inputs resident in HBM; HIP events on the context stream bracket it,
 with a device synchronisation on both sides).  `roofline` comes from a second,
 instrumented pass of the same scene in which every stage is bracketed by HIP
 events; achieved = algorithmic bytes of the P2G kernel (104 B per particle +
-56 B per active grid node) / its average launch time.  `cpu_baseline` runs the
+52 B per grid node) / its average launch time.  `cpu_baseline` runs the
 serial FP64 oracle on the host for a bounded sample of the same scene.
 """
 import argparse
@@ -109,12 +109,19 @@ def main():
     ctx.run(args.profile_steps, dt)
     kt = ctx.timing_end()
     st2 = ctx.stats()
-    p2g_launches, p2g_ms = kt.get("p2g", (0, 0.0))
+    # dominant transfer kernel alone (the block P2G: 8-channel sums, grid update
+    # and PCG set-up); algorithmic bytes: 104 B of particle state read per
+    # particle, 32 B of grid channels + 20 B of PCG vectors written per node
+    kname = "p2g_block" if "p2g_block" in kt else "p2g"
+    p2g_launches, p2g_ms = kt.get(kname, (0, 0.0))
     G = 64 * st2["numTotalPage"]
-    p2g_bytes = 104.0 * n + 56.0 * G
+    p2g_bytes = 104.0 * n + 52.0 * G
     achieved = p2g_bytes / (p2g_ms / p2g_launches * 1e-3) / 1e9 if p2g_launches else 0.0
-    g2p_launches, g2p_ms = kt.get("g2p", (0, 0.0))
-    g2p_bytes = 156.0 * n + 16.0 * G
+    # G2P: reads x, F, c (52 B) and writes x, v, B, F, c, key (104 B) per
+    # particle; reads v, c, sum w and x of the PCG (24 B) per node
+    gname = "g2p_block" if "g2p_block" in kt else "g2p"
+    g2p_launches, g2p_ms = kt.get(gname, (0, 0.0))
+    g2p_bytes = 156.0 * n + 24.0 * G
     g2p_gbs = g2p_bytes / (g2p_ms / g2p_launches * 1e-3) / 1e9 if g2p_launches else 0.0
 
     out = {
@@ -129,13 +136,14 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else "bf16/fp16 storage, f32 compute",
+        "stress_arith": "f64",
         "data": "synthetic (seeded SplitMix64 scene generator)",
         "config": {"workload": WORKLOADS.get(args.scene, "scene %d" % args.scene), "particles": n,
                    "grid": "%d^3" % p.N, "vpage": p.vpage, "precision": args.precision,
                    "pages_total": st["numTotalPage"], "active_nodes": G,
                    "cg_iters_last": st["cg_iters"], "wall_s": wall},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "p2g",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kname,
                      "algorithmic_bytes_per_launch": p2g_bytes,
                      "avg_launch_ms": p2g_ms / p2g_launches if p2g_launches else None,
                      "g2p_gbs": g2p_gbs},

@@ -16,6 +16,110 @@ struct Tables {
     int64_t cap_p, cap_tp, cap_vp;
 };
 
+// tables of the 2x2x2-page block kernels
+struct BlockTables {
+    const uint64_t* pmask;
+    const uint64_t* nmask;
+    const int* wpre;           // page rank prefix sums (particle, neighbour-only)
+    const int* spre;
+    const int* page2particle;
+    const uint32_t* keys;      // sorted particle keys
+    const uint32_t* block_key;
+    int64_t cap_p, cap_tp, cap_blocks;
+    int axis_pages;            // page coordinates lie in [0, axis_pages)
+};
+
+inline BlockTables block_tables_of(const mpmx_ctx* c) {
+    return BlockTables{c->pmask, c->nmask, c->wcount, c->scount, c->page2particle, c->keys, c->block_key,
+                       c->cap_p, c->cap_tp, c->tp_bound, 1 << c->axis_bits};
+}
+
+// ---- shared pieces of the single-wave block gathers --------------------------
+// The particle pages 2b-1 .. 2b+1 (per axis) of block b, their particle ranges
+// and the cell filter that keeps the particles reaching the block; lanes 32..39
+// resolve the block's 8 grid pages.  One wavefront; LDS arrays owned by the caller.
+struct BlockSources {
+    int start[27], pre[28], filter[27];
+    int out_page[8];
+};
+
+__device__ __forceinline__ void block_sources(const BlockTables& t, uint32_t bkey, const int (&bc)[3], int n,
+                                              int np_raw, int np, int nt, BlockSources& S) {
+    const int lane = threadIdx.x;
+    int cnt = 0;
+    if (lane < 27) {
+        const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
+        const int px = 2 * bc[0] + dx, py = 2 * bc[1] + dy, pz = 2 * bc[2] + dz;
+        int st = 0;
+        if (px >= 0 && py >= 0 && pz >= 0 && px < t.axis_pages && py < t.axis_pages && pz < t.axis_pages) {
+            const int id = page_rank(morton((uint32_t)px, (uint32_t)py, (uint32_t)pz), t.pmask, t.nmask, t.wpre,
+                                     t.spre, np_raw);
+            if (id >= 0 && id < np) {
+                int a = t.page2particle[id], b = t.page2particle[id + 1];
+                a = a < 0 ? 0 : (a > n ? n : a);
+                b = b < a ? a : (b > n ? n : b);
+                st = a;
+                cnt = b - a;
+            }
+        }
+        S.start[lane] = st;
+        S.filter[lane] = (dx < 0 ? 1 : 0) | (dy < 0 ? 2 : 0) | (dz < 0 ? 4 : 0);
+    } else if (lane >= 32 && lane < 40) {
+        const int o = lane - 32;
+        const int id = page_rank((bkey << 3) | (uint32_t)o, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
+        S.out_page[o] = (id >= 0 && id < nt) ? id : -1;
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        const int up = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += up;
+    }
+    if (lane < 27) S.pre[lane] = incl - cnt;
+    if (lane == 26) S.pre[27] = incl;
+}
+
+// entry j of the concatenated source ranges: particle index, whether it
+// reaches the block, and its rank inside its run of equal keys within the
+// wave (lanes of one rank never share a cell); rmax is the wave's largest rank
+__device__ __forceinline__ bool block_entry(const BlockTables& t, const BlockSources& S, int j, int total, int& i,
+                                            int& rank, int& rmax) {
+    const int lane = threadIdx.x;
+    bool valid = j < total;
+    uint32_t key = 0xffffffffu;
+    i = 0;
+    if (valid) {
+        int lo = 0, hi = 26;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (S.pre[mid] <= j) lo = mid;
+            else hi = mid - 1;
+        }
+        i = S.start[lo] + (j - S.pre[lo]);
+        key = t.keys[i];
+        const int f = S.filter[lo];
+        const uint32_t cell = key & 63u;
+        if (((f & 1) && !((cell >> 5) & 1u)) || ((f & 2) && !((cell >> 3) & 1u)) || ((f & 4) && !((cell >> 1) & 1u)))
+            valid = false;
+    }
+    // particles of one cell are adjacent; filtered lanes never split a run
+    const uint32_t kprev = __shfl_up(key, 1, 64);
+    const bool start = valid && (lane == 0 || kprev != key);
+    const unsigned long long starts = __ballot(start);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long sb = starts & upto;
+    rank = valid ? lane - (63 - __clzll((long long)sb)) : 0;
+    rmax = rank;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) rmax = max(rmax, __shfl_xor(rmax, o, 64));
+    return valid;
+}
+
+// node index inside an 8^3 block accumulator; z padded to 9 so that the nodes
+// of one page spread over the LDS banks
+constexpr int kAccZ = 9;
+constexpr int kAccStride = 8 * 8 * kAccZ;   // floats per channel
+
 __device__ __forceinline__ float ld_x(const PState& s, int i, int a) { return s.x[3 * i + a]; }
 __device__ __forceinline__ float ld_x(const MState& s, int i, int a) { return s.x[3 * i + a]; }
 __device__ __forceinline__ void st_x(const PState& s, int i, int a, float f) { s.x[3 * i + a] = f; }

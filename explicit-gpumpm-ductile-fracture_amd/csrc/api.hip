@@ -202,7 +202,9 @@ static void make_dev_params(mpmx_ctx* c) {
     DevParams& d = c->dp;
     d.N = P.N;
     d.transfer = P.transfer;
-    d.phase_field = P.phase_field;
+    // the fixed corotated energy has no fracture: integrate skips the solve (Energy_Model 0)
+    d.phase_field = (P.phase_field && P.model != 0) ? 1 : 0;
+    d.model = P.model;
     d.dot_mode = P.dot_mode;
     d.wall = P.wall_cells;
     d.grip_lo = P.grip_lo;
@@ -269,7 +271,7 @@ static void free_particles(mpmx_ctx* c) {
         dfree(m.x); dfree(m.v); dfree(m.B); dfree(m.F); dfree(m.c); dfree(m.alpha); dfree(m.psi); dfree(m.g);
         dfree(m.mass); dfree(m.id);
     }
-    dfree(c->sigma); dfree(c->FP); dfree(c->vol);
+    dfree(c->sigma); dfree(c->FP); dfree(c->vol); dfree(c->cg_col);
     dfree(c->keys); dfree(c->keys_alt); dfree(c->vals); dfree(c->vals_alt); dfree(c->hist);
     dfree(c->pmask); dfree(c->nmask); dfree(c->wcount); dfree(c->scount);
     dfree(c->page_key); dfree(c->page2particle); dfree(c->adj); dfree(c->nvp); dfree(c->vpo); dfree(c->target); dfree(c->block_key); dfree(c->bcount);
@@ -334,6 +336,7 @@ void mpmx_default_params(mpmx_params* p) {
     p->vmax = (double)11.137f;
     p->cg_tol = (double)1e-6f;
     p->crit_stretch = 0;
+    p->model = 2;
 }
 
 void mpmx_default_config(mpmx_config* c) {
@@ -372,6 +375,7 @@ mpmx_status mpmx_create(const mpmx_params* p, const mpmx_config* cfg, mpmx_ctx**
     if (p->N < 16 || p->N > 1024 || (p->N & 3)) return MPMX_ERR_ARG;
     if (p->vpage != 64 && p->vpage != 128 && p->vpage != 256) return MPMX_ERR_ARG;
     if (p->transfer < 0 || p->transfer > 1) return MPMX_ERR_ARG;
+    if (p->model < 0 || p->model > 2) return MPMX_ERR_ARG;
     if (mpmx_device_count() <= 0) return MPMX_ERR_NO_DEVICE;
     mpmx_ctx* c = new mpmx_ctx();
     c->P = *p;
@@ -459,6 +463,7 @@ mpmx_status mpmx_set_particles(mpmx_ctx* c, int64_t n, const double* x, const do
     HIPCHK(c, dalloc(&c->sigma, 9 * n));
     HIPCHK(c, dalloc(&c->FP, n));
     HIPCHK(c, dalloc(&c->vol, n));
+    HIPCHK(c, dalloc(&c->cg_col, 3 * n));
     HIPCHK(c, dalloc(&c->keys, n));
     HIPCHK(c, dalloc(&c->keys_alt, n));
     HIPCHK(c, dalloc(&c->vals, n));

@@ -8,16 +8,22 @@
 //   grid  undateGrid_kernal + collideWithGround / PullGrid (TimeIntegrator.cu:
 //         63-261) and the PCG initialisation of ExplicitTimeIntegrator.cu:914-935.
 //
-// Gather form: a workgroup owns one 2x2x2 block of grid pages (8^3 nodes, the
-// pages 8b .. 8b+7 in Morton order) at a time.  Particles of page P reach the
-// nodes of P and its +1 neighbours, so block b receives from the 27 particle
-// pages 2b-1 .. 2b+1 (per axis); of a page below the block along an axis only
-// the particles whose cell has index & 3 >= 2 on that axis reach it.  All 256
-// threads walk the concatenated particle ranges of those pages, accumulate the
-// 8 channels of the 512 nodes in LDS with LDS float atomics, and the
-// workgroup then updates its nodes in place and writes them once: no global
-// atomics and no zero fill of the grid.  Each particle is read by the one to
-// eight blocks it reaches (about twice on average).
+// Gather form: a workgroup of one wavefront owns one 2x2x2 block of grid
+// pages (8^3 nodes, the pages 8b .. 8b+7 in Morton order) at a time.
+// Particles of page P reach the nodes of P and its +1 neighbours, so block b
+// receives from the 27 particle pages 2b-1 .. 2b+1 (per axis); of a page below
+// the block along an axis only the particles whose cell has index & 3 >= 2 on
+// that axis reach it.  The wave walks the concatenated particle ranges of
+// those pages and adds each particle's 8 channels to the block's nodes in LDS
+// with plain read-add-write: LDS float atomics cost ~190 CU cycles per wave
+// instruction on gfx950 against ~18 for a read-add-write, and with one wave
+// per block no other wave touches the accumulator.  Lanes of one wave never
+// hit the same node in the same instruction: particles of one cell (adjacent
+// after the sort) are spread over rounds, and a compiler barrier between
+// stencil slots keeps the in-order LDS semantics across lanes.  The wave
+// then updates its nodes in place and writes them once: no atomics at all,
+// no zero fill of the grid.  Each particle is read by the one to eight blocks
+// it reaches (about twice on average).
 #include "access.h"
 
 namespace mpmx {
@@ -80,106 +86,100 @@ __global__ void k_cg_reset(DevParams dp, DevCounts* dc) {
     dc->cg_norm = 0.f;
 }
 
-// contributions of particle i to the R^3 nodes starting at node org (node
-// coordinates): the MLS / APIC momentum with the stress term, mass, sum w c,
-// sum w, sum V w and sum V FP w, added to acc[ch R^3 + (a R + b) R + c] with
-// LDS atomics
-template <int R, int TRANSFER, typename ST>
-__device__ __forceinline__ void p2g_particle(const DevParams& dp, const ST& s, const float* __restrict__ sigma,
-                                             const float* __restrict__ FP, const float* __restrict__ vol, int i,
-                                             int n, float dt, const int (&org)[3], float* acc) {
-    constexpr int R3 = R * R * R;
-    float d0[3], w[3][3], gw[3][3];
+// per-particle P2G terms, computed once and kept in registers across rounds
+struct P2GTerms {
+    float w[3][3], gw[3][3];
     int l0[3];
+    float m, cphase, V, VF;
+    float ar[3], Qd[9], sg[9];
+};
+
+template <int TRANSFER, typename ST>
+__device__ __forceinline__ void p2g_terms(const DevParams& dp, const ST& s, const float* __restrict__ sigma,
+                                          const float* __restrict__ FP, const float* __restrict__ vol, int i,
+                                          int n, float dt, const int (&org)[3], P2GTerms& T) {
+    float d0[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const float xa = ld_x(s, i, a);
         const int b = base_node(xa, dp.N);
         d0[a] = xa * dp.inv_dx - (float)b;
-        l0[a] = b - org[a];
-        if (TRANSFER == 0) bspline(d0[a], w[a]);
-        else bspline_grad(d0[a], w[a], gw[a]);
+        T.l0[a] = b - org[a];
+        if (TRANSFER == 0) bspline(d0[a], T.w[a]);
+        else bspline_grad(d0[a], T.w[a], T.gw[a]);
     }
-    const float m = ld_m(s, i);
-    const float cphase = ld_c(s, i);
-    const float V = vol[i];
-    const float VF = V * FP[i];
+    T.m = ld_m(s, i);
+    T.cphase = ld_c(s, i);
+    T.V = vol[i];
+    T.VF = T.V * FP[i];
     float v[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) v[a] = ld_v(s, i, a);
-    // affine matrix per node offset in units of dx: p_r = w (a_r + sum_c Qd(r,c) o_c)
-    float Qd[9], sg[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const float Bk = ld_B(s, i, k, n);
         const float sk = sigma[(int64_t)k * n + i];
-        if (TRANSFER == 0) Qd[k] = (Bk * m - sk * dt) * dp.Dinv * dp.dx;
-        else { Qd[k] = Bk * dp.Dinv * dp.dx * m; sg[k] = sk * dt * dp.inv_dx; }
+        if (TRANSFER == 0) T.Qd[k] = (Bk * T.m - sk * dt) * dp.Dinv * dp.dx;
+        else { T.Qd[k] = Bk * dp.Dinv * dp.dx * T.m; T.sg[k] = sk * dt * dp.inv_dx; }
     }
-    float ar[3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) ar[r] = m * v[r] - (Qd[r] * d0[0] + Qd[r + 3] * d0[1] + Qd[r + 6] * d0[2]);
+    for (int r = 0; r < 3; ++r) T.ar[r] = T.m * v[r] - (T.Qd[r] * d0[0] + T.Qd[r + 3] * d0[1] + T.Qd[r + 6] * d0[2]);
+}
+
+// add the particle's contributions to the 8^3 block accumulator (read-add-
+// write; the caller guarantees that no two active lanes share a cell)
+template <int TRANSFER>
+__device__ __forceinline__ void p2g_rmw(const P2GTerms& T, float* acc) {
 #pragma unroll
     for (int ii = 0; ii < 3; ++ii) {
-        const int la = l0[0] + ii;
-        if (la < 0 || la >= R) continue;
+        const int la = T.l0[0] + ii;
+        if (la < 0 || la >= 8) continue;
 #pragma unroll
         for (int jj = 0; jj < 3; ++jj) {
-            const int lb = l0[1] + jj;
-            if (lb < 0 || lb >= R) continue;
-            const float wxy = w[0][ii] * w[1][jj];
+            const int lb = T.l0[1] + jj;
+            if (lb < 0 || lb >= 8) continue;
+            const float wxy = T.w[0][ii] * T.w[1][jj];
 #pragma unroll
             for (int kk = 0; kk < 3; ++kk) {
-                const int lc = l0[2] + kk;
-                if (lc < 0 || lc >= R) continue;
-                const float wt = wxy * w[2][kk];
+                const int lc = T.l0[2] + kk;
+                if (lc < 0 || lc >= 8) continue;
+                const float wt = wxy * T.w[2][kk];
                 float pr[3];
 #pragma unroll
-                for (int r = 0; r < 3; ++r) pr[r] = wt * (ar[r] + Qd[r] * ii + Qd[r + 3] * jj + Qd[r + 6] * kk);
+                for (int r = 0; r < 3; ++r)
+                    pr[r] = wt * (T.ar[r] + T.Qd[r] * ii + T.Qd[r + 3] * jj + T.Qd[r + 6] * kk);
                 if (TRANSFER == 1) {
-                    // + dt * (-sigma grad w), gradient in units of 1/dx
-                    const float g0 = gw[0][ii] * w[1][jj] * w[2][kk];
-                    const float g1 = w[0][ii] * gw[1][jj] * w[2][kk];
-                    const float g2 = w[0][ii] * w[1][jj] * gw[2][kk];
+                    const float g0 = T.gw[0][ii] * T.w[1][jj] * T.w[2][kk];
+                    const float g1 = T.w[0][ii] * T.gw[1][jj] * T.w[2][kk];
+                    const float g2 = T.w[0][ii] * T.w[1][jj] * T.gw[2][kk];
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) pr[r] -= sg[r] * g0 + sg[r + 3] * g1 + sg[r + 6] * g2;
+                    for (int r = 0; r < 3; ++r) pr[r] -= T.sg[r] * g0 + T.sg[r + 3] * g1 + T.sg[r + 6] * g2;
                 }
-                const int node = (la * R + lb) * R + lc;
-                atomicAdd(&acc[0 * R3 + node], wt * m);
-                atomicAdd(&acc[1 * R3 + node], pr[0]);
-                atomicAdd(&acc[2 * R3 + node], pr[1]);
-                atomicAdd(&acc[3 * R3 + node], pr[2]);
-                atomicAdd(&acc[4 * R3 + node], wt * cphase);
-                atomicAdd(&acc[5 * R3 + node], wt);
-                atomicAdd(&acc[6 * R3 + node], wt * V);
-                atomicAdd(&acc[7 * R3 + node], wt * VF);
+                float* a = acc + (la * 8 + lb) * kAccZ + lc;
+                // all eight reads, then all eight writes: one LDS round trip
+                float o[kP2GCh];
+#pragma unroll
+                for (int k = 0; k < kP2GCh; ++k) o[k] = a[k * kAccStride];
+                const float add[kP2GCh] = {wt * T.m, pr[0], pr[1], pr[2], wt * T.cphase, wt, wt * T.V, wt * T.VF};
+#pragma unroll
+                for (int k = 0; k < kP2GCh; ++k) a[k * kAccStride] = o[k] + add[k];
+                // another lane may add to this node in a later slot
+                __asm__ volatile("" ::: "memory");
             }
         }
     }
 }
 
-struct BlockTables {
-    const uint64_t* pmask;
-    const uint64_t* nmask;
-    const int* wpre;           // page rank prefix sums (particle, neighbour-only)
-    const int* spre;
-    const int* page2particle;
-    const uint32_t* keys;      // sorted particle keys
-    const uint32_t* block_key;
-    int64_t cap_p, cap_tp, cap_blocks;
-    int axis_pages;            // page coordinates lie in [0, axis_pages)
-};
 
 template <int TRANSFER, typename ST>
-__global__ __launch_bounds__(256) void k_p2g_block(DevParams dp, ST s, const float* __restrict__ sigma,
-                                                   const float* __restrict__ FP, const float* __restrict__ vol,
-                                                   BlockTables t, float* __restrict__ grid,
-                                                   float* __restrict__ grid_raw, CGVec cg, float dt, int n,
-                                                   const DevCounts* __restrict__ dc) {
-    __shared__ float acc[kP2GCh * 512];
-    __shared__ int src_start[27], src_pre[28], src_cnt[27];
-    __shared__ int src_filter[27];
-    __shared__ int out_page[8];
+__global__ __launch_bounds__(64) void k_p2g_block(DevParams dp, ST s, const float* __restrict__ sigma,
+                                                  const float* __restrict__ FP, const float* __restrict__ vol,
+                                                  BlockTables t, float* __restrict__ grid,
+                                                  float* __restrict__ grid_raw, CGVec cg, float dt, int n,
+                                                  const DevCounts* __restrict__ dc) {
+    __shared__ float acc[kP2GCh * kAccStride];
+    __shared__ BlockSources S;
+    const int lane = threadIdx.x;
     const int np_raw = dc->num_page;
     const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
     int nt = dc->num_total;
@@ -192,72 +192,34 @@ __global__ __launch_bounds__(256) void k_p2g_block(DevParams dp, ST s, const flo
         const uint32_t bkey = t.block_key[blk];
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
-        if (threadIdx.x < 27) {
-            const int q = threadIdx.x;
-            const int dx = q / 9 - 1, dy = (q / 3) % 3 - 1, dz = q % 3 - 1;
-            const int px = 2 * bc[0] + dx, py = 2 * bc[1] + dy, pz = 2 * bc[2] + dz;
-            int st = 0, cnt = 0;
-            if (px >= 0 && py >= 0 && pz >= 0 && px < t.axis_pages && py < t.axis_pages && pz < t.axis_pages) {
-                const int id = page_rank(morton((uint32_t)px, (uint32_t)py, (uint32_t)pz), t.pmask, t.nmask, t.wpre,
-                                         t.spre, np_raw);
-                if (id >= 0 && id < np) {
-                    int a = t.page2particle[id], b = t.page2particle[id + 1];
-                    a = a < 0 ? 0 : (a > n ? n : a);
-                    b = b < a ? a : (b > n ? n : b);
-                    st = a;
-                    cnt = b - a;
-                }
-            }
-            src_start[q] = st;
-            src_cnt[q] = cnt;
-            src_filter[q] = (dx < 0 ? 1 : 0) | (dy < 0 ? 2 : 0) | (dz < 0 ? 4 : 0);
-        } else if (threadIdx.x >= 64 && threadIdx.x < 72) {
-            const int o = threadIdx.x - 64;
-            const int id = page_rank((bkey << 3) | (uint32_t)o, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
-            out_page[o] = (id >= 0 && id < nt) ? id : -1;
-        }
-        for (int q = threadIdx.x; q < kP2GCh * 512; q += 256) acc[q] = 0.f;
+        block_sources(t, bkey, bc, n, np_raw, np, nt, S);
+        for (int q = lane; q < kP2GCh * kAccStride; q += 64) acc[q] = 0.f;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            int run = 0;
-            for (int q = 0; q < 27; ++q) {
-                src_pre[q] = run;
-                run += src_cnt[q];
-            }
-            src_pre[27] = run;
-        }
-        __syncthreads();
-        const int total = src_pre[27];
+        const int total = S.pre[27];
         const int org[3] = {8 * bc[0], 8 * bc[1], 8 * bc[2]};
-        for (int j = threadIdx.x; j < total; j += 256) {
-            int lo = 0, hi = 26;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (src_pre[mid] <= j) lo = mid;
-                else hi = mid - 1;
+        for (int j0 = 0; j0 < total; j0 += 64) {
+            int i, rank, rmax;
+            const bool valid = block_entry(t, S, j0 + lane, total, i, rank, rmax);
+            if (!__any(valid)) continue;
+            P2GTerms T;
+            if (valid) p2g_terms<TRANSFER>(dp, s, sigma, FP, vol, i, n, dt, org, T);
+            for (int r = 0; r <= rmax; ++r) {
+                if (valid && rank == r) p2g_rmw<TRANSFER>(T, acc);
+                __asm__ volatile("" ::: "memory");
             }
-            const int i = src_start[lo] + (j - src_pre[lo]);
-            const int f = src_filter[lo];
-            if (f) {
-                const uint32_t cell = t.keys[i] & 63u;
-                if (((f & 1) && !((cell >> 5) & 1u)) || ((f & 2) && !((cell >> 3) & 1u)) ||
-                    ((f & 4) && !((cell >> 1) & 1u)))
-                    continue;
-            }
-            p2g_particle<8, TRANSFER>(dp, s, sigma, FP, vol, i, n, dt, org, acc);
         }
         __syncthreads();
         // grid update of the block's pages, written page-major (coalesced)
-        for (int q = threadIdx.x; q < 512; q += 256) {
+        for (int q = lane; q < 512; q += 64) {
             const int o = q >> 6, cell = q & 63;
-            const int G = out_page[o];
+            const int G = S.out_page[o];
             if (G < 0) continue;
             const int lx = ((o >> 2) & 1) * 4 + (cell >> 4), ly = ((o >> 1) & 1) * 4 + ((cell >> 2) & 3),
                       lz = (o & 1) * 4 + (cell & 3);
-            const int node = (lx * 8 + ly) * 8 + lz;
+            const int node = (lx * 8 + ly) * kAccZ + lz;
             float ch[kP2GCh];
 #pragma unroll
-            for (int k = 0; k < kP2GCh; ++k) ch[k] = acc[k * 512 + node];
+            for (int k = 0; k < kP2GCh; ++k) ch[k] = acc[k * kAccStride + node];
             if (grid_raw) {
                 float* rp = grid_raw + (int64_t)G * kPageFloats + cell;
 #pragma unroll
@@ -370,18 +332,20 @@ static CGVec cg_vec(mpmx_ctx* c) { return CGVec{c->cg_x, c->cg_r, c->cg_z, c->cg
 
 template <typename ST>
 static void gather_dispatch(mpmx_ctx* c, ST st, float dt) {
-    const BlockTables t{c->pmask, c->nmask, c->wcount, c->scount, c->page2particle, c->keys, c->block_key,
-                        c->cap_p, c->cap_tp, c->tp_bound, 1 << c->axis_bits};
+    const BlockTables t = block_tables_of(c);
     float* raw = (c->keep_p2g && c->grid_p2g) ? c->grid_p2g : nullptr;
+    // one wave per workgroup; ~8 resident per CU (18.4 KiB of LDS each)
     int64_t g = (c->tp_bound + 7) / 8 * 8;
-    g = g < (int64_t)c->num_cu * 8 ? g : (int64_t)c->num_cu * 8;
-    if (c->P.transfer == 0)
-        k_p2g_block<0, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, c->sigma, c->FP, c->vol, t, c->grid, raw,
-                                                          cg_vec(c), dt, (int)c->n, c->dc);
-    else
-        k_p2g_block<1, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, c->sigma, c->FP, c->vol, t, c->grid, raw,
-                                                          cg_vec(c), dt, (int)c->n, c->dc);
-    if (c->P.phase_field)
+    g = g < (int64_t)c->num_cu * 16 ? g : (int64_t)c->num_cu * 16;
+    MPMX_TIMED(c, "p2g_block", ({
+                   if (c->P.transfer == 0)
+                       k_p2g_block<0, ST><<<(int)g, 64, 0, c->stream>>>(c->dp, st, c->sigma, c->FP, c->vol, t, c->grid,
+                                                                         raw, cg_vec(c), dt, (int)c->n, c->dc);
+                   else
+                       k_p2g_block<1, ST><<<(int)g, 64, 0, c->stream>>>(c->dp, st, c->sigma, c->FP, c->vol, t, c->grid,
+                                                                         raw, cg_vec(c), dt, (int)c->n, c->dc);
+               }));
+    if (c->dp.phase_field)
         k_cg_rho0<<<c->cg_nb, 256, 0, c->stream>>>(c->dp, c->cg_r, c->cg_z, c->cg_part + c->cg_nb, c->cap_tp,
                                                     c->dc);
 }

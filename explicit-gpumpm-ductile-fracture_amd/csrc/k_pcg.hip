@@ -124,6 +124,193 @@ __global__ __launch_bounds__(BLOCK) void k_cg_apply(DevParams dp, ST s, const fl
     }
 }
 
+// ---- block form of (K s) (variant 2) ----------------------------------------
+// col_p = M V dx^2 sum_j grad w_jp s_j for every particle of a 2x2x2 page
+// block, from the block's 10^3-node s region staged in LDS
+template <typename ST>
+__global__ __launch_bounds__(256) void k_cg_col(DevParams dp, ST s, const float* __restrict__ vol, BlockTables t,
+                                                const float* __restrict__ cgs, float* __restrict__ col, int n,
+                                                DevCounts* dc) {
+    constexpr int R = 10, R3 = R * R * R;
+    __shared__ float ts[R3];
+    __shared__ int spg[27];
+    __shared__ int prange[2];
+    if (cg_stopped(dc)) return;
+    const int np_raw = dc->num_page;
+    const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
+    int nt = dc->num_total;
+    nt = nt < t.cap_tp ? nt : (int)t.cap_tp;
+    int nb = dc->num_blocks;
+    nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
+    const float para_scale = 4.0f * dp.L0 * dp.L0 * dp.inv_dx * dp.inv_dx;
+    int bb, be;
+    block_range(nb, bb, be);
+    const int tid = threadIdx.x;
+    for (int blk = bb; blk < be; ++blk) {
+        const uint32_t bkey = t.block_key[blk];
+        int bc[3];
+        unmorton(bkey, bc[0], bc[1], bc[2]);
+        if (tid < 27) {
+            const int px = 2 * bc[0] + tid / 9, py = 2 * bc[1] + (tid / 3) % 3, pz = 2 * bc[2] + tid % 3;
+            int id = -1;
+            if (px < t.axis_pages && py < t.axis_pages && pz < t.axis_pages)
+                id = page_rank(morton((uint32_t)px, (uint32_t)py, (uint32_t)pz), t.pmask, t.nmask, t.wpre, t.spre,
+                               np_raw);
+            spg[tid] = (id >= 0 && id < nt) ? id : -1;
+        }
+        if (tid >= 64 && tid < 128) {
+            const int o = tid - 64;
+            int id = -1;
+            if (o < 8) id = page_rank((bkey << 3) | (uint32_t)o, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
+            const bool own = id >= 0 && id < np;
+            int lo = own ? id : 0x7fffffff, hi = own ? id : -1;
+#pragma unroll
+            for (int w = 32; w > 0; w >>= 1) {
+                lo = min(lo, __shfl_xor(lo, w, 64));
+                hi = max(hi, __shfl_xor(hi, w, 64));
+            }
+            if (o == 0) {
+                int a = 0, b = 0;
+                if (hi >= 0) {
+                    a = t.page2particle[lo];
+                    b = t.page2particle[hi + 1];
+                    a = a < 0 ? 0 : (a > n ? n : a);
+                    b = b < a ? a : (b > n ? n : b);
+                }
+                prange[0] = a;
+                prange[1] = b;
+            }
+        }
+        __syncthreads();
+        const int pa = prange[0], pb = prange[1];
+        if (pa < pb) {
+            for (int q = tid; q < R3; q += 256) {
+                const int X = q / 100, Y = (q / 10) % 10, Z = q % 10;
+                const int G = spg[(X >> 2) * 9 + (Y >> 2) * 3 + (Z >> 2)];
+                ts[q] = G >= 0 ? cgs[(int64_t)G * 64 + (((X & 3) << 4) | ((Y & 3) << 2) | (Z & 3))] : 0.f;
+            }
+            __syncthreads();
+            const int org[3] = {8 * bc[0], 8 * bc[1], 8 * bc[2]};
+            for (int i = pa + tid; i < pb; i += 256) {
+                float d0[3], w[3][3], gw[3][3];
+                int l[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float xa = ld_x(s, i, a);
+                    const int b = base_node(xa, dp.N);
+                    d0[a] = xa * dp.inv_dx - (float)b;
+                    const int la = b - org[a];
+                    l[a] = la < 0 ? 0 : (la > R - 3 ? R - 3 : la);
+                    bspline_grad(d0[a], w[a], gw[a]);
+                }
+                float gs[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                        for (int kk = 0; kk < 3; ++kk) {
+                            const float sv = ts[((l[0] + ii) * R + (l[1] + jj)) * R + (l[2] + kk)];
+                            gs[0] += gw[0][ii] * w[1][jj] * w[2][kk] * sv;
+                            gs[1] += w[0][ii] * gw[1][jj] * w[2][kk] * sv;
+                            gs[2] += w[0][ii] * w[1][jj] * gw[2][kk] * sv;
+                        }
+                const float sc = vol[i] * para_scale * dp.M;
+                col[3 * (int64_t)i + 0] = gs[0] * sc;
+                col[3 * (int64_t)i + 1] = gs[1] * sc;
+                col[3 * (int64_t)i + 2] = gs[2] * sc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// q_i = sum_p grad w_ip . col_p + D_i s_i over the 8^3 nodes of a block: a
+// single-wave gather, read-add-write in LDS with the same-cell rounds of the
+// block P2G; q is written whole (no zero fill, no atomics)
+template <typename ST>
+__global__ __launch_bounds__(64) void k_cg_q(DevParams dp, ST s, BlockTables t, const float* __restrict__ col,
+                                             const float* __restrict__ grid, const float* __restrict__ cgs,
+                                             float* __restrict__ cgq, int n, DevCounts* dc) {
+    __shared__ float acc[kAccStride];
+    __shared__ BlockSources S;
+    if (cg_stopped(dc)) return;
+    const int lane = threadIdx.x;
+    const int np_raw = dc->num_page;
+    const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
+    int nt = dc->num_total;
+    nt = nt < t.cap_tp ? nt : (int)t.cap_tp;
+    int nb = dc->num_blocks;
+    nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
+    int bb, be;
+    block_range(nb, bb, be);
+    for (int blk = bb; blk < be; ++blk) {
+        const uint32_t bkey = t.block_key[blk];
+        int bc[3];
+        unmorton(bkey, bc[0], bc[1], bc[2]);
+        block_sources(t, bkey, bc, n, np_raw, np, nt, S);
+        for (int q = lane; q < kAccStride; q += 64) acc[q] = 0.f;
+        __syncthreads();
+        const int total = S.pre[27];
+        const int org[3] = {8 * bc[0], 8 * bc[1], 8 * bc[2]};
+        for (int j0 = 0; j0 < total; j0 += 64) {
+            int i, rank, rmax;
+            const bool valid = block_entry(t, S, j0 + lane, total, i, rank, rmax);
+            if (!__any(valid)) continue;
+            float w[3][3], gw[3][3], c3[3];
+            int l0[3];
+            if (valid) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float xa = ld_x(s, i, a);
+                    const int b = base_node(xa, dp.N);
+                    const float d0 = xa * dp.inv_dx - (float)b;
+                    l0[a] = b - org[a];
+                    bspline_grad(d0, w[a], gw[a]);
+                    c3[a] = col[3 * (int64_t)i + a];
+                }
+            }
+            for (int r = 0; r <= rmax; ++r) {
+                if (valid && rank == r) {
+#pragma unroll
+                    for (int ii = 0; ii < 3; ++ii) {
+                        const int la = l0[0] + ii;
+                        if (la < 0 || la >= 8) continue;
+#pragma unroll
+                        for (int jj = 0; jj < 3; ++jj) {
+                            const int lb = l0[1] + jj;
+                            if (lb < 0 || lb >= 8) continue;
+#pragma unroll
+                            for (int kk = 0; kk < 3; ++kk) {
+                                const int lc = l0[2] + kk;
+                                if (lc < 0 || lc >= 8) continue;
+                                const float hw = gw[0][ii] * w[1][jj] * w[2][kk] * c3[0] +
+                                                 w[0][ii] * gw[1][jj] * w[2][kk] * c3[1] +
+                                                 w[0][ii] * w[1][jj] * gw[2][kk] * c3[2];
+                                acc[(la * 8 + lb) * kAccZ + lc] += hw;
+                                __asm__ volatile("" ::: "memory");
+                            }
+                        }
+                    }
+                }
+                __asm__ volatile("" ::: "memory");
+            }
+        }
+        __syncthreads();
+        for (int q = lane; q < 512; q += 64) {
+            const int o = q >> 6, cell = q & 63;
+            const int G = S.out_page[o];
+            if (G < 0) continue;
+            const int lx = ((o >> 2) & 1) * 4 + (cell >> 4), ly = ((o >> 1) & 1) * 4 + ((cell >> 2) & 3),
+                      lz = (o & 1) * 4 + (cell & 3);
+            const int64_t nn = (int64_t)G * 64 + cell;
+            const float D = grid[(int64_t)G * kPageFloats + 7 * 64 + cell];
+            cgq[nn] = acc[(lx * 8 + ly) * kAccZ + lz] + cgs[nn] * D;
+        }
+        __syncthreads();
+    }
+}
+
 // helper: grid-stride over 128-node blocks, one wave each, two nodes per lane
 #define FOR_128_BLOCKS(nt)                                                   \
     const int lane = threadIdx.x & 63;                                       \
@@ -132,10 +319,11 @@ __global__ __launch_bounds__(BLOCK) void k_cg_apply(DevParams dp, ST s, const fl
     const int nblk = ((nt) + 1) / 2;                                         \
     for (int blk = wave; blk < nblk; blk += nwaves)
 
-// q = q + s D; partial of "dot"(s, q)
+// q = q + s D (page-tile variant; the block variant adds s D in k_cg_q);
+// partial of "dot"(s, q)
 __global__ __launch_bounds__(256) void k_cg_dot(DevParams dp, const float* __restrict__ grid,
                                                 const float* __restrict__ cgs, float* __restrict__ cgq,
-                                                float* __restrict__ part_sq, int64_t cap_tp,
+                                                float* __restrict__ part_sq, int add_diag, int64_t cap_tp,
                                                 const DevCounts* __restrict__ dc) {
     __shared__ float lds[4];
     if (cg_stopped(dc)) return;
@@ -147,12 +335,14 @@ __global__ __launch_bounds__(256) void k_cg_dot(DevParams dp, const float* __res
         const int n0 = blk * 128 + lane * 2;
         float part = 0.f;
         if ((n0 >> 6) < nt) {
-            const float* D = grid + (int64_t)(n0 >> 6) * kPageFloats + 7 * 64 + (n0 & 63);
             const float2 sv = *(const float2*)(cgs + n0);
             float2 qv = *(const float2*)(cgq + n0);
-            qv.x += sv.x * D[0];
-            qv.y += sv.y * D[1];
-            *(float2*)(cgq + n0) = qv;
+            if (add_diag) {
+                const float* D = grid + (int64_t)(n0 >> 6) * kPageFloats + 7 * 64 + (n0 & 63);
+                qv.x += sv.x * D[0];
+                qv.y += sv.y * D[1];
+                *(float2*)(cgq + n0) = qv;
+            }
             part = sv.x * qv.x + sv.y * qv.y;
         }
         const float bs = wave_sum(part);
@@ -266,13 +456,30 @@ static void apply_block(mpmx_ctx* c, ST st) {
 }
 
 // one PCG iteration k: four launches, all no-ops once the device flag says done
+template <typename ST>
+static void apply_blocks(mpmx_ctx* c, ST st) {
+    const BlockTables t = block_tables_of(c);
+    int64_t g = (c->tp_bound + 7) / 8 * 8;
+    const int64_t g1 = g < (int64_t)c->num_cu * 8 ? g : (int64_t)c->num_cu * 8;
+    const int64_t g2 = g < (int64_t)c->num_cu * 16 ? g : (int64_t)c->num_cu * 16;
+    MPMX_TIMED(c, "cg_col", (k_cg_col<ST><<<(int)g1, 256, 0, c->stream>>>(c->dp, st, c->vol, t, c->cg_s,
+                                                                         c->cg_col, (int)c->n, c->dc)));
+    MPMX_TIMED(c, "cg_q", (k_cg_q<ST><<<(int)g2, 64, 0, c->stream>>>(c->dp, st, t, c->cg_col, c->grid, c->cg_s,
+                                                                     c->cg_q, (int)c->n, c->dc)));
+}
+
 static hipError_t pcg_iteration(mpmx_ctx* c, int k) {
-    MPMX_TIMED(c, "cg_apply", (c->cfg.precision == MPMX_MIXED) ? apply_block(c, c->ms[c->cur])
-                                                               : apply_block(c, c->ps[c->cur]));
+    const bool blocks = c->cfg.p2g_variant != 1;
+    if (blocks)
+        MPMX_TIMED(c, "cg_apply", (c->cfg.precision == MPMX_MIXED) ? apply_blocks(c, c->ms[c->cur])
+                                                                   : apply_blocks(c, c->ps[c->cur]));
+    else
+        MPMX_TIMED(c, "cg_apply", (c->cfg.precision == MPMX_MIXED) ? apply_block(c, c->ms[c->cur])
+                                                                   : apply_block(c, c->ps[c->cur]));
     const int nb = c->cg_nb;
     MPMX_TIMED(c, "cg_vector", ({
-                   k_cg_dot<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_s, c->cg_q, c->cg_part, c->cap_tp,
-                                                       c->dc);
+                   k_cg_dot<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_s, c->cg_q, c->cg_part,
+                                                       blocks ? 0 : 1, c->cap_tp, c->dc);
                    k_cg_update<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_x, c->cg_r, c->cg_z, c->cg_s,
                                                           c->cg_q, c->cg_part, nb, k, c->cap_tp, c->dc);
                    k_cg_direction<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_z, c->cg_s, c->cg_part, nb, k,
@@ -281,27 +488,34 @@ static hipError_t pcg_iteration(mpmx_ctx* c, int k) {
     return hipGetLastError();
 }
 
+// Iterations are queued without host round trips and turn into no-ops once the
+// device flag says done.  The first chunk is as long as the previous substep's
+// solve (one host check per substep while the iteration count holds), later
+// chunks cg_check_every long.
 hipError_t launch_pcg(mpmx_ctx* c, float dt) {
     (void)dt;
-    if (!c->P.phase_field) return hipSuccess;
+    if (!c->dp.phase_field) return hipSuccess;
+    const int maxit = c->P.cg_max_iter;
     const int every = c->cfg.cg_check_every > 0 ? c->cfg.cg_check_every : 1;
+    int chunk = c->cg_spec > 0 ? c->cg_spec : 1;
     int it = 0;
-    while (it < c->P.cg_max_iter) {
-        const int chunk = (c->P.cg_max_iter - it) < every ? (c->P.cg_max_iter - it) : every;
+    while (it < maxit) {
+        chunk = (maxit - it) < chunk ? (maxit - it) : chunk;
         for (int k = 0; k < chunk; ++k) {
             hipError_t e = pcg_iteration(c, it + k);
             if (e != hipSuccess) return e;
         }
         it += chunk;
-        if (it >= c->P.cg_max_iter) break;
-        // read the convergence flag
-        hipError_t e = hipMemcpyAsync(&c->hc->cg_done, &c->dc->cg_done, sizeof(int), hipMemcpyDeviceToHost,
+        // read the convergence flag and the iteration count (adjacent ints)
+        hipError_t e = hipMemcpyAsync(&c->hc->cg_done, &c->dc->cg_done, 2 * sizeof(int), hipMemcpyDeviceToHost,
                                       c->stream);
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) return e;
-        if (c->hc->cg_done) break;
+        if (c->hc->cg_done || it >= maxit) break;
+        chunk = every;
     }
+    c->cg_spec = c->hc->cg_iters > 0 ? c->hc->cg_iters : 1;
     return hipSuccess;
 }
 

@@ -150,6 +150,104 @@ __device__ __forceinline__ void von_mises(const DevParams& dp, const StressIn& i
     out.vol = (float)(T(dp.V0) * det);
 }
 
+// sigma = V0 U diag(P_hat) V^T F^T, F = in.F
+template <typename T>
+__device__ __forceinline__ void contribution(const DevParams& dp, const StressIn& in, const T (&U)[3][3],
+                                             const T (&P_hat)[3], const T (&V)[3][3], StressOut& out) {
+    T P[3][3];
+    usv(U, P_hat, V, P);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+            out.sigma[r][cc] = (float)((P[r][0] * T(in.F[cc][0]) + P[r][1] * T(in.F[cc][1]) + P[r][2] * T(in.F[cc][2])) *
+                                       T(dp.V0));
+}
+
+template <typename T>
+__device__ __forceinline__ T det3(const float (&F)[3][3]) {
+    return T(F[0][0]) * (T(F[1][1]) * T(F[2][2]) - T(F[1][2]) * T(F[2][1])) -
+           T(F[0][1]) * (T(F[1][0]) * T(F[2][2]) - T(F[1][2]) * T(F[2][0])) +
+           T(F[0][2]) * (T(F[1][0]) * T(F[2][1]) - T(F[1][1]) * T(F[2][0]));
+}
+
+// computeContributionNeoHookean (ExplicitTimeIntegrator.cu:372-446): phase-
+// field fracture without plasticity.  F, g and alpha are left as they are; as
+// in the reference the psi+ history takes kappa/2 ((J - 1)/2 - ln J) while the
+// stress uses kappa/2 (J^2 - 1), FP takes G_c = 3 and (1 - 0.001) literally,
+// and ln J is unguarded.
+template <typename T>
+__device__ __forceinline__ void neo_hookean(const DevParams& dp, const StressIn& in, StressOut& out,
+                                            const T (&U)[3][3], const T (&S)[3], const T (&V)[3][3]) {
+    const T mu = dp.mu, kappa = dp.kappa;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) out.F[r][cc] = in.F[r][cc];
+    out.g = in.g;
+    out.alpha = in.alpha;
+    const T J = S[0] * S[1] * S[2];
+    const T J_bar = pow(J, -T(2.0f / 3.0f));
+    T be[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) be[a] = J_bar * S[a] * S[a];
+    const T be_sum = be[0] + be[1] + be[2];
+    const T ln_J = log(J);
+    const T Psi_vol = kappa / T(2) * ((J - T(1)) / T(2) - ln_J);
+    const T Psi_dev = mu / T(2) * (be_sum - T(3));
+    const T Psi_pos = (J < T(1)) ? Psi_dev : Psi_dev + Psi_vol;
+    const T smax = fmax(S[0], fmax(S[1], S[2]));
+    T psi = in.psi;
+    if (smax >= T(dp.crit_stretch) && Psi_pos > psi && Psi_pos < T(1e6)) psi = Psi_pos;
+    out.psi = (float)psi;
+    out.FP = (float)(T(4) * T(dp.L0) * (T(1) - T(0.001f)) * psi * (T(1) / T(3)) + T(1));
+    const T tau_vol = kappa / T(2) * (J * J - T(1));
+    const T third = (T(1) / T(3)) * be_sum;
+    const T theta = dp.theta;
+    const T g = T(in.c) * T(in.c) * (T(1) - theta) + theta;
+    T P_hat[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const T td = mu * (be[a] - third);
+        P_hat[a] = (J < T(1)) ? g * td + tau_vol : g * (td + tau_vol);
+    }
+    contribution<T>(dp, in, U, P_hat, V, out);
+    out.vol = (float)(T(dp.V0) * det3<T>(in.F));
+}
+
+// computeContributionFixedCorotated (ExplicitTimeIntegrator.cu:22-47): no
+// plasticity and no fracture; only sigma changes (FP = 1 and vol = V0 det F
+// for the record; the solve does not run)
+template <typename T>
+__device__ __forceinline__ void fixed_corotated(const DevParams& dp, const StressIn& in, StressOut& out,
+                                                const T (&U)[3][3], const T (&S)[3], const T (&V)[3][3]) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) out.F[r][cc] = in.F[r][cc];
+    out.g = in.g;
+    out.alpha = in.alpha;
+    out.psi = in.psi;
+    const T mu = dp.mu, lambda = dp.lambda;
+    const T J = S[0] * S[1] * S[2];
+    const T scaled_mu = T(2) * mu, scaled_lambda = lambda * (J - T(1));
+    const T P_hat[3] = {scaled_mu * (S[0] - T(1)) + scaled_lambda * (S[1] * S[2]),
+                        scaled_mu * (S[1] - T(1)) + scaled_lambda * (S[0] * S[2]),
+                        scaled_mu * (S[2] - T(1)) + scaled_lambda * (S[0] * S[1])};
+    contribution<T>(dp, in, U, P_hat, V, out);
+    out.FP = 1.0f;
+    out.vol = (float)(T(dp.V0) * det3<T>(in.F));
+}
+
+// the energy of the scene: von Mises (2), Neo-Hookean (1), fixed corotated (0)
+template <typename T>
+__device__ __forceinline__ void energy(const DevParams& dp, const StressIn& in, StressOut& out,
+                                       const T (&U)[3][3], T (&S)[3], const T (&V)[3][3]) {
+    if (dp.model == 2) von_mises<T>(dp, in, out, U, S, V);
+    else if (dp.model == 1) neo_hookean<T>(dp, in, out, U, S, V);
+    else fixed_corotated<T>(dp, in, out, U, S, V);
+}
+
 // ALL64: SVD and return mapping in FP64 for every particle.  Otherwise FP32,
 // redone in FP64 where the smallest singular-value gap is below svd_gap (the
 // directions of near-equal singular values are ill-conditioned in FP32).
@@ -164,7 +262,7 @@ __device__ __forceinline__ bool stress_one(const DevParams& dp, const StressIn& 
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) Ad[r][cc] = in.F[r][cc];
         svd3<double>(Ad, Ud, Sd, Vd);
-        von_mises<double>(dp, in, out, Ud, Sd, Vd);
+        energy<double>(dp, in, out, Ud, Sd, Vd);
         return false;
     }
     float A[3][3], U[3][3], S[3], V[3][3];
@@ -181,10 +279,10 @@ __device__ __forceinline__ bool stress_one(const DevParams& dp, const StressIn& 
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) Ad[r][cc] = A[r][cc];
         svd3<double>(Ad, Ud, Sd, Vd);
-        von_mises<double>(dp, in, out, Ud, Sd, Vd);
+        energy<double>(dp, in, out, Ud, Sd, Vd);
         return true;
     }
-    von_mises<float>(dp, in, out, U, S, V);
+    energy<float>(dp, in, out, U, S, V);
     return false;
 }
 

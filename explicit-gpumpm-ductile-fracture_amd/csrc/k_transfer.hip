@@ -119,6 +119,117 @@ __global__ __launch_bounds__(BLOCK) void k_p2g(DevParams dp, ST s, const float* 
     }
 }
 
+// G2P of particle i from a staged node region of R^3 nodes whose node (0,0,0)
+// is node org: channel ch of region node (a, b, c) at tv[ch R^3 + (a R + b) R + c]
+// (v0, v1, v2 and the damage increment).  MLS velocity clamp, c update,
+// F = (I + dt grad v) F, B, x, the next cell key and the error flags.
+template <int TRANSFER, int R, typename ST>
+__device__ __forceinline__ void g2p_particle(const DevParams& dp, const ST& s, int i, int n, float dt,
+                                             const int (&org)[3], const float* tv, uint32_t* __restrict__ keys,
+                                             DevCounts* dc) {
+    constexpr int R3 = R * R * R;
+    float d0[3], w[3][3], gw[3][3];
+    int l[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float xa = ld_x(s, i, a);
+        const int b = base_node(xa, dp.N);
+        d0[a] = xa * dp.inv_dx - (float)b;
+        const int la = b - org[a];
+        l[a] = la < 0 ? 0 : (la > R - 3 ? R - 3 : la);
+        if (TRANSFER == 0) bspline(d0[a], w[a]);
+        else bspline_grad(d0[a], w[a], gw[a]);
+    }
+    float vp3[3] = {0.f, 0.f, 0.f}, dcs = 0.f;
+    float Bn[9], G[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) { Bn[k] = 0.f; G[k] = 0.f; }
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) {
+                const int node = ((l[0] + ii) * R + (l[1] + jj)) * R + (l[2] + kk);
+                const float wt = w[0][ii] * w[1][jj] * w[2][kk];
+                const float vi[3] = {tv[node], tv[R3 + node], tv[2 * R3 + node]};
+                dcs += wt * tv[3 * R3 + node];
+                const float o[3] = {((float)ii - d0[0]) * dp.dx, ((float)jj - d0[1]) * dp.dx,
+                                    ((float)kk - d0[2]) * dp.dx};
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    vp3[r] += wt * vi[r];
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) Bn[r + 3 * cc] += wt * vi[r] * o[cc];
+                }
+                if (TRANSFER == 1) {
+                    const float g3[3] = {gw[0][ii] * w[1][jj] * w[2][kk] * dp.inv_dx,
+                                         w[0][ii] * gw[1][jj] * w[2][kk] * dp.inv_dx,
+                                         w[0][ii] * w[1][jj] * gw[2][kk] * dp.inv_dx};
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int cc = 0; cc < 3; ++cc) G[r + 3 * cc] += vi[r] * g3[cc];
+                }
+            }
+    if (TRANSFER == 0) {
+        const float nv = sqrtf(vp3[0] * vp3[0] + vp3[1] * vp3[1] + vp3[2] * vp3[2]);
+        if (nv > dp.vmax) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) vp3[r] = vp3[r] / nv * dp.vmax;
+        }
+    }
+    const float pc0 = ld_c(s, i);
+    const float nc = pc0 + dcs;
+    st_c(s, i, fmaxf(fminf(pc0, nc), 0.f));
+    float xn[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        st_v(s, i, r, vp3[r]);
+        xn[r] = ld_x(s, i, r) + vp3[r] * dt;
+        st_x(s, i, r, xn[r]);
+    }
+    float A[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) A[k] = (TRANSFER == 0) ? Bn[k] * dt * dp.Dinv : G[k] * dt;
+    A[0] += 1.f;
+    A[4] += 1.f;
+    A[8] += 1.f;
+    float F[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] = ld_F(s, i, k, n);
+    bool finite = true;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            const float f = A[r] * F[3 * cc] + A[r + 3] * F[3 * cc + 1] + A[r + 6] * F[3 * cc + 2];
+            finite = finite && isfinite(f);
+            st_F(s, i, r + 3 * cc, n, f);
+        }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) st_B(s, i, k, n, Bn[k]);
+    const float lo = 2.0f * dp.dx, hi = 1.0f - 2.0f * dp.dx;
+    if (!(xn[0] >= lo && xn[0] <= hi && xn[1] >= lo && xn[1] <= hi && xn[2] >= lo && xn[2] <= hi))
+        atomicOr(&dc->err, isfinite(xn[0] + xn[1] + xn[2]) ? kErrDomain : kErrNonFinite);
+    if (!finite) atomicOr(&dc->err, kErrNonFinite);
+    keys[i] = key_of(base_node(xn[0], dp.N), base_node(xn[1], dp.N), base_node(xn[2], dp.N));
+}
+
+// the grid values G2P reads at global node gn (page * 64 + cell): v0..v2 and
+// the damage increment x - c of update_grid_phase (touched nodes only)
+__device__ __forceinline__ void g2p_node(const DevParams& dp, const float* __restrict__ grid,
+                                         const float* __restrict__ cgx, bool breakdown, int64_t gn, float& v0,
+                                         float& v1, float& v2, float& dcn) {
+    const float* gp = grid + (gn >> 6) * kPageFloats + (gn & 63);
+    v0 = gp[64];
+    v1 = gp[128];
+    v2 = gp[192];
+    dcn = 0.f;
+    if (dp.phase_field && gp[5 * 64] > 0.f) dcn = (breakdown ? 1e20f : cgx[gn]) - gp[4 * 64];
+}
+
+// page-tile G2P (variant 1): one virtual page at a time, its 6^3 tile in LDS
 template <int BLOCK, int TRANSFER, typename ST>
 __global__ __launch_bounds__(BLOCK) void k_g2p(DevParams dp, ST s, Tables t, const float* __restrict__ grid,
                                                const float* __restrict__ cgx, uint32_t* __restrict__ keys,
@@ -129,7 +240,6 @@ __global__ __launch_bounds__(BLOCK) void k_g2p(DevParams dp, ST s, Tables t, con
     const bool breakdown = dc->cg_breakdown != 0;
     int vb, ve;
     block_range(nvp, vb, ve);
-    const float lo = 2.0f * dp.dx, hi = 1.0f - 2.0f * dp.dx;
     for (int vp = vb; vp < ve; ++vp) {
         int p, s0, e;
         vpage_range(t, vp, BLOCK, n, p, s0, e);
@@ -137,14 +247,7 @@ __global__ __launch_bounds__(BLOCK) void k_g2p(DevParams dp, ST s, Tables t, con
             const int a = q / 36, b = (q / 6) % 6, cc = q % 6;
             const int64_t gn = tile_node(t, p, a, b, cc);
             float v0 = 0.f, v1 = 0.f, v2 = 0.f, dcn = 0.f;
-            if (gn >= 0) {
-                const float* gp = grid + (gn >> 6) * kPageFloats + (gn & 63);
-                v0 = gp[64];
-                v1 = gp[128];
-                v2 = gp[192];
-                // damage increment x - c (update_grid_phase); touched nodes only
-                if (dp.phase_field && gp[5 * 64] > 0.f) dcn = (breakdown ? 1e20f : cgx[gn]) - gp[4 * 64];
-            }
+            if (gn >= 0) g2p_node(dp, grid, cgx, breakdown, gn, v0, v1, v2, dcn);
             tv[q] = v0;
             tv[kTileN + q] = v1;
             tv[2 * kTileN + q] = v2;
@@ -155,91 +258,92 @@ __global__ __launch_bounds__(BLOCK) void k_g2p(DevParams dp, ST s, Tables t, con
         if (i < e) {
             int pc[3];
             unmorton(t.page_key[p], pc[0], pc[1], pc[2]);
-            float d0[3], w[3][3], gw[3][3];
-            int l[3];
+            const int org[3] = {4 * pc[0], 4 * pc[1], 4 * pc[2]};
+            g2p_particle<TRANSFER, kTile>(dp, s, i, n, dt, org, tv, keys, dc);
+        }
+        __syncthreads();
+    }
+}
+
+// block G2P (variant 2): a workgroup stages the 10^3 nodes that the particles
+// of one 2x2x2 page block reach (the block's 8^3 nodes and two layers of the
+// +x/+y/+z neighbours) and runs every particle of the block; the block's
+// particles are one contiguous range because its 8 pages are consecutive
+// Morton codes
+constexpr int kBR = 10;                       // staged region edge
+constexpr int kBR3 = kBR * kBR * kBR;
+
+template <int TRANSFER, typename ST>
+__global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTables t, const float* __restrict__ grid,
+                                                   const float* __restrict__ cgx, uint32_t* __restrict__ keys,
+                                                   float dt, int n, DevCounts* dc) {
+    __shared__ float tv[4 * kBR3];
+    __shared__ int spg[27];
+    __shared__ int prange[2];
+    const int np_raw = dc->num_page;
+    const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
+    int nt = dc->num_total;
+    nt = nt < t.cap_tp ? nt : (int)t.cap_tp;
+    int nb = dc->num_blocks;
+    nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
+    const bool breakdown = dc->cg_breakdown != 0;
+    int bb, be;
+    block_range(nb, bb, be);
+    for (int blk = bb; blk < be; ++blk) {
+        const uint32_t bkey = t.block_key[blk];
+        int bc[3];
+        unmorton(bkey, bc[0], bc[1], bc[2]);
+        const int tid = threadIdx.x;
+        if (tid < 27) {
+            const int px = 2 * bc[0] + tid / 9, py = 2 * bc[1] + (tid / 3) % 3, pz = 2 * bc[2] + tid % 3;
+            int id = -1;
+            if (px < t.axis_pages && py < t.axis_pages && pz < t.axis_pages)
+                id = page_rank(morton((uint32_t)px, (uint32_t)py, (uint32_t)pz), t.pmask, t.nmask, t.wpre, t.spre,
+                               np_raw);
+            spg[tid] = (id >= 0 && id < nt) ? id : -1;
+        }
+        if (tid >= 64 && tid < 128) {
+            // own particle pages: ids of the block's pages below num_page, min and max
+            const int o = tid - 64;
+            int id = -1;
+            if (o < 8) id = page_rank((bkey << 3) | (uint32_t)o, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
+            const bool own = id >= 0 && id < np;
+            int lo = own ? id : 0x7fffffff, hi = own ? id : -1;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const float xa = ld_x(s, i, a);
-                const int b = base_node(xa, dp.N);
-                d0[a] = xa * dp.inv_dx - (float)b;
-                int la = b - 4 * pc[a];
-                l[a] = la < 0 ? 0 : (la > 3 ? 3 : la);
-                if (TRANSFER == 0) bspline(d0[a], w[a]);
-                else bspline_grad(d0[a], w[a], gw[a]);
+            for (int w = 32; w > 0; w >>= 1) {
+                lo = min(lo, __shfl_xor(lo, w, 64));
+                hi = max(hi, __shfl_xor(hi, w, 64));
             }
-            float vp3[3] = {0.f, 0.f, 0.f}, dcs = 0.f;
-            float Bn[9], G[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) { Bn[k] = 0.f; G[k] = 0.f; }
-#pragma unroll
-            for (int ii = 0; ii < 3; ++ii)
-#pragma unroll
-                for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-                    for (int kk = 0; kk < 3; ++kk) {
-                        const int node = (l[0] + ii) * 36 + (l[1] + jj) * 6 + (l[2] + kk);
-                        const float wt = w[0][ii] * w[1][jj] * w[2][kk];
-                        const float vi[3] = {tv[node], tv[kTileN + node], tv[2 * kTileN + node]};
-                        dcs += wt * tv[3 * kTileN + node];
-                        const float o[3] = {((float)ii - d0[0]) * dp.dx, ((float)jj - d0[1]) * dp.dx,
-                                            ((float)kk - d0[2]) * dp.dx};
-#pragma unroll
-                        for (int r = 0; r < 3; ++r) {
-                            vp3[r] += wt * vi[r];
-#pragma unroll
-                            for (int cc = 0; cc < 3; ++cc) Bn[r + 3 * cc] += wt * vi[r] * o[cc];
-                        }
-                        if (TRANSFER == 1) {
-                            const float g3[3] = {gw[0][ii] * w[1][jj] * w[2][kk] * dp.inv_dx,
-                                                 w[0][ii] * gw[1][jj] * w[2][kk] * dp.inv_dx,
-                                                 w[0][ii] * w[1][jj] * gw[2][kk] * dp.inv_dx};
-#pragma unroll
-                            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                                for (int cc = 0; cc < 3; ++cc) G[r + 3 * cc] += vi[r] * g3[cc];
-                        }
-                    }
-            if (TRANSFER == 0) {
-                const float nv = sqrtf(vp3[0] * vp3[0] + vp3[1] * vp3[1] + vp3[2] * vp3[2]);
-                if (nv > dp.vmax) {
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) vp3[r] = vp3[r] / nv * dp.vmax;
+            if (o == 0) {
+                int a = 0, b = 0;
+                if (hi >= 0) {
+                    a = t.page2particle[lo];
+                    b = t.page2particle[hi + 1];
+                    a = a < 0 ? 0 : (a > n ? n : a);
+                    b = b < a ? a : (b > n ? n : b);
                 }
+                prange[0] = a;
+                prange[1] = b;
             }
-            const float pc0 = ld_c(s, i);
-            const float nc = pc0 + dcs;
-            st_c(s, i, fmaxf(fminf(pc0, nc), 0.f));
-            float xn[3];
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                st_v(s, i, r, vp3[r]);
-                xn[r] = ld_x(s, i, r) + vp3[r] * dt;
-                st_x(s, i, r, xn[r]);
+        }
+        __syncthreads();
+        const int pa = prange[0], pb = prange[1];
+        if (pa < pb) {
+            for (int q = tid; q < kBR3; q += 256) {
+                const int X = q / 100, Y = (q / 10) % 10, Z = q % 10;
+                const int G = spg[(X >> 2) * 9 + (Y >> 2) * 3 + (Z >> 2)];
+                float v0 = 0.f, v1 = 0.f, v2 = 0.f, dcn = 0.f;
+                if (G >= 0)
+                    g2p_node(dp, grid, cgx, breakdown, (int64_t)G * 64 + (((X & 3) << 4) | ((Y & 3) << 2) | (Z & 3)),
+                             v0, v1, v2, dcn);
+                tv[q] = v0;
+                tv[kBR3 + q] = v1;
+                tv[2 * kBR3 + q] = v2;
+                tv[3 * kBR3 + q] = dcn;
             }
-            float A[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) A[k] = (TRANSFER == 0) ? Bn[k] * dt * dp.Dinv : G[k] * dt;
-            A[0] += 1.f;
-            A[4] += 1.f;
-            A[8] += 1.f;
-            float F[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) F[k] = ld_F(s, i, k, n);
-            bool finite = true;
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int cc = 0; cc < 3; ++cc) {
-                    const float f = A[r] * F[3 * cc] + A[r + 3] * F[3 * cc + 1] + A[r + 6] * F[3 * cc + 2];
-                    finite = finite && isfinite(f);
-                    st_F(s, i, r + 3 * cc, n, f);
-                }
-#pragma unroll
-            for (int k = 0; k < 9; ++k) st_B(s, i, k, n, Bn[k]);
-            if (!(xn[0] >= lo && xn[0] <= hi && xn[1] >= lo && xn[1] <= hi && xn[2] >= lo && xn[2] <= hi))
-                atomicOr(&dc->err, isfinite(xn[0] + xn[1] + xn[2]) ? kErrDomain : kErrNonFinite);
-            if (!finite) atomicOr(&dc->err, kErrNonFinite);
-            keys[i] = key_of(base_node(xn[0], dp.N), base_node(xn[1], dp.N), base_node(xn[2], dp.N));
+            __syncthreads();
+            const int org[3] = {8 * bc[0], 8 * bc[1], 8 * bc[2]};
+            for (int i = pa + tid; i < pb; i += 256) g2p_particle<TRANSFER, kBR>(dp, s, i, n, dt, org, tv, keys, dc);
         }
         __syncthreads();
     }
@@ -309,9 +413,29 @@ static void g2p_block(mpmx_ctx* c, ST st, float dt) {
     }
 }
 
+template <typename ST>
+static void g2p_block_kernel(mpmx_ctx* c, ST st, float dt) {
+    const BlockTables t = block_tables_of(c);
+    int64_t g = (c->tp_bound + 7) / 8 * 8;
+    g = g < (int64_t)c->num_cu * 8 ? g : (int64_t)c->num_cu * 8;
+    MPMX_TIMED(c, "g2p_block", ({
+                   if (c->P.transfer == 0)
+                       k_g2p_block<0, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys, dt,
+                                                                         (int)c->n, c->dc);
+                   else
+                       k_g2p_block<1, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys, dt,
+                                                                         (int)c->n, c->dc);
+               }));
+}
+
 hipError_t launch_g2p(mpmx_ctx* c, float dt) {
-    if (c->cfg.precision == MPMX_MIXED) g2p_block(c, c->ms[c->cur], dt);
-    else g2p_block(c, c->ps[c->cur], dt);
+    if (c->cfg.p2g_variant == 1) {
+        if (c->cfg.precision == MPMX_MIXED) g2p_block(c, c->ms[c->cur], dt);
+        else g2p_block(c, c->ps[c->cur], dt);
+    } else {
+        if (c->cfg.precision == MPMX_MIXED) g2p_block_kernel(c, c->ms[c->cur], dt);
+        else g2p_block_kernel(c, c->ps[c->cur], dt);
+    }
     return hipGetLastError();
 }
 

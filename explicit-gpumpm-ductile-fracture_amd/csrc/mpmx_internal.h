@@ -45,7 +45,8 @@ enum ErrBits : unsigned {
 struct DevParams {
     int N;
     int transfer;
-    int phase_field;
+    int phase_field;   // the solve runs (phase_field set and the energy has fracture)
+    int model;         // 2 von Mises, 1 Neo-Hookean phase field, 0 fixed corotated
     int dot_mode;
     int wall;
     int grip_lo, grip_hi;
@@ -166,7 +167,9 @@ struct mpmx_ctx {
     float* cg_s = nullptr;
     float* cg_q = nullptr;
     float* cg_part = nullptr;    // [4][cg_nb] per-workgroup partials
+    float* cg_col = nullptr;     // [3n] per-particle M V dx^2 grad s (block K s)
     int cg_nb = 0;               // workgroups of every node-space kernel
+    int cg_spec = 1;             // PCG iterations queued before the first host check
     bool grid_fused = false;     // last P2G also ran the grid update (gather variant)
     // counters
     mpmx::DevCounts* dc = nullptr;

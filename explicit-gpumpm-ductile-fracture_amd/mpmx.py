@@ -40,7 +40,7 @@ class Params(C.Structure):
         ("alpha_c", C.c_double), ("dlambda_cap", C.c_double), ("eps_cap", C.c_double),
         ("Gc0", C.c_double), ("residual_c", C.c_double), ("ps", C.c_double), ("theta", C.c_double),
         ("M", C.c_double), ("gravity", C.c_double * 3), ("vmax", C.c_double), ("cg_tol", C.c_double),
-        ("crit_stretch", C.c_double),
+        ("crit_stretch", C.c_double), ("model", C.c_int),
     ]
 
     def as_dict(self):

@@ -69,6 +69,10 @@ typedef struct mpmx_params {
     double vmax;         /* MLS grid-to-particle speed clamp */
     double cg_tol;
     double crit_stretch; /* psi+ history grows only while max principal stretch >= this (0: always) */
+    int    model;        /* energy (Setting.h:38 Energy_Model): 2 = von Mises plastic phase-field
+                            fracture (default), 1 = Neo-Hookean phase-field fracture
+                            (computeContributionNeoHookean), 0 = fixed corotated, no
+                            fracture (computeContributionFixedCorotated; phase field off) */
 } mpmx_params;
 
 typedef enum mpmx_precision {
@@ -83,9 +87,13 @@ typedef struct mpmx_config {
     int     device;         /* HIP device ordinal */
     int     precision;      /* mpmx_precision */
     double  svd_gap;        /* FP32 stress arithmetic: FP64 redo when min singular-value gap < this (1e-4) */
-    int     cg_check_every; /* host reads the PCG convergence flag every this many iterations (1) */
-    int     p2g_variant;    /* 0 = auto (autotuned), 1 = particle-per-lane LDS atomics,
-                               2 = cell-gather (lanes own (cell, node column) pairs) */
+    int     cg_check_every; /* PCG: the first host check of a substep comes after as many
+                               iterations as the previous substep took, later checks every
+                               this many iterations (1)                                */
+    int     p2g_variant;    /* transfer kernels: 0 = default (2); 1 = page tiles over
+                               virtual pages with LDS and global float atomics (P2G,
+                               G2P, PCG K s); 2 = 2x2x2-page blocks, P2G and K s as
+                               single-wave gathers without atomics               */
     int     block;          /* threads per transfer block; 0 = auto (= vpage) */
     int     stress_fp64;    /* arithmetic of the SVD and return mapping: 1 = FP64 for every
                                particle (default; state stays in its storage precision),

@@ -28,7 +28,7 @@ class Params(C.Structure):
         ("dlambda_cap", C.c_double), ("eps_cap", C.c_double), ("Gc0", C.c_double),
         ("residual_c", C.c_double), ("ps", C.c_double), ("theta", C.c_double),
         ("M", C.c_double), ("gravity", C.c_double * 3), ("vmax", C.c_double),
-        ("cg_tol", C.c_double), ("crit_stretch", C.c_double),
+        ("cg_tol", C.c_double), ("crit_stretch", C.c_double), ("model", C.c_int),
     ]
 
 
@@ -76,6 +76,10 @@ def lib():
         L.mo_svd3.argtypes = [f64p, f64p, f64p, f64p]
         L.mo_svd3_last_sweeps.restype = C.c_int
         L.mo_inverse3.argtypes = [f64p, f64p]
+        L.mo_neo_hookean.argtypes = [C.POINTER(Params), f64p, C.c_double, f64p, f64p, f64p, f64p]
+        L.mo_neo_hookean.restype = None
+        L.mo_fixed_corotated.argtypes = [C.POINTER(Params), f64p, f64p]
+        L.mo_fixed_corotated.restype = None
         L.mo_von_mises.argtypes = [C.POINTER(Params), f64p, C.c_double, f64p, f64p, f64p,
                                    f64p, f64p, f64p]
         L.mo_sim_create.argtypes = [C.POINTER(Params), C.c_int, f64p, f64p, f64p]
@@ -217,6 +221,25 @@ def von_mises(params, F, c, g, alpha, psi):
     lib().mo_von_mises(C.byref(params), Fc, float(c), gg, al, ps, fp, sig, vol)
     return {"F": Fc.reshape(3, 3).T.copy(), "g": gg[0], "alpha": al[0], "psi": ps[0],
             "FP": fp[0], "sigma": sig.reshape(3, 3).T.copy(), "vol": vol[0]}
+
+
+def neo_hookean(params, F, c, psi):
+    """One particle's Neo-Hookean phase-field update.  Returns dict."""
+    Fc = np.ascontiguousarray(np.asarray(F, np.float64).T).reshape(-1).copy()
+    ps = np.array([psi], np.float64)
+    fp = np.zeros(1)
+    sig = np.zeros(9)
+    vol = np.zeros(1)
+    lib().mo_neo_hookean(C.byref(params), Fc, float(c), ps, fp, sig, vol)
+    return {"psi": ps[0], "FP": fp[0], "sigma": sig.reshape(3, 3).T.copy(), "vol": vol[0]}
+
+
+def fixed_corotated(params, F):
+    """One particle's fixed corotated stress contribution V0 P F^T."""
+    Fc = np.ascontiguousarray(np.asarray(F, np.float64).T).reshape(-1).copy()
+    sig = np.zeros(9)
+    lib().mo_fixed_corotated(C.byref(params), Fc, sig)
+    return sig.reshape(3, 3).T.copy()
 
 
 class Sim:

@@ -46,6 +46,7 @@ void mo_default_params(mo_params* p) {
     p->vmax = (double)11.137f;
     p->cg_tol = (double)1e-6f;
     p->crit_stretch = 0.0;
+    p->model = 2;
 }
 
 /* SPGridMask.cpp:78-88, 96-100, 119-124 with struct_size = 6, element_size = 2

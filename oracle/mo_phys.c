@@ -209,6 +209,75 @@ void mo_von_mises(const mo_params* p, double F[9], double phase, double* g_io, d
     *vol_out = p->V0 * det;
 }
 
+static double det3(const double F[9]) {
+    return F[0] * F[4] * F[8] + F[3] * F[7] * F[2] + F[6] * F[1] * F[5] - F[2] * F[4] * F[6] -
+           F[1] * F[3] * F[8] - F[0] * F[5] * F[7];
+}
+
+/* sigma = V0 * U diag(P_hat) V^T * F^T */
+static void contribution(const mo_params* p, const double U[9], const double P_hat[3], const double V[9],
+                         const double F[9], double sigma[9]) {
+    double P[9];
+    usv(U, P_hat, V, P);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            sigma[r + 3 * c] = (P[r] * F[c] + P[r + 3] * F[c + 3] + P[r + 6] * F[c + 6]) * p->V0;
+}
+
+/* computeContributionNeoHookean (ExplicitTimeIntegrator.cu:372-446): phase-field
+ * fracture without plasticity; F is not changed.  As in the reference the
+ * psi+ history takes the volumetric term kappa/2 ((J - 1)/2 - ln J) (:400)
+ * while the stress uses kappa/2 (J^2 - 1) (:421-422), FP takes G_c = 3 and
+ * (1 - 0.001) as literals (:416-418), ln J is unguarded (:397), and
+ * g = c^2 (1 - theta) + theta is used but not stored (:426). */
+void mo_neo_hookean(const mo_params* p, const double F[9], double phase, double* psi_io, double* FP_out,
+                    double sigma[9], double* vol_out) {
+    double lambda, mu, kappa;
+    lame(p, &lambda, &mu, &kappa);
+    double U[9], S[3], V[9];
+    mo_svd3(F, U, S, V);
+    const double J = S[0] * S[1] * S[2];
+    const double J_bar = pow(J, -F23);
+    double be[3];
+    for (int a = 0; a < 3; ++a) be[a] = J_bar * S[a] * S[a];
+    const double be_sum = be[0] + be[1] + be[2];
+    const double ln_J = log(J);
+    const double Psi_vol = kappa / 2.0 * ((J - 1.0) / 2.0 - ln_J);
+    const double Psi_dev = mu / 2.0 * (be_sum - 3.0);
+    const double Psi_pos = (J < 1.0) ? Psi_dev : Psi_dev + Psi_vol;
+    const double smax = fmax(S[0], fmax(S[1], S[2]));
+    if (smax >= p->crit_stretch && Psi_pos > *psi_io && Psi_pos < 1e6) *psi_io = Psi_pos;
+    const double G_C = 3.0;
+    const double L0 = 0.5 * (1.0 / p->N);
+    *FP_out = 4.0 * L0 * (1.0 - (double)0.001f) * (*psi_io) * (1.0 / G_C) + 1.0;
+    const double tau_vol = kappa / 2.0 * (J * J - 1.0);
+    const double third = (1.0 / 3.0) * be_sum;
+    const double theta = p->theta;
+    const double g = phase * phase * (1.0 - theta) + theta;
+    double P_hat[3];
+    for (int a = 0; a < 3; ++a) {
+        const double td = mu * (be[a] - third);
+        P_hat[a] = (J < 1.0) ? g * td + tau_vol : g * (td + tau_vol);
+    }
+    contribution(p, U, P_hat, V, F, sigma);
+    *vol_out = p->V0 * det3(F);
+}
+
+/* computeContributionFixedCorotated (ExplicitTimeIntegrator.cu:22-47): no
+ * plasticity, no fracture */
+void mo_fixed_corotated(const mo_params* p, const double F[9], double sigma[9]) {
+    double lambda, mu, kappa;
+    lame(p, &lambda, &mu, &kappa);
+    double U[9], S[3], V[9];
+    mo_svd3(F, U, S, V);
+    const double J = S[0] * S[1] * S[2];
+    const double scaled_mu = 2.0 * mu, scaled_lambda = lambda * (J - 1.0);
+    const double P_hat[3] = {scaled_mu * (S[0] - 1.0) + scaled_lambda * (S[1] * S[2]),
+                             scaled_mu * (S[1] - 1.0) + scaled_lambda * (S[0] * S[2]),
+                             scaled_mu * (S[2] - 1.0) + scaled_lambda * (S[0] * S[1])};
+    contribution(p, U, P_hat, V, F, sigma);
+}
+
 /* ------------------------------------------------------------------ simulator */
 struct mo_sim {
     mo_params p;
@@ -711,17 +780,25 @@ int mo_sim_substep(mo_sim* s, double dt) {
     if (!in_domain(s)) return -2;
     /* DomainTransformer::rebuild (Transformer.cpp:71-82) */
     if (build_tables(s)) return -1;
-    /* computeForceCoefficient -> applyVonMises (ExplicitTimeIntegrator.cu:846-865) */
-    for (int id = 0; id < s->n; ++id)
-        mo_von_mises(&s->p, s->F + 9 * id, s->c[id], &s->g[id], &s->alpha[id], &s->psi[id],
-                     &s->FP[id], s->sigma + 9 * id, &s->vol[id]);
+    /* computeForceCoefficient (ExplicitTimeIntegrator.cu:845-865), by Energy_Model */
+    for (int id = 0; id < s->n; ++id) {
+        if (s->p.model == 2)
+            mo_von_mises(&s->p, s->F + 9 * id, s->c[id], &s->g[id], &s->alpha[id], &s->psi[id],
+                         &s->FP[id], s->sigma + 9 * id, &s->vol[id]);
+        else if (s->p.model == 1)
+            mo_neo_hookean(&s->p, s->F + 9 * id, s->c[id], &s->psi[id], &s->FP[id], s->sigma + 9 * id,
+                           &s->vol[id]);
+        else
+            mo_fixed_corotated(&s->p, s->F + 9 * id, s->sigma + 9 * id);
+    }
     const size_t gbytes = sizeof(double) * NCH * (size_t)s->gcap * 64;
     p2g(s, dt);
     memcpy(s->grid_p2g, s->grid, gbytes);
     grid_update(s, dt);
     memcpy(s->grid_upd, s->grid, gbytes);
     int rc = 0;
-    if (s->p.phase_field) {
+    /* Energy_Model 0 has no fracture: integrate skips the solve (:905-907) */
+    if (s->p.phase_field && s->p.model != 0) {
         rc = phase_field(s, dt);
     } else {
         /* no damage solve: the damage increment is zero */

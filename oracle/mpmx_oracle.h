@@ -54,6 +54,7 @@ typedef struct mo_params {
     double vmax;         /* MLS |v| clamp 11.137 (G2PKernal.cu:134-142) */
     double cg_tol;       /* 1e-6 (ExplicitTimeIntegrator.cu:924) */
     double crit_stretch; /* psi+ history only grows while max stretch >= this; 0 = always (reference) */
+    int    model;        /* 2 von Mises (default), 1 Neo-Hookean phase field, 0 fixed corotated */
 } mo_params;
 
 /* reference defaults (constants listed above), N = 128, E = 1e5, nu = 0.3 */
@@ -95,6 +96,13 @@ void mo_inverse3(const double in[9], double out[9]);
  * stress contribution sigma = V0 * P * F^T and vol = V0 * det F. */
 void mo_von_mises(const mo_params* p, double F[9], double c, double* g, double* alpha,
                   double* psi, double* FP, double sigma[9], double* vol);
+
+/* Neo-Hookean phase-field energy (ExplicitTimeIntegrator.cu:372-446): psi+
+ * history, FP, sigma = V0 P F^T, vol = V0 det F; F unchanged */
+void mo_neo_hookean(const mo_params* p, const double F[9], double c, double* psi, double* FP, double sigma[9],
+                    double* vol);
+/* fixed corotated energy (ExplicitTimeIntegrator.cu:22-47): sigma = V0 P F^T */
+void mo_fixed_corotated(const mo_params* p, const double F[9], double sigma[9]);
 
 /* ---------------- whole-substep simulator ----------------------------------- */
 typedef struct mo_sim mo_sim;

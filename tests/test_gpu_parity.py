@@ -63,13 +63,16 @@ def test_keys_sort_tables_bitexact_after_steps(mpmx, oracle, vpage):
         oracle_tables_check(oracle, ctx, p, prev, x32)
 
 
-@pytest.mark.parametrize("stress_fp64,bar", [(1, 1e-6), (0, 1e-3)])
-def test_stress_parity(mpmx, oracle, stress_fp64, bar):
-    """Return mapping on identical FP32 inputs.  FP64 arithmetic (default):
-    only the FP32 rounding of the outputs remains, bound 1e-6 of each field's
-    scale.  FP32 arithmetic: the SVD of near-equal singular values and the
-    cancellation in y = |tau| - g R sqrt(2/3) cost up to ~1e-4; bound 1e-3."""
+@pytest.mark.parametrize("model,stress_fp64,bar", [(2, 1, 1e-6), (2, 0, 1e-3), (1, 1, 1e-6), (0, 1, 1e-6)])
+def test_stress_parity(mpmx, oracle, model, stress_fp64, bar):
+    """Constitutive update on identical FP32 inputs, for the von Mises (2),
+    Neo-Hookean (1) and fixed corotated (0) energies.  FP64 arithmetic
+    (default): only the FP32 rounding of the outputs remains, bound 1e-6 of
+    each field's scale.  FP32 arithmetic: the SVD of near-equal singular values
+    and the cancellation in y = |tau| - g R sqrt(2/3) cost up to ~1e-4; bound
+    1e-3."""
     p, x, v, m, dt = mpmx.scene(0, seed=1, scale=0.5)
+    p.model = model
     ctx = mpmx.Context(p, mpmx.default_config(stress_fp64=stress_fp64, svd_gap=0.0))
     ctx.set_particles(x, v, m)
     n = ctx.n
@@ -91,14 +94,21 @@ def test_stress_parity(mpmx, oracle, stress_fp64, bar):
     op = oracle_params(oracle, p)
     ref = {k: [] for k in gpu}
     for i in range(n):
-        r = oracle.von_mises(op, ins["F"][i], float(ins["c"][i]), float(ins["g"][i]),
-                             float(ins["alpha"][i]), float(ins["psi"][i]))
+        args = (float(ins["c"][i]), float(ins["g"][i]), float(ins["alpha"][i]), float(ins["psi"][i]))
+        if model == 2:
+            r = oracle.von_mises(op, ins["F"][i], *args)
+        elif model == 1:
+            r = oracle.neo_hookean(op, ins["F"][i], args[0], args[3])
+            r.update(F=ins["F"][i], g=args[1], alpha=args[2])
+        else:
+            r = {"sigma": oracle.fixed_corotated(op, ins["F"][i]), "F": ins["F"][i], "g": args[1],
+                 "alpha": args[2], "psi": args[3], "FP": 1.0, "vol": gpu["vol"][i]}
         for k in ref:
             ref[k].append(r[k])
     ref = {k: np.array(a) for k, a in ref.items()}
     errs = {k: max_rel_field(gpu[k], ref[k]) for k in ("F", "sigma", "FP", "vol", "g", "psi")}
     errs["alpha"] = np.abs(gpu["alpha"] - ref["alpha"]).max()
-    print("stress parity (fp64=%d):" % stress_fp64, {k: "%.1e" % e for k, e in errs.items()})
+    print("stress parity (model %d, fp64=%d):" % (model, stress_fp64), {k: "%.1e" % e for k, e in errs.items()})
     for k, e in errs.items():
         assert e < bar, k
 
@@ -133,20 +143,22 @@ def test_p2g_parity(mpmx, oracle, variant):
 STATE = ("x", "v", "F", "B", "c", "alpha", "psi", "g")
 
 
-@pytest.mark.parametrize("transfer", [0, 1])
-def test_one_step_from_gpu_state(mpmx, oracle, transfer):
+@pytest.mark.parametrize("transfer,variant,model", [(0, 2, 2), (1, 2, 2), (0, 1, 2), (1, 1, 2), (0, 2, 1), (1, 2, 0)])
+def test_one_step_from_gpu_state(mpmx, oracle, transfer, variant, model):
     """Shadow step: the GPU's own state 45 substeps into the impact (yielded,
     damaged particles) is loaded into the oracle and both advance one substep.
     The model amplifies rounding in yielded material over many substeps, so
     per-substep parity on identical inputs is the bar for every kernel."""
     p, x, v, m, dt = mpmx.scene(3, seed=1, scale=0.1)
     p.transfer = transfer
-    ctx = mpmx.Context(p)
+    p.model = model
+    ctx = mpmx.Context(p, mpmx.default_config(p2g_variant=variant))
     ctx.set_particles(x, v, m)
     ctx.run(45, dt)
     st = {k: ctx.get(k) for k in STATE}
     mass = ctx.get("mass")
-    assert (st["alpha"] > 0).sum() > 100 and st["c"].min() < 0.99
+    if model == 2:
+        assert (st["alpha"] > 0).sum() > 100 and st["c"].min() < 0.99
     sim = oracle.Sim(oracle_params(oracle, p), st["x"], st["v"], mass)
     for k in ("F", "B", "c", "alpha", "psi", "g"):
         sim.set(k, st[k])
@@ -155,7 +167,8 @@ def test_one_step_from_gpu_state(mpmx, oracle, transfer):
     errs = {k: max_rel_field(ctx.get(k), sim.get(k)) for k in ("x", "v", "F", "B", "psi")}
     for k in ("c", "alpha"):
         errs[k] = np.abs(ctx.get(k) - sim.get(k)).max()
-    print("one step (transfer %d):" % transfer, {k: "%.1e" % e for k, e in errs.items()})
+    print("one step (transfer %d, variant %d, model %d):" % (transfer, variant, model),
+          {k: "%.1e" % e for k, e in errs.items()})
     bars = {"x": 1e-6, "v": 1e-4, "F": 1e-5, "B": 1e-3, "psi": 1e-4, "c": 1e-4, "alpha": 1e-4}
     for k, e in errs.items():
         assert e < bars[k], k

@@ -488,3 +488,91 @@ def test_pcg_matches_dense_solve(oracle):
     ref = np.linalg.solve(A, bg[act])
     assert np.abs(rhs_o[act] - bg[act]).max() < 1e-12 * np.abs(bg).max()
     assert np.abs(xsol[act] - ref).max() < 1e-9 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------------- other energies
+def _fd_gradient(energy, F, h=1e-6):
+    P = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(3):
+            E = np.zeros((3, 3))
+            E[i, j] = h
+            P[i, j] = (energy(F + E) - energy(F - E)) / (2 * h)
+    return P
+
+
+def test_fixed_corotated_is_energy_gradient(oracle):
+    """sigma = V0 P F^T with P the gradient of the fixed corotated energy
+    mu sum (s_i - 1)^2 + lambda/2 (J - 1)^2 (central differences)."""
+    E, nu = 1e5, 0.3
+    lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+    p = oracle.default_params(E=E, nu=nu, V0=2.0, model=0)
+    rng = np.random.default_rng(21)
+
+    def psi(F):
+        s = np.linalg.svd(F, compute_uv=False)
+        return mu * ((s - 1) ** 2).sum() + lam / 2 * (np.linalg.det(F) - 1) ** 2
+
+    for _ in range(20):
+        F = np.eye(3) + 0.2 * rng.standard_normal((3, 3))
+        if np.linalg.det(F) < 0.3:
+            continue
+        sig = oracle.fixed_corotated(p, F)
+        P = sig @ np.linalg.inv(F).T / 2.0
+        Pfd = _fd_gradient(psi, F)
+        assert np.abs(P - Pfd).max() < 1e-5 * np.abs(Pfd).max()
+    ang = 0.3
+    R = np.array([[1, 0, 0], [0, np.cos(ang), -np.sin(ang)], [0, np.sin(ang), np.cos(ang)]])
+    assert np.abs(oracle.fixed_corotated(p, R)).max() < 1e-9
+
+
+def test_neo_hookean_stress_is_energy_gradient(oracle):
+    """The reference fills P_hat with the principal Kirchhoff stresses of the
+    Neo-Hookean split g mu/2 (tr b_bar - 3) + w kappa/2 ((J^2 - 1)/2 - ln J)
+    (w = g for J >= 1, 1 for J < 1) and assembles V0 U diag(P_hat) V^T F^T,
+    which is V0 tau sqrt(F F^T): tau from central differences of the energy,
+    the left stretch from the eigen decomposition of F F^T.  Rotations carry
+    no stress."""
+    E, nu = 1e5, 0.3
+    mu, kappa = E / (2 * (1 + nu)), E / (3 * (1 - 2 * nu))
+    p = oracle.default_params(E=E, nu=nu, V0=1.0, model=1)
+    rng = np.random.default_rng(22)
+    c = 0.7
+    g = c * c * (1 - p.theta) + p.theta
+    for target in (0.9, 1.1):
+        for _ in range(10):
+            A = np.eye(3) + 0.05 * rng.standard_normal((3, 3))
+            F = A * (target / np.linalg.det(A)) ** (1 / 3)   # J = target
+
+            def psi(Fx):
+                J = np.linalg.det(Fx)
+                b = Fx @ Fx.T * J ** (-2 / 3)
+                w = g if J >= 1 else 1.0
+                return g * mu / 2 * (np.trace(b) - 3) + w * kappa / 2 * ((J * J - 1) / 2 - np.log(J))
+
+            r = oracle.neo_hookean(p, F, c, 0.0)
+            tau_fd = _fd_gradient(psi, F) @ F.T
+            lam_b, Q = np.linalg.eigh(F @ F.T)
+            stretch = Q @ np.diag(np.sqrt(lam_b)) @ Q.T
+            expect = tau_fd @ stretch
+            assert np.abs(r["sigma"] - expect).max() < 1e-5 * np.abs(expect).max()
+            assert r["vol"] == pytest.approx(np.linalg.det(F), rel=1e-12)
+    r = oracle.neo_hookean(p, np.array([[0.0, -1, 0], [1, 0, 0], [0, 0, 1]]), c, 0.0)
+    assert np.abs(r["sigma"]).max() < 1e-9 and r["psi"] == 0.0
+
+
+def test_neo_hookean_history_at_unit_volume(oracle):
+    """J = 1: psi+ is the deviatoric energy mu/2 (sum s_i^2 - 3) alone, taken
+    only when it exceeds the history, and FP = 4 L0 (1 - 0.001) psi / 3 + 1."""
+    E, nu, N = 1e5, 0.3, 64
+    mu = E / (2 * (1 + nu))
+    p = oracle.default_params(E=E, nu=nu, V0=1.0, N=N, model=1)
+    s = np.array([1.1, 1 / 1.1 / 0.95, 0.95])
+    F = np.diag(s)
+    dev = mu / 2 * ((s ** 2).sum() - 3)
+    r = oracle.neo_hookean(p, F, 1.0, 0.0)
+    assert r["psi"] == pytest.approx(dev, rel=1e-10)
+    L0 = 0.5 / N
+    assert r["FP"] == pytest.approx(4 * L0 * (1 - float(np.float32(0.001))) * dev / 3 + 1, rel=1e-12)
+    r2 = oracle.neo_hookean(p, F, 1.0, 2 * dev)
+    assert r2["psi"] == 2 * dev
