@@ -34,12 +34,14 @@ inline BlockTables block_tables_of(const mpmx_ctx* c) {
                        c->cap_p, c->cap_tp, c->tp_bound, 1 << c->axis_bits};
 }
 
-// ---- shared pieces of the single-wave block gathers --------------------------
+// ---- shared pieces of the block gathers ---------------------------------------
 // The particle pages 2b-1 .. 2b+1 (per axis) of block b, their particle ranges
 // and the cell filter that keeps the particles reaching the block; lanes 32..39
-// resolve the block's 8 grid pages.  One wavefront; LDS arrays owned by the caller.
+// resolve the block's 8 grid pages.  Called by one wavefront (the first of the
+// workgroup); LDS arrays owned by the caller.
 struct BlockSources {
     int start[27], pre[28], filter[27];
+    int page[27];          // particle page id of each source, or -1
     int out_page[8];
 };
 
@@ -50,11 +52,12 @@ __device__ __forceinline__ void block_sources(const BlockTables& t, uint32_t bke
     if (lane < 27) {
         const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
         const int px = 2 * bc[0] + dx, py = 2 * bc[1] + dy, pz = 2 * bc[2] + dz;
-        int st = 0;
+        int st = 0, page_of_source = -1;
         if (px >= 0 && py >= 0 && pz >= 0 && px < t.axis_pages && py < t.axis_pages && pz < t.axis_pages) {
             const int id = page_rank(morton((uint32_t)px, (uint32_t)py, (uint32_t)pz), t.pmask, t.nmask, t.wpre,
                                      t.spre, np_raw);
             if (id >= 0 && id < np) {
+                page_of_source = id;
                 int a = t.page2particle[id], b = t.page2particle[id + 1];
                 a = a < 0 ? 0 : (a > n ? n : a);
                 b = b < a ? a : (b > n ? n : b);
@@ -63,6 +66,7 @@ __device__ __forceinline__ void block_sources(const BlockTables& t, uint32_t bke
             }
         }
         S.start[lane] = st;
+        S.page[lane] = cnt > 0 ? page_of_source : -1;
         S.filter[lane] = (dx < 0 ? 1 : 0) | (dy < 0 ? 2 : 0) | (dz < 0 ? 4 : 0);
     } else if (lane >= 32 && lane < 40) {
         const int o = lane - 32;
@@ -79,40 +83,51 @@ __device__ __forceinline__ void block_sources(const BlockTables& t, uint32_t bke
     if (lane == 26) S.pre[27] = incl;
 }
 
-// entry j of the concatenated source ranges: particle index, whether it
-// reaches the block, and its rank inside its run of equal keys within the
-// wave (lanes of one rank never share a cell); rmax is the wave's largest rank
-__device__ __forceinline__ bool block_entry(const BlockTables& t, const BlockSources& S, int j, int total, int& i,
-                                            int& rank, int& rmax) {
-    const int lane = threadIdx.x;
-    bool valid = j < total;
-    uint32_t key = 0xffffffffu;
+// entry j of the concatenated source ranges: particle index and whether it
+// reaches the block (any thread count)
+__device__ __forceinline__ bool block_entry_simple(const BlockTables& t, const BlockSources& S, int j, int total,
+                                                   int& i) {
     i = 0;
-    if (valid) {
-        int lo = 0, hi = 26;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (S.pre[mid] <= j) lo = mid;
-            else hi = mid - 1;
-        }
-        i = S.start[lo] + (j - S.pre[lo]);
-        key = t.keys[i];
-        const int f = S.filter[lo];
-        const uint32_t cell = key & 63u;
-        if (((f & 1) && !((cell >> 5) & 1u)) || ((f & 2) && !((cell >> 3) & 1u)) || ((f & 4) && !((cell >> 1) & 1u)))
-            valid = false;
+    if (j >= total) return false;
+    int lo = 0, hi = 26;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S.pre[mid] <= j) lo = mid;
+        else hi = mid - 1;
     }
-    // particles of one cell are adjacent; filtered lanes never split a run
-    const uint32_t kprev = __shfl_up(key, 1, 64);
-    const bool start = valid && (lane == 0 || kprev != key);
-    const unsigned long long starts = __ballot(start);
-    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    const unsigned long long sb = starts & upto;
-    rank = valid ? lane - (63 - __clzll((long long)sb)) : 0;
-    rmax = rank;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) rmax = max(rmax, __shfl_xor(rmax, o, 64));
-    return valid;
+    i = S.start[lo] + (j - S.pre[lo]);
+    const int f = S.filter[lo];
+    if (!f) return true;
+    const uint32_t cell = t.keys[i] & 63u;
+    return !(((f & 1) && !((cell >> 5) & 1u)) || ((f & 2) && !((cell >> 3) & 1u)) ||
+             ((f & 4) && !((cell >> 1) & 1u)));
+}
+
+// ---- fixed-point accumulation -----------------------------------------------
+// LDS float atomics cost ~190 CU cycles per wave instruction on gfx950 and
+// 64-bit integer atomics ~7, so the block gathers add in fixed point: a
+// channel whose largest single contribution in the block is b uses the scale
+// 2^e with b 2^e <= 2^30, each contribution is truncated to a 32-bit integer
+// (three VALU operations: scale, convert, sign extension) and added to a
+// 64-bit accumulator, which holds 2^33 such terms.  Integer sums are exact and
+// order-free: the result is bit-reproducible, and each term carries an error
+// below 2^-30 of the block's largest contribution (finer than an FP32 sum).
+constexpr int kFixBits = 30;
+
+// exponent e for a channel bound b (>= 0): b 2^e <= 2^kFixBits
+__device__ __forceinline__ int fix_exp(float b) {
+    if (!(b > 0.f) || !isfinite(b)) return 0;
+    int ex;
+    frexpf(b, &ex);   // b < 2^ex
+    const int e = kFixBits - ex;
+    return e < -120 ? -120 : (e > 120 ? 120 : e);
+}
+// x scaled by sc = 2^e (|x sc| <= 2^30), truncated, sign-extended to 64 bits
+__device__ __forceinline__ unsigned long long to_fix(float x, float sc) {
+    return (unsigned long long)(long long)(int)(x * sc);
+}
+__device__ __forceinline__ float from_fix(unsigned long long v, int e) {
+    return (float)ldexp((double)(long long)v, -e);
 }
 
 // node index inside an 8^3 block accumulator; z padded to 9 so that the nodes

@@ -271,7 +271,7 @@ static void free_particles(mpmx_ctx* c) {
         dfree(m.x); dfree(m.v); dfree(m.B); dfree(m.F); dfree(m.c); dfree(m.alpha); dfree(m.psi); dfree(m.g);
         dfree(m.mass); dfree(m.id);
     }
-    dfree(c->sigma); dfree(c->FP); dfree(c->vol); dfree(c->cg_col);
+    dfree(c->sigma); dfree(c->FP); dfree(c->vol); dfree(c->cg_col); dfree(c->payload); dfree(c->pbound);
     dfree(c->keys); dfree(c->keys_alt); dfree(c->vals); dfree(c->vals_alt); dfree(c->hist);
     dfree(c->pmask); dfree(c->nmask); dfree(c->wcount); dfree(c->scount);
     dfree(c->page_key); dfree(c->page2particle); dfree(c->adj); dfree(c->nvp); dfree(c->vpo); dfree(c->target); dfree(c->block_key); dfree(c->bcount);
@@ -464,6 +464,7 @@ mpmx_status mpmx_set_particles(mpmx_ctx* c, int64_t n, const double* x, const do
     HIPCHK(c, dalloc(&c->FP, n));
     HIPCHK(c, dalloc(&c->vol, n));
     HIPCHK(c, dalloc(&c->cg_col, 3 * n));
+    HIPCHK(c, dalloc(&c->payload, 28 * n));
     HIPCHK(c, dalloc(&c->keys, n));
     HIPCHK(c, dalloc(&c->keys_alt, n));
     HIPCHK(c, dalloc(&c->vals, n));
@@ -487,6 +488,7 @@ mpmx_status mpmx_set_particles(mpmx_ctx* c, int64_t n, const double* x, const do
     HIPCHK(c, dalloc(&c->page_key, tp_bound));
     c->tp_bound = tp_bound;
     HIPCHK(c, dalloc(&c->block_key, tp_bound));
+    HIPCHK(c, dalloc(&c->pbound, 8 * (c->cap_p + 1)));
     HIPCHK(c, dalloc(&c->bcount, c->mask_words));
     HIPCHK(c, dalloc(&c->page2particle, c->cap_p + 1));
     HIPCHK(c, dalloc(&c->adj, 7 * c->cap_p));

@@ -8,22 +8,18 @@
 //   grid  undateGrid_kernal + collideWithGround / PullGrid (TimeIntegrator.cu:
 //         63-261) and the PCG initialisation of ExplicitTimeIntegrator.cu:914-935.
 //
-// Gather form: a workgroup of one wavefront owns one 2x2x2 block of grid
-// pages (8^3 nodes, the pages 8b .. 8b+7 in Morton order) at a time.
-// Particles of page P reach the nodes of P and its +1 neighbours, so block b
-// receives from the 27 particle pages 2b-1 .. 2b+1 (per axis); of a page below
-// the block along an axis only the particles whose cell has index & 3 >= 2 on
-// that axis reach it.  The wave walks the concatenated particle ranges of
-// those pages and adds each particle's 8 channels to the block's nodes in LDS
-// with plain read-add-write: LDS float atomics cost ~190 CU cycles per wave
-// instruction on gfx950 against ~18 for a read-add-write, and with one wave
-// per block no other wave touches the accumulator.  Lanes of one wave never
-// hit the same node in the same instruction: particles of one cell (adjacent
-// after the sort) are spread over rounds, and a compiler barrier between
-// stencil slots keeps the in-order LDS semantics across lanes.  The wave
-// then updates its nodes in place and writes them once: no atomics at all,
-// no zero fill of the grid.  Each particle is read by the one to eight blocks
-// it reaches (about twice on average).
+// Gather form: a workgroup owns one 2x2x2 block of grid pages (8^3 nodes, the
+// pages 8b .. 8b+7 in Morton order) at a time.  Particles of page P reach the
+// nodes of P and its +1 neighbours, so block b receives from the 27 particle
+// pages 2b-1 .. 2b+1 (per axis); of a page below the block along an axis only
+// the particles whose cell has index & 3 >= 2 on that axis reach it.  The
+// workgroup walks the concatenated particle ranges of those pages twice: once
+// for the largest contribution per channel (the fixed-point scale), once to
+// add every contribution to the block's nodes in LDS with 64-bit integer
+// atomics.  It then updates its nodes in place and writes them once: no
+// global atomics, no zero fill of the grid, and sums that do not depend on
+// the order of the additions.  Each particle is read by the one to eight
+// blocks it reaches (about twice on average).
 #include "access.h"
 
 namespace mpmx {
@@ -126,10 +122,26 @@ __device__ __forceinline__ void p2g_terms(const DevParams& dp, const ST& s, cons
     for (int r = 0; r < 3; ++r) T.ar[r] = T.m * v[r] - (T.Qd[r] * d0[0] + T.Qd[r + 3] * d0[1] + T.Qd[r + 6] * d0[2]);
 }
 
-// add the particle's contributions to the 8^3 block accumulator (read-add-
-// write; the caller guarantees that no two active lanes share a cell)
+// largest |contribution| of the particle to any node, per channel (weights <= 1,
+// gradients of the quadratic B-spline <= 1 in magnitude per axis)
 template <int TRANSFER>
-__device__ __forceinline__ void p2g_rmw(const P2GTerms& T, float* acc) {
+__device__ __forceinline__ void p2g_bounds(const P2GTerms& T, float (&b)[kP2GCh]) {
+    b[0] = fabsf(T.m);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        float q = fabsf(T.ar[r]) + 2.f * (fabsf(T.Qd[r]) + fabsf(T.Qd[r + 3]) + fabsf(T.Qd[r + 6]));
+        if (TRANSFER == 1) q += fabsf(T.sg[r]) + fabsf(T.sg[r + 3]) + fabsf(T.sg[r + 6]);
+        b[1 + r] = q;
+    }
+    b[4] = fabsf(T.cphase);
+    b[5] = 1.f;
+    b[6] = fabsf(T.V);
+    b[7] = fabsf(T.VF);
+}
+
+// add the particle's contributions to the block's 8^3 nodes in fixed point
+template <int TRANSFER>
+__device__ __forceinline__ void p2g_fix(const P2GTerms& T, const float (&sc)[kP2GCh], unsigned long long* acc) {
 #pragma unroll
     for (int ii = 0; ii < 3; ++ii) {
         const int la = T.l0[0] + ii;
@@ -155,62 +167,186 @@ __device__ __forceinline__ void p2g_rmw(const P2GTerms& T, float* acc) {
 #pragma unroll
                     for (int r = 0; r < 3; ++r) pr[r] -= T.sg[r] * g0 + T.sg[r + 3] * g1 + T.sg[r + 6] * g2;
                 }
-                float* a = acc + (la * 8 + lb) * kAccZ + lc;
-                // all eight reads, then all eight writes: one LDS round trip
-                float o[kP2GCh];
-#pragma unroll
-                for (int k = 0; k < kP2GCh; ++k) o[k] = a[k * kAccStride];
+                unsigned long long* a = acc + (la * 8 + lb) * kAccZ + lc;
                 const float add[kP2GCh] = {wt * T.m, pr[0], pr[1], pr[2], wt * T.cphase, wt, wt * T.V, wt * T.VF};
 #pragma unroll
-                for (int k = 0; k < kP2GCh; ++k) a[k * kAccStride] = o[k] + add[k];
-                // another lane may add to this node in a later slot
-                __asm__ volatile("" ::: "memory");
+                for (int k = 0; k < kP2GCh; ++k) atomicAdd(a + k * kAccStride, to_fix(add[k], sc[k]));
             }
         }
     }
 }
 
+// ---- per-particle P2G payload ------------------------------------------------
+// The terms every node contribution of a particle needs, computed once per
+// substep in sorted order and stored as float4 records: d0 (the particle's
+// offset from its base node in cells), m, ar, Qd, c, V, V FP and, for APIC,
+// the stress term sg (MLS: 5 records of 4 floats; APIC: 7, sg[8] in the pad
+// slot of the fifth).  The block P2G reads one record per visit (a particle
+// is visited by the one to eight blocks it reaches) instead of recomputing
+// them from 26 scattered fields; the base node itself comes from the key.
+template <int TRANSFER> struct Payload { static constexpr int kFloats = TRANSFER ? 28 : 20; };
+
+template <int TRANSFER>
+__device__ __forceinline__ void payload_store(float* __restrict__ pay, int64_t i, const float (&d0)[3],
+                                              const P2GTerms& T) {
+    float4* q = reinterpret_cast<float4*>(pay + i * Payload<TRANSFER>::kFloats);
+    q[0] = make_float4(d0[0], d0[1], d0[2], T.m);
+    q[1] = make_float4(T.ar[0], T.ar[1], T.ar[2], T.cphase);
+    q[2] = make_float4(T.Qd[0], T.Qd[1], T.Qd[2], T.Qd[3]);
+    q[3] = make_float4(T.Qd[4], T.Qd[5], T.Qd[6], T.Qd[7]);
+    q[4] = make_float4(T.Qd[8], T.V, T.VF, TRANSFER == 1 ? T.sg[8] : 0.f);
+    if (TRANSFER == 1) {
+        q[5] = make_float4(T.sg[0], T.sg[1], T.sg[2], T.sg[3]);
+        q[6] = make_float4(T.sg[4], T.sg[5], T.sg[6], T.sg[7]);
+    }
+}
+
+// the terms of particle i (key k) relative to the block whose first node is org
+template <int TRANSFER>
+__device__ __forceinline__ void payload_load(const float* __restrict__ pay, int64_t i, uint32_t key,
+                                             const int (&org)[3], P2GTerms& T) {
+    const float4* q = reinterpret_cast<const float4*>(pay + i * Payload<TRANSFER>::kFloats);
+    const float4 a = q[0], b = q[1], c0 = q[2], c1 = q[3], c2 = q[4];
+    const float d0[3] = {a.x, a.y, a.z};
+    T.m = a.w;
+    T.ar[0] = b.x; T.ar[1] = b.y; T.ar[2] = b.z;
+    T.cphase = b.w;
+    T.Qd[0] = c0.x; T.Qd[1] = c0.y; T.Qd[2] = c0.z; T.Qd[3] = c0.w;
+    T.Qd[4] = c1.x; T.Qd[5] = c1.y; T.Qd[6] = c1.z; T.Qd[7] = c1.w;
+    T.Qd[8] = c2.x; T.V = c2.y; T.VF = c2.z;
+    if (TRANSFER == 1) {
+        const float4 s0 = q[5], s1 = q[6];
+        T.sg[0] = s0.x; T.sg[1] = s0.y; T.sg[2] = s0.z; T.sg[3] = s0.w;
+        T.sg[4] = s1.x; T.sg[5] = s1.y; T.sg[6] = s1.z; T.sg[7] = s1.w;
+        T.sg[8] = c2.w;
+    }
+    int px, py, pz;
+    unmorton(key >> 6, px, py, pz);
+    const int cell = (int)(key & 63u);
+    T.l0[0] = 4 * px + (cell >> 4) - org[0];
+    T.l0[1] = 4 * py + ((cell >> 2) & 3) - org[1];
+    T.l0[2] = 4 * pz + (cell & 3) - org[2];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        if (TRANSFER == 0) bspline(d0[ax], T.w[ax]);
+        else bspline_grad(d0[ax], T.w[ax], T.gw[ax]);
+    }
+}
+
+// payload of every particle and, per particle page, the largest contribution
+// per channel (a wave-segmented max over the page's contiguous particles, then
+// one integer atomicMax per page and channel on the float bits, which order
+// like the floats for non-negative values); pbound is zero on entry
+template <int TRANSFER, typename ST>
+__global__ __launch_bounds__(256) void k_p2g_prep(DevParams dp, ST s, const float* __restrict__ sigma,
+                                                  const float* __restrict__ FP, const float* __restrict__ vol,
+                                                  BlockTables t, float* __restrict__ pay, unsigned* __restrict__ pbound,
+                                                  float dt, int n, const DevCounts* __restrict__ dc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool valid = i < n;
+    float b[kP2GCh];
+    uint32_t pk = 0xffffffffu;
+    if (valid) {
+        const int org[3] = {0, 0, 0};
+        P2GTerms T;
+        p2g_terms<TRANSFER>(dp, s, sigma, FP, vol, i, n, dt, org, T);
+        float d0[3];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const float xa = ld_x(s, i, ax);
+            d0[ax] = xa * dp.inv_dx - (float)base_node(xa, dp.N);
+        }
+        payload_store<TRANSFER>(pay, i, d0, T);
+        p2g_bounds<TRANSFER>(T, b);
+        pk = t.keys[i] >> 6;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kP2GCh; ++k) b[k] = 0.f;
+    }
+    const int np_raw = dc->num_page;
+    const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
+    unsigned long long todo = __ballot(valid);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const uint32_t lpk = __shfl(pk, leader, 64);
+        const bool mine = valid && pk == lpk;
+        todo &= ~__ballot(mine);
+        float m[kP2GCh];
+#pragma unroll
+        for (int k = 0; k < kP2GCh; ++k) m[k] = wave_max(mine ? b[k] : 0.f);
+        if (lane == leader) {
+            const int id = page_rank(lpk, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
+            if (id >= 0 && id < np) {
+#pragma unroll
+                for (int k = 0; k < kP2GCh; ++k) atomicMax(&pbound[(int64_t)id * kP2GCh + k], __float_as_uint(m[k]));
+            }
+        }
+    }
+}
 
 template <int TRANSFER, typename ST>
-__global__ __launch_bounds__(64) void k_p2g_block(DevParams dp, ST s, const float* __restrict__ sigma,
-                                                  const float* __restrict__ FP, const float* __restrict__ vol,
-                                                  BlockTables t, float* __restrict__ grid,
-                                                  float* __restrict__ grid_raw, CGVec cg, float dt, int n,
-                                                  const DevCounts* __restrict__ dc) {
-    __shared__ float acc[kP2GCh * kAccStride];
+__global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float* __restrict__ pay,
+                                                   const unsigned* __restrict__ pbound, BlockTables t,
+                                                   float* __restrict__ grid, float* __restrict__ grid_raw, CGVec cg,
+                                                   float dt, int n, const DevCounts* __restrict__ dc) {
+    __shared__ unsigned long long acc[kP2GCh * kAccStride];
     __shared__ BlockSources S;
-    const int lane = threadIdx.x;
+    __shared__ int sexp[kP2GCh];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int np_raw = dc->num_page;
     const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
     int nt = dc->num_total;
     nt = nt < t.cap_tp ? nt : (int)t.cap_tp;
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
-    int bb, be;
-    block_range(nb, bb, be);
-    for (int blk = bb; blk < be; ++blk) {
+    FOR_LOGICAL_ITEMS(blk, nb) {
         const uint32_t bkey = t.block_key[blk];
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
-        block_sources(t, bkey, bc, n, np_raw, np, nt, S);
-        for (int q = lane; q < kP2GCh * kAccStride; q += 64) acc[q] = 0.f;
+        if (wave == 0) block_sources(t, bkey, bc, n, np_raw, np, nt, S);
+        for (int q = tid; q < kP2GCh * kAccStride; q += 256) acc[q] = 0ull;
         __syncthreads();
         const int total = S.pre[27];
         const int org[3] = {8 * bc[0], 8 * bc[1], 8 * bc[2]};
-        for (int j0 = 0; j0 < total; j0 += 64) {
-            int i, rank, rmax;
-            const bool valid = block_entry(t, S, j0 + lane, total, i, rank, rmax);
-            if (!__any(valid)) continue;
-            P2GTerms T;
-            if (valid) p2g_terms<TRANSFER>(dp, s, sigma, FP, vol, i, n, dt, org, T);
-            for (int r = 0; r <= rmax; ++r) {
-                if (valid && rank == r) p2g_rmw<TRANSFER>(T, acc);
-                __asm__ volatile("" ::: "memory");
+        // scales: the largest contribution per channel of the 27 source pages
+        if (wave == 0) {
+            float bmax[kP2GCh];
+#pragma unroll
+            for (int k = 0; k < kP2GCh; ++k) {
+                const int pg = lane < 27 ? S.page[lane] : -1;
+                bmax[k] = wave_max(pg >= 0 ? __uint_as_float(pbound[(int64_t)pg * kP2GCh + k]) : 0.f);
+            }
+            if (lane < kP2GCh) {
+                float v = bmax[0];
+#pragma unroll
+                for (int k = 1; k < kP2GCh; ++k) v = (lane == k) ? bmax[k] : v;
+                sexp[lane] = fix_exp(v);
             }
         }
         __syncthreads();
+        int e[kP2GCh];
+        float sc[kP2GCh];
+#pragma unroll
+        for (int k = 0; k < kP2GCh; ++k) {
+            e[k] = sexp[k];
+            sc[k] = ldexpf(1.0f, e[k]);
+        }
+        // thread tid takes the run of entries [tid per, (tid + 1) per): the
+        // particles of one cell (adjacent after the sort) go through one lane
+        // in turn instead of hitting the same nodes in one instruction
+        const int per = (total + 255) / 256;
+        for (int k = 0; k < per; ++k) {
+            const int j = tid * per + k;
+            int i;
+            if (!block_entry_simple(t, S, j, total, i)) continue;
+            P2GTerms T;
+            payload_load<TRANSFER>(pay, i, t.keys[i], org, T);
+            p2g_fix<TRANSFER>(T, sc, acc);
+        }
+        __syncthreads();
         // grid update of the block's pages, written page-major (coalesced)
-        for (int q = lane; q < 512; q += 64) {
+        for (int q = tid; q < 512; q += 256) {
             const int o = q >> 6, cell = q & 63;
             const int G = S.out_page[o];
             if (G < 0) continue;
@@ -219,7 +355,7 @@ __global__ __launch_bounds__(64) void k_p2g_block(DevParams dp, ST s, const floa
             const int node = (lx * 8 + ly) * kAccZ + lz;
             float ch[kP2GCh];
 #pragma unroll
-            for (int k = 0; k < kP2GCh; ++k) ch[k] = acc[k * kAccStride + node];
+            for (int k = 0; k < kP2GCh; ++k) ch[k] = from_fix(acc[k * kAccStride + node], e[k]);
             if (grid_raw) {
                 float* rp = grid_raw + (int64_t)G * kPageFloats + cell;
 #pragma unroll
@@ -330,24 +466,28 @@ hipError_t launch_cg_reset(mpmx_ctx* c) {
 
 static CGVec cg_vec(mpmx_ctx* c) { return CGVec{c->cg_x, c->cg_r, c->cg_z, c->cg_s, c->cg_q}; }
 
-template <typename ST>
-static void gather_dispatch(mpmx_ctx* c, ST st, float dt) {
+template <int TRANSFER, typename ST>
+static hipError_t gather_dispatch_t(mpmx_ctx* c, ST st, float dt) {
     const BlockTables t = block_tables_of(c);
     float* raw = (c->keep_p2g && c->grid_p2g) ? c->grid_p2g : nullptr;
-    // one wave per workgroup; ~8 resident per CU (18.4 KiB of LDS each)
-    int64_t g = (c->tp_bound + 7) / 8 * 8;
-    g = g < (int64_t)c->num_cu * 16 ? g : (int64_t)c->num_cu * 16;
-    MPMX_TIMED(c, "p2g_block", ({
-                   if (c->P.transfer == 0)
-                       k_p2g_block<0, ST><<<(int)g, 64, 0, c->stream>>>(c->dp, st, c->sigma, c->FP, c->vol, t, c->grid,
-                                                                         raw, cg_vec(c), dt, (int)c->n, c->dc);
-                   else
-                       k_p2g_block<1, ST><<<(int)g, 64, 0, c->stream>>>(c->dp, st, c->sigma, c->FP, c->vol, t, c->grid,
-                                                                         raw, cg_vec(c), dt, (int)c->n, c->dc);
-               }));
+    hipError_t e = fill_zero_dev_count(c, reinterpret_cast<float*>(c->pbound), &c->dc->num_page, kP2GCh, c->cap_p);
+    if (e != hipSuccess) return e;
+    const int n = (int)c->n;
+    MPMX_TIMED(c, "p2g_prep", (k_p2g_prep<TRANSFER, ST><<<(n + 255) / 256, 256, 0, c->stream>>>(
+                                   c->dp, st, c->sigma, c->FP, c->vol, t, c->payload, c->pbound, dt, n, c->dc)));
+    // about one workgroup per block: the hardware balances their uneven work
+    const int g = block_grid(c);
+    MPMX_TIMED(c, "p2g_block", (k_p2g_block<TRANSFER, ST><<<g, 256, 0, c->stream>>>(
+                                    c->dp, c->payload, c->pbound, t, c->grid, raw, cg_vec(c), dt, n, c->dc)));
     if (c->dp.phase_field)
         k_cg_rho0<<<c->cg_nb, 256, 0, c->stream>>>(c->dp, c->cg_r, c->cg_z, c->cg_part + c->cg_nb, c->cap_tp,
                                                     c->dc);
+    return hipGetLastError();
+}
+
+template <typename ST>
+static hipError_t gather_dispatch(mpmx_ctx* c, ST st, float dt) {
+    return c->P.transfer == 0 ? gather_dispatch_t<0>(c, st, dt) : gather_dispatch_t<1>(c, st, dt);
 }
 
 // P2G; the gather variant (default) also runs the grid update and PCG set-up
@@ -358,10 +498,9 @@ hipError_t launch_p2g(mpmx_ctx* c, float dt) {
         c->grid_fused = false;
         return launch_p2g_scatter(c, dt);
     }
-    if (c->cfg.precision == MPMX_MIXED) gather_dispatch(c, c->ms[c->cur], dt);
-    else gather_dispatch(c, c->ps[c->cur], dt);
+    e = (c->cfg.precision == MPMX_MIXED) ? gather_dispatch(c, c->ms[c->cur], dt) : gather_dispatch(c, c->ps[c->cur], dt);
     c->grid_fused = true;
-    return hipGetLastError();
+    return e;
 }
 
 hipError_t launch_grid_update(mpmx_ctx* c, float dt) {

@@ -143,10 +143,8 @@ __global__ __launch_bounds__(256) void k_cg_col(DevParams dp, ST s, const float*
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
     const float para_scale = 4.0f * dp.L0 * dp.L0 * dp.inv_dx * dp.inv_dx;
-    int bb, be;
-    block_range(nb, bb, be);
     const int tid = threadIdx.x;
-    for (int blk = bb; blk < be; ++blk) {
+    FOR_LOGICAL_ITEMS(blk, nb) {
         const uint32_t bkey = t.block_key[blk];
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
@@ -225,79 +223,87 @@ __global__ __launch_bounds__(256) void k_cg_col(DevParams dp, ST s, const float*
     }
 }
 
-// q_i = sum_p grad w_ip . col_p + D_i s_i over the 8^3 nodes of a block: a
-// single-wave gather, read-add-write in LDS with the same-cell rounds of the
-// block P2G; q is written whole (no zero fill, no atomics)
+// q_i = sum_p grad w_ip . col_p + D_i s_i over the 8^3 nodes of a block, as a
+// gather with 64-bit fixed-point LDS atomics (|grad w . col| <= sum |col|
+// gives the scale); q is written whole (no zero fill, no global atomics)
 template <typename ST>
-__global__ __launch_bounds__(64) void k_cg_q(DevParams dp, ST s, BlockTables t, const float* __restrict__ col,
-                                             const float* __restrict__ grid, const float* __restrict__ cgs,
-                                             float* __restrict__ cgq, int n, DevCounts* dc) {
-    __shared__ float acc[kAccStride];
+__global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t, const float* __restrict__ col,
+                                              const float* __restrict__ grid, const float* __restrict__ cgs,
+                                              float* __restrict__ cgq, int n, DevCounts* dc) {
+    __shared__ unsigned long long acc[kAccStride];
     __shared__ BlockSources S;
+    __shared__ float wmax[4];
+    __shared__ int sexp;
     if (cg_stopped(dc)) return;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int np_raw = dc->num_page;
     const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
     int nt = dc->num_total;
     nt = nt < t.cap_tp ? nt : (int)t.cap_tp;
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
-    int bb, be;
-    block_range(nb, bb, be);
-    for (int blk = bb; blk < be; ++blk) {
+    FOR_LOGICAL_ITEMS(blk, nb) {
         const uint32_t bkey = t.block_key[blk];
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
-        block_sources(t, bkey, bc, n, np_raw, np, nt, S);
-        for (int q = lane; q < kAccStride; q += 64) acc[q] = 0.f;
+        if (wave == 0) block_sources(t, bkey, bc, n, np_raw, np, nt, S);
+        for (int q = tid; q < kAccStride; q += 256) acc[q] = 0ull;
         __syncthreads();
         const int total = S.pre[27];
         const int org[3] = {8 * bc[0], 8 * bc[1], 8 * bc[2]};
-        for (int j0 = 0; j0 < total; j0 += 64) {
-            int i, rank, rmax;
-            const bool valid = block_entry(t, S, j0 + lane, total, i, rank, rmax);
-            if (!__any(valid)) continue;
+        float bm = 0.f;
+        const int per = (total + 255) / 256;   // runs of entries per thread (see k_p2g_block)
+        for (int k = 0; k < per; ++k) {
+            const int j = tid * per + k;
+            int i;
+            if (!block_entry_simple(t, S, j, total, i)) continue;
+            bm = fmaxf(bm, fabsf(col[3 * (int64_t)i]) + fabsf(col[3 * (int64_t)i + 1]) +
+                               fabsf(col[3 * (int64_t)i + 2]));
+        }
+        bm = wave_max(bm);
+        if (lane == 0) wmax[wave] = bm;
+        __syncthreads();
+        if (tid == 0) sexp = fix_exp(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3])));
+        __syncthreads();
+        const int e = sexp;
+        const float sc = ldexpf(1.0f, e);
+        for (int k = 0; k < per; ++k) {
+            const int j = tid * per + k;
+            int i;
+            if (!block_entry_simple(t, S, j, total, i)) continue;
             float w[3][3], gw[3][3], c3[3];
             int l0[3];
-            if (valid) {
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const float xa = ld_x(s, i, a);
-                    const int b = base_node(xa, dp.N);
-                    const float d0 = xa * dp.inv_dx - (float)b;
-                    l0[a] = b - org[a];
-                    bspline_grad(d0, w[a], gw[a]);
-                    c3[a] = col[3 * (int64_t)i + a];
-                }
+            for (int a = 0; a < 3; ++a) {
+                const float xa = ld_x(s, i, a);
+                const int b = base_node(xa, dp.N);
+                const float d0 = xa * dp.inv_dx - (float)b;
+                l0[a] = b - org[a];
+                bspline_grad(d0, w[a], gw[a]);
+                c3[a] = col[3 * (int64_t)i + a];
             }
-            for (int r = 0; r <= rmax; ++r) {
-                if (valid && rank == r) {
 #pragma unroll
-                    for (int ii = 0; ii < 3; ++ii) {
-                        const int la = l0[0] + ii;
-                        if (la < 0 || la >= 8) continue;
+            for (int ii = 0; ii < 3; ++ii) {
+                const int la = l0[0] + ii;
+                if (la < 0 || la >= 8) continue;
 #pragma unroll
-                        for (int jj = 0; jj < 3; ++jj) {
-                            const int lb = l0[1] + jj;
-                            if (lb < 0 || lb >= 8) continue;
+                for (int jj = 0; jj < 3; ++jj) {
+                    const int lb = l0[1] + jj;
+                    if (lb < 0 || lb >= 8) continue;
 #pragma unroll
-                            for (int kk = 0; kk < 3; ++kk) {
-                                const int lc = l0[2] + kk;
-                                if (lc < 0 || lc >= 8) continue;
-                                const float hw = gw[0][ii] * w[1][jj] * w[2][kk] * c3[0] +
-                                                 w[0][ii] * gw[1][jj] * w[2][kk] * c3[1] +
-                                                 w[0][ii] * w[1][jj] * gw[2][kk] * c3[2];
-                                acc[(la * 8 + lb) * kAccZ + lc] += hw;
-                                __asm__ volatile("" ::: "memory");
-                            }
-                        }
+                    for (int kk = 0; kk < 3; ++kk) {
+                        const int lc = l0[2] + kk;
+                        if (lc < 0 || lc >= 8) continue;
+                        const float hw = gw[0][ii] * w[1][jj] * w[2][kk] * c3[0] +
+                                         w[0][ii] * gw[1][jj] * w[2][kk] * c3[1] +
+                                         w[0][ii] * w[1][jj] * gw[2][kk] * c3[2];
+                        atomicAdd(&acc[(la * 8 + lb) * kAccZ + lc], to_fix(hw, sc));
                     }
                 }
-                __asm__ volatile("" ::: "memory");
             }
         }
         __syncthreads();
-        for (int q = lane; q < 512; q += 64) {
+        for (int q = tid; q < 512; q += 256) {
             const int o = q >> 6, cell = q & 63;
             const int G = S.out_page[o];
             if (G < 0) continue;
@@ -305,7 +311,7 @@ __global__ __launch_bounds__(64) void k_cg_q(DevParams dp, ST s, BlockTables t, 
                       lz = (o & 1) * 4 + (cell & 3);
             const int64_t nn = (int64_t)G * 64 + cell;
             const float D = grid[(int64_t)G * kPageFloats + 7 * 64 + cell];
-            cgq[nn] = acc[(lx * 8 + ly) * kAccZ + lz] + cgs[nn] * D;
+            cgq[nn] = from_fix(acc[(lx * 8 + ly) * kAccZ + lz], e) + cgs[nn] * D;
         }
         __syncthreads();
     }
@@ -459,12 +465,10 @@ static void apply_block(mpmx_ctx* c, ST st) {
 template <typename ST>
 static void apply_blocks(mpmx_ctx* c, ST st) {
     const BlockTables t = block_tables_of(c);
-    int64_t g = (c->tp_bound + 7) / 8 * 8;
-    const int64_t g1 = g < (int64_t)c->num_cu * 8 ? g : (int64_t)c->num_cu * 8;
-    const int64_t g2 = g < (int64_t)c->num_cu * 16 ? g : (int64_t)c->num_cu * 16;
+    const int g1 = block_grid(c), g2 = g1;
     MPMX_TIMED(c, "cg_col", (k_cg_col<ST><<<(int)g1, 256, 0, c->stream>>>(c->dp, st, c->vol, t, c->cg_s,
                                                                          c->cg_col, (int)c->n, c->dc)));
-    MPMX_TIMED(c, "cg_q", (k_cg_q<ST><<<(int)g2, 64, 0, c->stream>>>(c->dp, st, t, c->cg_col, c->grid, c->cg_s,
+    MPMX_TIMED(c, "cg_q", (k_cg_q<ST><<<(int)g2, 256, 0, c->stream>>>(c->dp, st, t, c->cg_col, c->grid, c->cg_s,
                                                                      c->cg_q, (int)c->n, c->dc)));
 }
 
@@ -506,9 +510,9 @@ hipError_t launch_pcg(mpmx_ctx* c, float dt) {
             if (e != hipSuccess) return e;
         }
         it += chunk;
-        // read the convergence flag and the iteration count (adjacent ints)
-        hipError_t e = hipMemcpyAsync(&c->hc->cg_done, &c->dc->cg_done, 2 * sizeof(int), hipMemcpyDeviceToHost,
-                                      c->stream);
+        // read the convergence flag and the counters (the block count sizes
+        // the next launches of the block kernels)
+        hipError_t e = hipMemcpyAsync(c->hc, c->dc, sizeof(DevCounts), hipMemcpyDeviceToHost, c->stream);
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) return e;

@@ -287,9 +287,7 @@ __global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTabl
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
     const bool breakdown = dc->cg_breakdown != 0;
-    int bb, be;
-    block_range(nb, bb, be);
-    for (int blk = bb; blk < be; ++blk) {
+    FOR_LOGICAL_ITEMS(blk, nb) {
         const uint32_t bkey = t.block_key[blk];
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
@@ -351,6 +349,18 @@ __global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTabl
 
 static Tables tables_of(mpmx_ctx* c) {
     return Tables{c->page_key, c->page2particle, c->adj, c->vpo, c->target, c->cap_p, c->cap_tp, c->cap_vp};
+}
+
+// workgroups for the 2x2x2-block kernels: the last block count the host saw
+// (refreshed at every PCG check) plus a margin, a multiple of 8.  The kernels
+// stride over the device-side count, so a stale hint changes only the balance.
+int block_grid(mpmx_ctx* c) {
+    int64_t nb = c->hc->num_blocks;
+    nb = nb + nb / 8 + 64;
+    const int64_t cap = c->tp_bound / 8 + 8;
+    nb = nb < cap ? nb : cap;
+    nb = (nb + 7) / 8 * 8;
+    return nb < 8 ? 8 : (int)nb;
 }
 
 int transfer_grid_blocks(mpmx_ctx* c) {
@@ -416,8 +426,7 @@ static void g2p_block(mpmx_ctx* c, ST st, float dt) {
 template <typename ST>
 static void g2p_block_kernel(mpmx_ctx* c, ST st, float dt) {
     const BlockTables t = block_tables_of(c);
-    int64_t g = (c->tp_bound + 7) / 8 * 8;
-    g = g < (int64_t)c->num_cu * 8 ? g : (int64_t)c->num_cu * 8;
+    const int g = block_grid(c);
     MPMX_TIMED(c, "g2p_block", ({
                    if (c->P.transfer == 0)
                        k_g2p_block<0, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys, dt,

@@ -168,6 +168,8 @@ struct mpmx_ctx {
     float* cg_q = nullptr;
     float* cg_part = nullptr;    // [4][cg_nb] per-workgroup partials
     float* cg_col = nullptr;     // [3n] per-particle M V dx^2 grad s (block K s)
+    float* payload = nullptr;    // [28n] per-particle P2G terms (block P2G)
+    unsigned* pbound = nullptr;  // [cap_p][8] per-page largest contribution per channel
     int cg_nb = 0;               // workgroups of every node-space kernel
     int cg_spec = 1;             // PCG iterations queued before the first host check
     bool grid_fused = false;     // last P2G also ran the grid update (gather variant)
@@ -329,6 +331,10 @@ __device__ __forceinline__ int xcd_logical_block() {
     if (G % 8) return b;
     return (b % 8) * (G / 8) + b / 8;
 }
+// items lb, lb + G, lb + 2G, ... of `count` for the calling logical block lb:
+// neighbouring items run on the same XCD and the load evens out
+#define FOR_LOGICAL_ITEMS(var, count) for (int var = xcd_logical_block(); var < (count); var += (int)gridDim.x)
+
 // contiguous [begin, end) chunk of `count` items for the calling logical block
 __device__ __forceinline__ void block_range(int count, int& begin, int& end) {
     const int lb = xcd_logical_block();
@@ -358,6 +364,7 @@ hipError_t fill_zero_dev_count(mpmx_ctx* c, float* p, const int* count_dev, int6
                                int64_t cap_units);
 void timer_push(mpmx_ctx* c, const char* name, bool begin);
 int transfer_grid_blocks(mpmx_ctx* c);
+int block_grid(mpmx_ctx* c);
 }  // namespace mpmx
 
 #define MPMX_TIMED(ctx, name, stmt)          \
