@@ -23,17 +23,50 @@ __device__ __forceinline__ uint32_t neighbor_key(uint32_t pk, int q) {
 }
 
 // markPageBoundary + buildHashMapFromPage + supplementAdjacentPages, as bits
-__global__ void k_mark_pages(const uint32_t* __restrict__ keys, int n, uint64_t* pmask, uint64_t* nmask) {
+__global__ void k_mark_pages(const uint32_t* __restrict__ keys, int n, uint64_t* pmask) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t pk = keys[i] >> 6;
     if (i > 0 && (keys[i - 1] >> 6) == pk) return;
     atomicOr((unsigned long long*)&pmask[pk >> 6], 1ull << (pk & 63u));
-#pragma unroll
-    for (int q = 0; q < 7; ++q) {
-        const uint32_t nk = neighbor_key(pk, q);
-        atomicOr((unsigned long long*)&nmask[nk >> 6], 1ull << (nk & 63u));
-    }
+}
+
+// supplementAdjacentPages as a bit dilation: nmask = the particle pages P plus
+// P + e for e in {0,1}^3 (the pages a particle page's nodes reach).  A 64-bit
+// word holds the 4x4x4 page cube of one coarse Morton code; inside it the
+// local index has z at bits 0 and 3, y at 1 and 4, x at 2 and 5.  +1 along an
+// axis moves a page with low bit 0 up by 2^a, one with low bit 1 and high bit
+// 0 up by 7 2^a, and the top layer (both bits set) leaves the word for the
+// next word along that axis, landing 9 2^a lower in its local index.
+__host__ __device__ constexpr uint64_t axis_bits_where(int a, int lo, int hi) {
+    uint64_t m = 0;
+    for (int i = 0; i < 64; ++i)
+        if (((i >> a) & 1) == lo && (hi < 0 || ((i >> (a + 3)) & 1) == hi)) m |= 1ull << i;
+    return m;
+}
+// m dilated by +1 along axis a, prev = the word below along that axis
+template <int A>
+__device__ __forceinline__ uint64_t dilate(uint64_t m, uint64_t prev) {
+    constexpr uint64_t lo0 = axis_bits_where(A, 0, -1), lo1hi0 = axis_bits_where(A, 1, 0),
+                       top = axis_bits_where(A, 1, 1);
+    return m | ((m & lo0) << (1 << A)) | ((m & lo1hi0) << (7 << A)) | ((prev & top) >> (9 << A));
+}
+
+__global__ void k_neighbour_mask(const uint64_t* __restrict__ pmask, uint64_t* __restrict__ nmask, int64_t words) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    int cx, cy, cz;
+    unmorton((uint32_t)w, cx, cy, cz);
+    auto P = [&](int dx, int dy, int dz) -> uint64_t {
+        const int x = cx - dx, y = cy - dy, z = cz - dz;
+        if (x < 0 || y < 0 || z < 0) return 0ull;
+        return pmask[morton((uint32_t)x, (uint32_t)y, (uint32_t)z)];
+    };
+    // dilate along z the four words the y and x steps need, then along y, then x
+    const uint64_t a00 = dilate<0>(P(0, 0, 0), P(0, 0, 1)), a01 = dilate<0>(P(0, 1, 0), P(0, 1, 1));
+    const uint64_t a10 = dilate<0>(P(1, 0, 0), P(1, 0, 1)), a11 = dilate<0>(P(1, 1, 0), P(1, 1, 1));
+    const uint64_t b0 = dilate<1>(a00, a01), b1 = dilate<1>(a10, a11);
+    nmask[w] = dilate<2>(b0, b1);
 }
 
 __global__ void k_word_counts(const uint64_t* __restrict__ pmask, const uint64_t* __restrict__ nmask,
@@ -157,8 +190,8 @@ hipError_t build_tables(mpmx_ctx* c) {
     const int64_t words = c->mask_words;
     hipError_t e;
     if ((e = fill_zero(c, c->pmask, words * 8)) != hipSuccess) return e;
-    if ((e = fill_zero(c, c->nmask, words * 8)) != hipSuccess) return e;
-    k_mark_pages<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask, c->nmask);
+    k_mark_pages<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask);
+    k_neighbour_mask<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pmask, c->nmask, words);
     k_word_counts<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pmask, c->nmask, c->wcount,
                                                                       c->scount, words);
     if ((e = scan_exclusive(c, c->wcount, c->wcount, words, nullptr, words, &c->dc->num_page)) != hipSuccess)
