@@ -9,9 +9,17 @@ Prints one JSON line.  `value` = particles x substeps / seconds of the timed
 region (inputs resident in HBM; HIP events on the context stream bracket it,
 with a device synchronisation on both sides).  `roofline` comes from a second,
 instrumented pass of the same scene in which every stage is bracketed by HIP
-events; achieved = algorithmic bytes of the P2G kernel (104 B per particle +
-52 B per grid node) / its average launch time.  `cpu_baseline` runs the
+events; achieved = algorithmic bytes of the block P2G kernel (its payload
+read once, 80 B per particle for MLS and 112 B for APIC, plus 52 B written per
+grid node) / its average launch time.  `cpu_baseline` runs the
 serial FP64 oracle on the host for a bounded sample of the same scene.
+`roofline.traffic` is the P2G kernel's HBM traffic per launch as measured by
+rocprofv3 counter runs (FETCH_SIZE + WRITE_SIZE, each scaled by what the
+counter reads for a plain copy of known size), read from the committed counter
+summary of this workload; null for other workloads.
+
+`--copy-probe MIB` runs only a plain device copy (out = 1 * in over MIB MiB of
+float32, three times): the calibration kernel of those counter runs.
 """
 import argparse
 import importlib.util
@@ -69,6 +77,37 @@ def cpu_baseline(scene_id, steps):
                       % (steps, len(m))}
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_%s_notched_bar_1M.md")
+
+
+def measured_traffic(kname, scene, precision):
+    """HBM bytes per launch of kernel `kname` from its committed counter summary
+    (line `traffic_bytes_per_launch: B`), for the default workload only."""
+    path = TRAFFIC_SUMMARY % kname
+    if scene != 1 or precision != "fp32" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_probe(mib):
+    """Plain device copy of `mib` MiB, three launches, for counter calibration."""
+    import torch
+
+    n = (mib << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_kib_read": n * 4 // 1024, "copy_probe_kib_written": n * 4 // 1024,
+                      "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,12 +119,16 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box here has one "
               "GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     mpmx = load_binding()
     p, x, v, m, dt = mpmx.scene(args.scene, seed=1, scale=1.0)
@@ -109,14 +152,21 @@ def main():
     ctx.run(args.profile_steps, dt)
     kt = ctx.timing_end()
     st2 = ctx.stats()
-    # dominant transfer kernel alone (the block P2G: 8-channel sums, grid update
-    # and PCG set-up); algorithmic bytes: 104 B of particle state read per
-    # particle, 32 B of grid channels + 20 B of PCG vectors written per node
+    # dominant kernel: the block P2G (8-channel sums, grid update and PCG
+    # set-up).  Algorithmic bytes: its per-particle payload read once (MLS 80 B,
+    # APIC 112 B) and 32 B of grid channels + 20 B of PCG vectors written per
+    # node.  The whole P2G stage (payload kernel + block kernel) is also
+    # reported against the 104 B per particle + 56 B per node of a P2G that
+    # reads x, v, C, sigma, c and the permutation directly.
     kname = "p2g_block" if "p2g_block" in kt else "p2g"
     p2g_launches, p2g_ms = kt.get(kname, (0, 0.0))
     G = 64 * st2["numTotalPage"]
-    p2g_bytes = 104.0 * n + 52.0 * G
+    pay = 112.0 if p.transfer == 1 else 80.0
+    p2g_bytes = (pay if kname == "p2g_block" else 104.0) * n + 52.0 * G
     achieved = p2g_bytes / (p2g_ms / p2g_launches * 1e-3) / 1e9 if p2g_launches else 0.0
+    prep_launches, prep_ms = kt.get("p2g_prep", (0, 0.0))
+    stage_ms = (p2g_ms + prep_ms) / p2g_launches if p2g_launches else 0.0
+    p2g_stage_gbs = (104.0 * n + 56.0 * G) / (stage_ms * 1e-3) / 1e9 if stage_ms else 0.0
     # G2P: reads x, F, c (52 B) and writes x, v, B, F, c, key (104 B) per
     # particle; reads v, c, sum w and x of the PCG (24 B) per node
     gname = "g2p_block" if "g2p_block" in kt else "g2p"
@@ -143,10 +193,11 @@ def main():
                    "pages_total": st["numTotalPage"], "active_nodes": G,
                    "cg_iters_last": st["cg_iters"], "wall_s": wall},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kname,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(kname, args.scene, args.precision), "kernel": kname,
                      "algorithmic_bytes_per_launch": p2g_bytes,
                      "avg_launch_ms": p2g_ms / p2g_launches if p2g_launches else None,
-                     "g2p_gbs": g2p_gbs},
+                     "p2g_stage_gbs": p2g_stage_gbs, "g2p_gbs": g2p_gbs},
         "stage_ms_per_step": {k: v[1] / max(args.profile_steps, 1) for k, v in kt.items()},
     }
     if not args.no_cpu:
