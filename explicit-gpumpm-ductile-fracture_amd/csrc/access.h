@@ -26,19 +26,27 @@ struct BlockTables {
     const uint32_t* keys;      // sorted particle keys
     const uint32_t* block_key;
     int64_t cap_p, cap_tp, cap_blocks;
-    int axis_pages;            // page coordinates lie in [0, axis_pages)
+    int slot_shift;            // scene of a page key = key >> slot_shift
 };
 
 inline BlockTables block_tables_of(const mpmx_ctx* c) {
     return BlockTables{c->pmask, c->nmask, c->wcount, c->scount, c->page2particle, c->keys, c->block_key,
-                       c->cap_p, c->cap_tp, c->tp_bound, 1 << c->axis_bits};
+                       c->cap_p, c->cap_tp, c->tp_bound, c->dp.slot_shift};
+}
+
+// page key of page (px, py, pz) (non-negative) when that page lies in the
+// slot of `scene` (the page space of one scene), else ~0u
+__device__ __forceinline__ uint32_t scene_page_key(const BlockTables& t, int px, int py, int pz, int scene) {
+    const uint32_t k = morton((uint32_t)px, (uint32_t)py, (uint32_t)pz);
+    return (int)(k >> t.slot_shift) == scene ? k : ~0u;
 }
 
 // ---- shared pieces of the block gathers ---------------------------------------
 // The particle pages 2b-1 .. 2b+1 (per axis) of block b, their particle ranges
 // and the cell filter that keeps the particles reaching the block; lanes 32..39
-// resolve the block's 8 grid pages.  Called by one wavefront (the first of the
-// workgroup); LDS arrays owned by the caller.
+// resolve the block's 8 grid pages.  Pages of another scene's slot are not
+// sources.  Called by one wavefront (the first of the workgroup); LDS arrays
+// owned by the caller.
 struct BlockSources {
     int start[27], pre[28], filter[27];
     int page[27];          // particle page id of each source, or -1
@@ -49,13 +57,14 @@ __device__ __forceinline__ void block_sources(const BlockTables& t, uint32_t bke
                                               int np_raw, int np, int nt, BlockSources& S) {
     const int lane = threadIdx.x;
     int cnt = 0;
+    const int scene = (int)((bkey << 3) >> t.slot_shift);
     if (lane < 27) {
         const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
         const int px = 2 * bc[0] + dx, py = 2 * bc[1] + dy, pz = 2 * bc[2] + dz;
         int st = 0, page_of_source = -1;
-        if (px >= 0 && py >= 0 && pz >= 0 && px < t.axis_pages && py < t.axis_pages && pz < t.axis_pages) {
-            const int id = page_rank(morton((uint32_t)px, (uint32_t)py, (uint32_t)pz), t.pmask, t.nmask, t.wpre,
-                                     t.spre, np_raw);
+        const uint32_t key = (px >= 0 && py >= 0 && pz >= 0) ? scene_page_key(t, px, py, pz, scene) : ~0u;
+        if (key != ~0u) {
+            const int id = page_rank(key, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
             if (id >= 0 && id < np) {
                 page_of_source = id;
                 int a = t.page2particle[id], b = t.page2particle[id + 1];

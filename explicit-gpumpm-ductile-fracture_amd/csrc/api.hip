@@ -113,6 +113,11 @@ __global__ void k_cell_ids(const uint32_t* __restrict__ keys, int n, const int* 
     if (i == n - 1) c2p[*num_cell] = n;
 }
 
+int particle_blocks(const mpmx_ctx* c) {
+    const int64_t threads = c->wmap.first ? c->wmap.waves * 64 : c->n;
+    return (int)std::max<int64_t>(1, (threads + 255) / 256);
+}
+
 void timer_push(mpmx_ctx* c, const char* name, bool begin) {
     if (!c->timing) return;
     TimerEntry* t = nullptr;
@@ -197,9 +202,10 @@ static void gauss_jordan3(const double in[9], double mat[9]) {
             for (int j = 0; j < 3; ++j) std::swap(mat[j * 3 + swapC[i]], mat[j * 3 + swapR[i]]);
 }
 
-static void make_dev_params(mpmx_ctx* c) {
-    const mpmx_params& P = c->P;
-    DevParams& d = c->dp;
+// the constants of one scene (the batch fields are set by set_scene_geometry)
+static DevParams scene_dev_params(const mpmx_params& P, const mpmx_config& cfg) {
+    DevParams d;
+    std::memset(&d, 0, sizeof(d));
     d.N = P.N;
     d.transfer = P.transfer;
     // the fixed corotated energy has no fracture: integrate skips the solve (Energy_Model 0)
@@ -243,7 +249,62 @@ static void make_dev_params(mpmx_ctx* c) {
     d.crit_stretch = (float)P.crit_stretch;
     d.L0 = (float)(0.5 / P.N);
     d.sq23f = sqrtf(2.0f / 3.0f);
-    d.svd_gap = (float)c->cfg.svd_gap;
+    d.svd_gap = (float)cfg.svd_gap;
+    return d;
+}
+
+static int64_t morton3_host(int64_t x, int64_t y, int64_t z) {
+    int64_t k = 0;
+    for (int b = 0; b < 20; ++b)
+        k |= (((z >> b) & 1) << (3 * b)) | (((y >> b) & 1) << (3 * b + 1)) | (((x >> b) & 1) << (3 * b + 2));
+    return k;
+}
+static void unmorton3_host(int64_t k, int64_t& x, int64_t& y, int64_t& z) {
+    x = y = z = 0;
+    for (int b = 0; b < 20; ++b) {
+        z |= ((k >> (3 * b)) & 1) << b;
+        y |= ((k >> (3 * b + 1)) & 1) << b;
+        x |= ((k >> (3 * b + 2)) & 1) << b;
+    }
+}
+
+// page space of the context: one scene spans the Morton cube of 2^axis_bits
+// pages per axis that holds its pages and their +1 neighbours; a batch gives
+// scene s slot s, a cube of 2^slot_log2 pages per axis (the largest scene's
+// pages), at slot coordinates unmorton(s)
+static void set_scene_geometry(mpmx_ctx* c) {
+    const int S = c->num_scenes;
+    if (S == 1) {
+        c->axis_bits = ceil_log2(c->scene_P[0].N / 4 + 1);
+        c->slot_log2 = c->axis_bits;
+    } else {
+        int b = 2;
+        for (const auto& P : c->scene_P) b = std::max(b, ceil_log2(P.N / 4));
+        c->slot_log2 = b;
+        c->axis_bits = b + (ceil_log2(S) + 2) / 3;
+    }
+    const int shift = 3 * c->slot_log2;
+    c->mask_words = std::max<int64_t>(1, ((int64_t)S << shift) / 64);
+    std::vector<DevParams> sd((size_t)S);
+    bool any_pf = false;
+    for (int k = 0; k < S; ++k) {
+        DevParams d = scene_dev_params(c->scene_P[k], c->cfg);
+        d.num_scenes = S;
+        d.slot_shift = shift;
+        int64_t ox, oy, oz;
+        unmorton3_host(k, ox, oy, oz);
+        const int64_t cells = (int64_t)4 << c->slot_log2;
+        d.origin[0] = (int)(ox * cells);
+        d.origin[1] = (int)(oy * cells);
+        d.origin[2] = (int)(oz * cells);
+        d.page_limit = std::min(1 << ceil_log2(c->scene_P[k].N / 4 + 1), 1 << c->slot_log2);
+        any_pf = any_pf || d.phase_field;
+        sd[(size_t)k] = d;
+    }
+    for (auto& d : sd) d.scenes = c->scene_dp;
+    c->dp = sd[0];
+    c->dp.phase_field = any_pf ? 1 : 0;   // batch-wide: some scene runs the solve
+    (void)hipMemcpy(c->scene_dp, sd.data(), sizeof(DevParams) * S, hipMemcpyHostToDevice);
 }
 
 static mpmx_status read_counts(mpmx_ctx* c) {
@@ -288,8 +349,45 @@ static mpmx_status alloc_grid(mpmx_ctx* c, int64_t cap_tp) {
     // capacity, at most 4 per CU, a multiple of 8 (one share per XCD)
     int64_t nb = ((cap_tp + 1) / 2 + 3) / 4;
     nb = std::min<int64_t>(nb, (int64_t)c->num_cu * 4);
+    nb = std::max<int64_t>(nb, c->num_scenes);   // at least one per scene
     c->cg_nb = (int)std::max<int64_t>(8, (nb + 7) / 8 * 8);
     HIPCHK(c, dalloc(&c->cg_part, 4 * (int64_t)c->cg_nb));
+    // PCG workgroups per scene: one each, the rest in proportion to the
+    // scenes' particle counts (largest remainders); a lone scene gets all
+    {
+        const int S = c->num_scenes, W = c->cg_nb;
+        std::vector<int> nwg((size_t)S, 1);
+        const int64_t ntot = std::accumulate(c->scene_n.begin(), c->scene_n.end(), (int64_t)0);
+        const int extra = W - S;
+        std::vector<std::pair<double, int>> rem;
+        int given = 0;
+        for (int k = 0; k < S; ++k) {
+            const double share = ntot > 0 ? (double)extra * (double)c->scene_n[(size_t)k] / (double)ntot : 0.0;
+            const int whole = (int)share;
+            nwg[(size_t)k] += whole;
+            given += whole;
+            rem.push_back({share - whole, k});
+        }
+        std::sort(rem.begin(), rem.end(), [](const auto& a, const auto& b) {
+            return a.first != b.first ? a.first > b.first : a.second < b.second;
+        });
+        for (int q = 0; q < extra - given; ++q) nwg[(size_t)rem[(size_t)q % S].second] += 1;
+        std::vector<SceneCG> h((size_t)S);
+        std::vector<int> wg_scene((size_t)W, 0);
+        std::memset(h.data(), 0, sizeof(SceneCG) * h.size());
+        int w0 = 0;
+        for (int k = 0; k < S; ++k) {
+            h[(size_t)k].wg0 = w0;
+            h[(size_t)k].nwg = nwg[(size_t)k];
+            for (int q = 0; q < nwg[(size_t)k]; ++q) wg_scene[(size_t)(w0 + q)] = k;
+            w0 += nwg[(size_t)k];
+        }
+        dfree(c->wg_scene);
+        HIPCHK(c, dalloc(&c->wg_scene, W));
+        HIPCHK(c, hipMemcpy(c->wg_scene, wg_scene.data(), sizeof(int) * W, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->scg, h.data(), sizeof(SceneCG) * S, hipMemcpyHostToDevice));
+        HIPCHK(c, launch_scene_counts(c));   // page ranges of the current tables
+    }
     HIPCHK(c, dalloc(&c->grid, cap_tp * kPageFloats));
     if (c->keep_p2g) HIPCHK(c, dalloc(&c->grid_p2g, cap_tp * kPageFloats));
     HIPCHK(c, dalloc(&c->cg_x, cap_tp * 64));
@@ -370,15 +468,33 @@ int mpmx_device_count(void) {
 }
 
 mpmx_status mpmx_create(const mpmx_params* p, const mpmx_config* cfg, mpmx_ctx** out) {
-    if (!p || !out) return MPMX_ERR_ARG;
+    return mpmx_create_batch(1, p, cfg, out);
+}
+
+mpmx_status mpmx_create_batch(int num_scenes, const mpmx_params* ps, const mpmx_config* cfg, mpmx_ctx** out) {
+    if (!ps || !out || num_scenes < 1) return MPMX_ERR_ARG;
     *out = nullptr;
-    if (p->N < 16 || p->N > 1024 || (p->N & 3)) return MPMX_ERR_ARG;
-    if (p->vpage != 64 && p->vpage != 128 && p->vpage != 256) return MPMX_ERR_ARG;
-    if (p->transfer < 0 || p->transfer > 1) return MPMX_ERR_ARG;
-    if (p->model < 0 || p->model > 2) return MPMX_ERR_ARG;
+    int slot_log2 = 2;
+    for (int k = 0; k < num_scenes; ++k) {
+        const mpmx_params* p = ps + k;
+        if (p->N < 16 || p->N > 1024 || (p->N & 3)) return MPMX_ERR_ARG;
+        if (p->vpage != 64 && p->vpage != 128 && p->vpage != 256) return MPMX_ERR_ARG;
+        if (p->transfer < 0 || p->transfer > 1) return MPMX_ERR_ARG;
+        if (p->model < 0 || p->model > 2) return MPMX_ERR_ARG;
+        // batch-wide settings
+        if (p->transfer != ps[0].transfer || p->vpage != ps[0].vpage || p->dot_mode != ps[0].dot_mode ||
+            p->cg_max_iter != ps[0].cg_max_iter)
+            return MPMX_ERR_ARG;
+        slot_log2 = std::max(slot_log2, ceil_log2(p->N / 4));
+    }
+    // keys: 6 cell bits + the slot's page bits + the slot index
+    if (num_scenes > 1 && 6 + 3 * slot_log2 + ceil_log2(num_scenes) > 32) return MPMX_ERR_ARG;
+    if (num_scenes > 1 && cfg && cfg->p2g_variant == 1) return MPMX_ERR_ARG;
     if (mpmx_device_count() <= 0) return MPMX_ERR_NO_DEVICE;
     mpmx_ctx* c = new mpmx_ctx();
-    c->P = *p;
+    c->P = ps[0];
+    c->num_scenes = num_scenes;
+    c->scene_P.assign(ps, ps + num_scenes);
     if (cfg) c->cfg = *cfg;
     else mpmx_default_config(&c->cfg);
     c->device = c->cfg.device;
@@ -400,7 +516,13 @@ mpmx_status mpmx_create(const mpmx_params* p, const mpmx_config* cfg, mpmx_ctx**
     }
     std::memset(c->hc, 0, sizeof(DevCounts));
     (void)hipMemcpy(c->dc, c->hc, sizeof(DevCounts), hipMemcpyHostToDevice);
-    make_dev_params(c);
+    if (hipMalloc((void**)&c->scene_dp, sizeof(DevParams) * num_scenes) != hipSuccess ||
+        hipMalloc((void**)&c->scg, sizeof(SceneCG) * num_scenes) != hipSuccess ||
+        hipMalloc((void**)&c->scene_start, sizeof(int) * (num_scenes + 1)) != hipSuccess) {
+        mpmx_destroy(c);
+        return MPMX_ERR_HIP;
+    }
+    set_scene_geometry(c);
     *out = c;
     return MPMX_OK;
 }
@@ -415,6 +537,11 @@ mpmx_status mpmx_destroy(mpmx_ctx* c) {
             (void)hipEventDestroy(e.second);
         }
     free_particles(c);
+    dfree(c->scene_dp);
+    dfree(c->scg);
+    dfree(c->scene_start);
+    dfree(c->wg_scene);
+    dfree(c->wave_buf);
     if (c->dc) (void)hipFree(c->dc);
     if (c->hc) (void)hipHostFree(c->hc);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -423,13 +550,50 @@ mpmx_status mpmx_destroy(mpmx_ctx* c) {
 }
 
 mpmx_status mpmx_set_particles(mpmx_ctx* c, int64_t n, const double* x, const double* v, const double* mass) {
-    if (!c || !x || !v || !mass || n <= 0 || n > (1ll << 30)) return fail(c, MPMX_ERR_ARG, "bad particle arrays");
+    if (c && c->num_scenes != 1) return fail(c, MPMX_ERR_ARG, "batch context: use mpmx_set_particles_batch");
+    return mpmx_set_particles_batch(c, &n, x, v, mass);
+}
+
+mpmx_status mpmx_set_particles_batch(mpmx_ctx* c, const int64_t* counts, const double* x, const double* v,
+                                     const double* mass) {
+    if (!c || !counts || !x || !v || !mass) return fail(c, MPMX_ERR_ARG, "bad particle arrays");
+    int64_t n = 0;
+    std::vector<int> start((size_t)c->num_scenes + 1, 0);
+    for (int k = 0; k < c->num_scenes; ++k) {
+        if (counts[k] <= 0) return fail(c, MPMX_ERR_ARG, "every scene needs particles");
+        n += counts[k];
+        if (n > (1ll << 30)) return fail(c, MPMX_ERR_ARG, "too many particles");
+        start[(size_t)k + 1] = (int)n;
+    }
     HIPCHK(c, hipSetDevice(c->device));
-    const int N = c->P.N;
-    const double lo = 2.0 / N, hi = 1.0 - 2.0 / N;
-    for (int64_t i = 0; i < 3 * n; ++i)
-        if (!(x[i] >= lo && x[i] <= hi)) return fail(c, MPMX_ERR_OUT_OF_DOMAIN, "particle outside [2dx, 1-2dx]");
+    for (int k = 0; k < c->num_scenes; ++k) {
+        const int Nk = c->scene_P[(size_t)k].N;
+        const double lo = 2.0 / Nk, hi = 1.0 - 2.0 / Nk;
+        for (int64_t i = 3 * (int64_t)start[(size_t)k]; i < 3 * (int64_t)start[(size_t)k + 1]; ++i)
+            if (!(x[i] >= lo && x[i] <= hi)) return fail(c, MPMX_ERR_OUT_OF_DOMAIN, "particle outside [2dx, 1-2dx]");
+    }
     free_particles(c);
+    c->scene_n.assign(counts, counts + c->num_scenes);
+    HIPCHK(c, hipMemcpy(c->scene_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+    // waves of the per-particle kernels: each scene's range (contiguous in
+    // every sorted order) padded to whole waves
+    dfree(c->wave_buf);
+    c->wmap = WaveMap{};
+    if (c->num_scenes > 1) {
+        std::vector<int> first, count, scene;
+        for (int k = 0; k < c->num_scenes; ++k)
+            for (int64_t q = 0; q < counts[k]; q += 64) {
+                first.push_back(start[(size_t)k] + (int)q);
+                count.push_back((int)std::min<int64_t>(64, counts[k] - q));
+                scene.push_back(k);
+            }
+        const int64_t W = (int64_t)first.size();
+        HIPCHK(c, dalloc(&c->wave_buf, 3 * W));
+        HIPCHK(c, hipMemcpy(c->wave_buf, first.data(), sizeof(int) * W, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->wave_buf + W, count.data(), sizeof(int) * W, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->wave_buf + 2 * W, scene.data(), sizeof(int) * W, hipMemcpyHostToDevice));
+        c->wmap = WaveMap{c->wave_buf, c->wave_buf + W, c->wave_buf + 2 * W, W};
+    }
     c->n = n;
     c->cap_n = std::max<int64_t>(n, c->cfg.max_particles);
     const bool mixed = c->cfg.precision == MPMX_MIXED;
@@ -471,17 +635,20 @@ mpmx_status mpmx_set_particles(mpmx_ctx* c, int64_t n, const double* x, const do
     HIPCHK(c, dalloc(&c->vals_alt, n));
     c->sort_tiles = (int)((n + 2047) / 2048);
     HIPCHK(c, dalloc(&c->hist, (int64_t)256 * c->sort_tiles));
-    c->key_bits = 6 + 3 * std::max(1, ceil_log2(N / 4));
-    // pages
-    c->axis_bits = ceil_log2(N / 4 + 1);
-    const int64_t space = 1ll << (3 * c->axis_bits);
-    c->mask_words = std::max<int64_t>(1, space / 64);
+    // keys: cell bits, the slot's page bits and (batch) the slot index
+    c->key_bits = c->num_scenes == 1 ? 6 + 3 * std::max(1, ceil_log2(c->P.N / 4))
+                                     : 6 + 3 * c->slot_log2 + ceil_log2(c->num_scenes);
+    // pages (mask_words from the scene geometry)
     HIPCHK(c, dalloc(&c->pmask, c->mask_words));
     HIPCHK(c, dalloc(&c->nmask, c->mask_words));
     HIPCHK(c, dalloc(&c->wcount, c->mask_words));
     HIPCHK(c, dalloc(&c->scount, c->mask_words));
-    const int64_t pages_domain = (int64_t)(N / 4) * (N / 4) * (N / 4);
-    const int64_t pages_nb = (int64_t)(N / 4 + 1) * (N / 4 + 1) * (N / 4 + 1);
+    int64_t pages_domain = 0, pages_nb = 0;
+    for (const auto& P : c->scene_P) {
+        const int64_t q = P.N / 4;
+        pages_domain += q * q * q;
+        pages_nb += (q + 1) * (q + 1) * (q + 1);
+    }
     c->cap_p = std::min<int64_t>(n, pages_domain);
     const int64_t tp_bound = std::min<int64_t>(8 * c->cap_p, pages_nb);
     c->cap_vp = n / c->P.vpage + c->cap_p + 1;
@@ -637,8 +804,10 @@ mpmx_status mpmx_compute_dt(mpmx_ctx* c, double dt_default, double cfl, double* 
     if (st != MPMX_OK) return st;
     double dt = dt_default;
     const double vmax = std::sqrt(s2);
-    // Simulator.cpp:282
-    if (vmax > 0 && cfl / c->P.N / vmax < dt_default) dt = cfl / c->P.N / vmax;
+    // Simulator.cpp:282, with the finest grid of the batch
+    int N = 0;
+    for (const auto& P : c->scene_P) N = std::max(N, P.N);
+    if (vmax > 0 && cfl / N / vmax < dt_default) dt = cfl / N / vmax;
     *dt_out = dt;
     return MPMX_OK;
 }
@@ -922,6 +1091,22 @@ mpmx_status mpmx_get_grid_p2g(mpmx_ctx* c, int ch, float* host, size_t count) {
     return get_grid_from(c, c->grid_p2g, ch, host, count);
 }
 
+// FP64 mass, momentum and kinetic energy of particles [first, first + count)
+// of the current order
+static mpmx_status conservation_sums(mpmx_ctx* c, int64_t first, int64_t count, double h[5]) {
+    double* d = nullptr;
+    HIPCHK(c, dalloc(&d, 5));
+    HIPCHK(c, hipMemsetAsync(d, 0, sizeof(double) * 5, c->stream));
+    const bool mixed = c->cfg.precision == MPMX_MIXED;
+    k_conservation<<<256, 256, 0, c->stream>>>((mixed ? c->ms[c->cur].mass : c->ps[c->cur].mass) + first,
+                                               mixed ? nullptr : c->ps[c->cur].v + 3 * first,
+                                               mixed ? c->ms[c->cur].v + 3 * first : nullptr, (int)count, d);
+    HIPCHK(c, hipMemcpyAsync(h, d, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(d);
+    return MPMX_OK;
+}
+
 mpmx_status mpmx_get_stats(mpmx_ctx* c, mpmx_stats* out) {
     if (!c || !out) return MPMX_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
@@ -929,17 +1114,9 @@ mpmx_status mpmx_get_stats(mpmx_ctx* c, mpmx_stats* out) {
     out->num_particles = c->n;
     out->substeps = c->substeps;
     if (c->n > 0) {
-        double* d = nullptr;
-        HIPCHK(c, dalloc(&d, 5));
-        HIPCHK(c, hipMemsetAsync(d, 0, sizeof(double) * 5, c->stream));
-        const bool mixed = c->cfg.precision == MPMX_MIXED;
-        k_conservation<<<256, 256, 0, c->stream>>>(mixed ? c->ms[c->cur].mass : c->ps[c->cur].mass,
-                                                   mixed ? nullptr : c->ps[c->cur].v,
-                                                   mixed ? c->ms[c->cur].v : nullptr, (int)c->n, d);
         double h[5];
-        HIPCHK(c, hipMemcpyAsync(h, d, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        dfree(d);
+        mpmx_status s = conservation_sums(c, 0, c->n, h);
+        if (s != MPMX_OK) return s;
         out->mass = h[0];
         out->momentum[0] = h[1];
         out->momentum[1] = h[2];
@@ -953,9 +1130,42 @@ mpmx_status mpmx_get_stats(mpmx_ctx* c, mpmx_stats* out) {
     out->num_virtual_page = c->hc->num_vp;
     out->cg_iters = c->hc->cg_iters;
     out->cg_breakdown = c->hc->cg_breakdown;
-    out->cg_norm = c->hc->cg_norm;
+    std::vector<SceneCG> g((size_t)c->num_scenes);
+    HIPCHK(c, hipMemcpy(g.data(), c->scg, sizeof(SceneCG) * g.size(), hipMemcpyDeviceToHost));
+    double norm = 0;
+    for (const auto& e : g) norm = std::max(norm, (double)e.norm);
+    out->cg_norm = norm;
     return check_err(c);
 }
+
+mpmx_status mpmx_get_scene_stats(mpmx_ctx* c, int scene, mpmx_scene_stats* out) {
+    if (!c || !out || scene < 0 || scene >= c->num_scenes || c->n == 0) return MPMX_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::memset(out, 0, sizeof(*out));
+    int64_t first = 0;
+    for (int k = 0; k < scene; ++k) first += c->scene_n[(size_t)k];
+    out->num_particles = c->scene_n[(size_t)scene];
+    out->first_particle = first;
+    // the sort keeps each scene's particles contiguous, in scene order
+    double h[5];
+    mpmx_status s = conservation_sums(c, first, out->num_particles, h);
+    if (s != MPMX_OK) return s;
+    out->mass = h[0];
+    out->momentum[0] = h[1];
+    out->momentum[1] = h[2];
+    out->momentum[2] = h[3];
+    out->kinetic_energy = h[4];
+    SceneCG g;
+    HIPCHK(c, hipMemcpy(&g, c->scg + scene, sizeof(SceneCG), hipMemcpyDeviceToHost));
+    out->num_page = g.np;
+    out->num_total_page = g.np + g.nn;
+    out->cg_iters = g.iters;
+    out->cg_breakdown = g.breakdown;
+    out->cg_norm = g.norm;
+    return check_err(c);
+}
+
+int mpmx_num_scenes(const mpmx_ctx* c) { return c ? c->num_scenes : 0; }
 
 mpmx_status mpmx_get_device_pointers(mpmx_ctx* c, mpmx_device_ptrs* o) {
     if (!c || !o) return MPMX_ERR_ARG;

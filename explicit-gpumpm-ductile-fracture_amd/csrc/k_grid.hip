@@ -74,12 +74,26 @@ struct CGVec {
 };
 
 // PCG scalars start over every substep
-__global__ void k_cg_reset(DevParams dp, DevCounts* dc) {
-    if (threadIdx.x != 0) return;
-    dc->cg_done = dp.phase_field ? 0 : 1;
-    dc->cg_iters = 0;
-    dc->cg_breakdown = 0;
-    dc->cg_norm = 0.f;
+// (per scene: a scene whose energy has no fracture starts done)
+__global__ void k_cg_reset(DevParams dp, SceneCG* __restrict__ scg, DevCounts* dc) {
+    int active = 0;
+    for (int s = threadIdx.x; s < dp.num_scenes; s += blockDim.x) {
+        const int on = dp.scenes[s].phase_field ? 1 : 0;
+        scg[s].done = 1 - on;
+        scg[s].iters = 0;
+        scg[s].breakdown = 0;
+        scg[s].norm = 0.f;
+        active += on;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) active += __shfl_xor(active, o, 64);
+    if (threadIdx.x == 0) {
+        dc->cg_active = active;
+        dc->cg_done = active ? 0 : 1;
+        dc->cg_iters = 0;
+        dc->cg_breakdown = 0;
+        dc->cg_norm = 0.f;
+    }
 }
 
 // per-particle P2G terms, computed once and kept in registers across rounds
@@ -238,31 +252,30 @@ __device__ __forceinline__ void payload_load(const float* __restrict__ pay, int6
 // one integer atomicMax per page and channel on the float bits, which order
 // like the floats for non-negative values); pbound is zero on entry
 template <int TRANSFER, typename ST>
-__global__ __launch_bounds__(256) void k_p2g_prep(DevParams dp, ST s, const float* __restrict__ sigma,
+__global__ __launch_bounds__(256) void k_p2g_prep(DevParams dp, ST s, WaveMap wm, const float* __restrict__ sigma,
                                                   const float* __restrict__ FP, const float* __restrict__ vol,
                                                   BlockTables t, float* __restrict__ pay, unsigned* __restrict__ pbound,
                                                   float dt, int n, const DevCounts* __restrict__ dc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const bool valid = i < n;
+    int i, scene;
+    const bool valid = wave_particle(wm, n, i, scene);
     float b[kP2GCh];
-    uint32_t pk = 0xffffffffu;
+#pragma unroll
+    for (int k = 0; k < kP2GCh; ++k) b[k] = 0.f;
+    const uint32_t pk = valid ? t.keys[i] >> 6 : 0xffffffffu;
     if (valid) {
+        const DevParams& sp = dp.scenes[scene];
         const int org[3] = {0, 0, 0};
         P2GTerms T;
-        p2g_terms<TRANSFER>(dp, s, sigma, FP, vol, i, n, dt, org, T);
+        p2g_terms<TRANSFER>(sp, s, sigma, FP, vol, i, n, dt, org, T);
         float d0[3];
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             const float xa = ld_x(s, i, ax);
-            d0[ax] = xa * dp.inv_dx - (float)base_node(xa, dp.N);
+            d0[ax] = xa * sp.inv_dx - (float)base_node(xa, sp.N);
         }
         payload_store<TRANSFER>(pay, i, d0, T);
         p2g_bounds<TRANSFER>(T, b);
-        pk = t.keys[i] >> 6;
-    } else {
-#pragma unroll
-        for (int k = 0; k < kP2GCh; ++k) b[k] = 0.f;
     }
     const int np_raw = dc->num_page;
     const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
@@ -304,6 +317,7 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
         const uint32_t bkey = t.block_key[blk];
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
+        const DevParams sp = dp.scenes[__builtin_amdgcn_readfirstlane(scene_of_page(dp, bkey << 3))];
         if (wave == 0) block_sources(t, bkey, bc, n, np_raw, np, nt, S);
         for (int q = tid; q < kP2GCh * kAccStride; q += 256) acc[q] = 0ull;
         __syncthreads();
@@ -362,11 +376,12 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
                 for (int k = 0; k < kP2GCh; ++k) rp[k * 64] = ch[k];
             }
             float r, z;
-            node_update(dp, org[0] + lx, org[1] + ly, org[2] + lz, ch, dt, r, z);
+            node_update(sp, org[0] + lx - sp.origin[0], org[1] + ly - sp.origin[1], org[2] + lz - sp.origin[2], ch,
+                        dt, r, z);
             float* gp = grid + (int64_t)G * kPageFloats + cell;
 #pragma unroll
             for (int k = 0; k < kP2GCh; ++k) gp[k * 64] = ch[k];
-            if (dp.phase_field) {
+            if (sp.phase_field) {
                 const int64_t nn = (int64_t)G * 64 + cell;
                 cg.x[nn] = 0.f;
                 cg.r[nn] = r;
@@ -379,22 +394,21 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
     }
 }
 
-// partials of the PCG's first "dot"(r, z): one wave per 128-node block (the
-// reference's 128-entry dot blocks follow page order, not the 2x2x2 blocks)
+// partials of the PCG's first "dot"(r, z), per scene (the reference's
+// 128-entry dot blocks follow page order, not the 2x2x2 blocks)
 __global__ __launch_bounds__(256) void k_cg_rho0(DevParams dp, const float* __restrict__ cgr,
                                                  const float* __restrict__ cgz, float* __restrict__ part_rho,
-                                                 int64_t cap_tp, const DevCounts* __restrict__ dc) {
+                                                 int64_t cap_tp, const SceneCG* __restrict__ scg,
+                                                 const int* __restrict__ wg_scene, const DevCounts* __restrict__ dc) {
     __shared__ float lds[4];
-    int nt = dc->num_total;
-    nt = nt < cap_tp ? nt : (int)cap_tp;
-    const int nblk = (nt + 1) / 2;
-    const int lane = threadIdx.x & 63;
+    const SceneBlocks sb = scene_blocks(scg, wg_scene);
+    const int num_page = dc->num_page;
     const bool maxdot = dp.dot_mode == 0;
     float wacc = maxdot ? -INFINITY : 0.f;
-    for (int blk = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; blk < nblk; blk += (gridDim.x * blockDim.x) >> 6) {
-        const int n0 = blk * 128 + lane * 2;
+    for (int blk = sb.wave; blk < sb.nblk; blk += sb.nwaves) {
+        const int64_t n0 = scene_block_node(sb, num_page, cap_tp, blk);
         float rz = 0.f;
-        if ((n0 >> 6) < nt) {
+        if (n0 >= 0) {
             const float2 r = *(const float2*)(cgr + n0);
             const float2 z = *(const float2*)(cgz + n0);
             rz = r.x * z.x + r.y * z.y;
@@ -460,7 +474,7 @@ __global__ __launch_bounds__(256) void k_grid_update(DevParams dp, const uint32_
 hipError_t launch_p2g_scatter(mpmx_ctx* c, float dt);
 
 hipError_t launch_cg_reset(mpmx_ctx* c) {
-    k_cg_reset<<<1, 64, 0, c->stream>>>(c->dp, c->dc);
+    k_cg_reset<<<1, 64, 0, c->stream>>>(c->dp, c->scg, c->dc);
     return hipGetLastError();
 }
 
@@ -473,15 +487,16 @@ static hipError_t gather_dispatch_t(mpmx_ctx* c, ST st, float dt) {
     hipError_t e = fill_zero_dev_count(c, reinterpret_cast<float*>(c->pbound), &c->dc->num_page, kP2GCh, c->cap_p);
     if (e != hipSuccess) return e;
     const int n = (int)c->n;
-    MPMX_TIMED(c, "p2g_prep", (k_p2g_prep<TRANSFER, ST><<<(n + 255) / 256, 256, 0, c->stream>>>(
-                                   c->dp, st, c->sigma, c->FP, c->vol, t, c->payload, c->pbound, dt, n, c->dc)));
+    MPMX_TIMED(c, "p2g_prep", (k_p2g_prep<TRANSFER, ST><<<particle_blocks(c), 256, 0, c->stream>>>(
+                                   c->dp, st, c->wmap, c->sigma, c->FP, c->vol, t, c->payload, c->pbound, dt, n,
+                                   c->dc)));
     // about one workgroup per block: the hardware balances their uneven work
     const int g = block_grid(c);
     MPMX_TIMED(c, "p2g_block", (k_p2g_block<TRANSFER, ST><<<g, 256, 0, c->stream>>>(
                                     c->dp, c->payload, c->pbound, t, c->grid, raw, cg_vec(c), dt, n, c->dc)));
     if (c->dp.phase_field)
         k_cg_rho0<<<c->cg_nb, 256, 0, c->stream>>>(c->dp, c->cg_r, c->cg_z, c->cg_part + c->cg_nb, c->cap_tp,
-                                                    c->dc);
+                                                    c->scg, c->wg_scene, c->dc);
     return hipGetLastError();
 }
 

@@ -25,7 +25,9 @@
 //    counted in the statistics, not raised as an error.
 //
 // Partials: cg_part[0] s.q, cg_part[1 + (k & 1)] r.z entering iteration k,
-// cg_part[3] max|r|, each cg_nb floats.
+// cg_part[3] max|r|, each cg_nb floats.  Every scene of a batch runs its own
+// solve: its own 128-node blocks, partials, scalars and stopping test
+// (SceneBlocks); the batch iterates until every scene has stopped.
 #include "access.h"
 
 namespace mpmx {
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_apply(DevParams dp, ST s, const fl
 template <typename ST>
 __global__ __launch_bounds__(256) void k_cg_col(DevParams dp, ST s, const float* __restrict__ vol, BlockTables t,
                                                 const float* __restrict__ cgs, float* __restrict__ col, int n,
-                                                DevCounts* dc) {
+                                                const SceneCG* __restrict__ scg, DevCounts* dc) {
     constexpr int R = 10, R3 = R * R * R;
     __shared__ float ts[R3];
     __shared__ int spg[27];
@@ -142,18 +144,20 @@ __global__ __launch_bounds__(256) void k_cg_col(DevParams dp, ST s, const float*
     nt = nt < t.cap_tp ? nt : (int)t.cap_tp;
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
-    const float para_scale = 4.0f * dp.L0 * dp.L0 * dp.inv_dx * dp.inv_dx;
     const int tid = threadIdx.x;
     FOR_LOGICAL_ITEMS(blk, nb) {
         const uint32_t bkey = t.block_key[blk];
+        const int scene = __builtin_amdgcn_readfirstlane(scene_of_page(dp, bkey << 3));
+        if (scg[scene].done) continue;   // uniform over the workgroup
+        const DevParams sp = dp.scenes[scene];
+        const float para_scale = 4.0f * sp.L0 * sp.L0 * sp.inv_dx * sp.inv_dx;
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
         if (tid < 27) {
             const int px = 2 * bc[0] + tid / 9, py = 2 * bc[1] + (tid / 3) % 3, pz = 2 * bc[2] + tid % 3;
             int id = -1;
-            if (px < t.axis_pages && py < t.axis_pages && pz < t.axis_pages)
-                id = page_rank(morton((uint32_t)px, (uint32_t)py, (uint32_t)pz), t.pmask, t.nmask, t.wpre, t.spre,
-                               np_raw);
+            const uint32_t key = scene_page_key(t, px, py, pz, scene);
+            if (key != ~0u) id = page_rank(key, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
             spg[tid] = (id >= 0 && id < nt) ? id : -1;
         }
         if (tid >= 64 && tid < 128) {
@@ -188,15 +192,16 @@ __global__ __launch_bounds__(256) void k_cg_col(DevParams dp, ST s, const float*
                 ts[q] = G >= 0 ? cgs[(int64_t)G * 64 + (((X & 3) << 4) | ((Y & 3) << 2) | (Z & 3))] : 0.f;
             }
             __syncthreads();
-            const int org[3] = {8 * bc[0], 8 * bc[1], 8 * bc[2]};
+            // the region's first node in scene-local cells
+            const int org[3] = {8 * bc[0] - sp.origin[0], 8 * bc[1] - sp.origin[1], 8 * bc[2] - sp.origin[2]};
             for (int i = pa + tid; i < pb; i += 256) {
                 float d0[3], w[3][3], gw[3][3];
                 int l[3];
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     const float xa = ld_x(s, i, a);
-                    const int b = base_node(xa, dp.N);
-                    d0[a] = xa * dp.inv_dx - (float)b;
+                    const int b = base_node(xa, sp.N);
+                    d0[a] = xa * sp.inv_dx - (float)b;
                     const int la = b - org[a];
                     l[a] = la < 0 ? 0 : (la > R - 3 ? R - 3 : la);
                     bspline_grad(d0[a], w[a], gw[a]);
@@ -213,7 +218,7 @@ __global__ __launch_bounds__(256) void k_cg_col(DevParams dp, ST s, const float*
                             gs[1] += w[0][ii] * gw[1][jj] * w[2][kk] * sv;
                             gs[2] += w[0][ii] * w[1][jj] * gw[2][kk] * sv;
                         }
-                const float sc = vol[i] * para_scale * dp.M;
+                const float sc = vol[i] * para_scale * sp.M;
                 col[3 * (int64_t)i + 0] = gs[0] * sc;
                 col[3 * (int64_t)i + 1] = gs[1] * sc;
                 col[3 * (int64_t)i + 2] = gs[2] * sc;
@@ -229,7 +234,8 @@ __global__ __launch_bounds__(256) void k_cg_col(DevParams dp, ST s, const float*
 template <typename ST>
 __global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t, const float* __restrict__ col,
                                               const float* __restrict__ grid, const float* __restrict__ cgs,
-                                              float* __restrict__ cgq, int n, DevCounts* dc) {
+                                              float* __restrict__ cgq, int n, const SceneCG* __restrict__ scg,
+                                              DevCounts* dc) {
     __shared__ unsigned long long acc[kAccStride];
     __shared__ BlockSources S;
     __shared__ float wmax[4];
@@ -244,13 +250,17 @@ __global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t,
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
     FOR_LOGICAL_ITEMS(blk, nb) {
         const uint32_t bkey = t.block_key[blk];
+        const int scene = __builtin_amdgcn_readfirstlane(scene_of_page(dp, bkey << 3));
+        if (scg[scene].done) continue;   // uniform over the workgroup
+        const DevParams sp = dp.scenes[scene];
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
         if (wave == 0) block_sources(t, bkey, bc, n, np_raw, np, nt, S);
         for (int q = tid; q < kAccStride; q += 256) acc[q] = 0ull;
         __syncthreads();
         const int total = S.pre[27];
-        const int org[3] = {8 * bc[0], 8 * bc[1], 8 * bc[2]};
+        // the block's first node in scene-local cells
+        const int org[3] = {8 * bc[0] - sp.origin[0], 8 * bc[1] - sp.origin[1], 8 * bc[2] - sp.origin[2]};
         float bm = 0.f;
         const int per = (total + 255) / 256;   // runs of entries per thread (see k_p2g_block)
         for (int k = 0; k < per; ++k) {
@@ -276,8 +286,8 @@ __global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t,
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const float xa = ld_x(s, i, a);
-                const int b = base_node(xa, dp.N);
-                const float d0 = xa * dp.inv_dx - (float)b;
+                const int b = base_node(xa, sp.N);
+                const float d0 = xa * sp.inv_dx - (float)b;
                 l0[a] = b - org[a];
                 bspline_grad(d0, w[a], gw[a]);
                 c3[a] = col[3 * (int64_t)i + a];
@@ -317,34 +327,35 @@ __global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t,
     }
 }
 
-// helper: grid-stride over 128-node blocks, one wave each, two nodes per lane
-#define FOR_128_BLOCKS(nt)                                                   \
-    const int lane = threadIdx.x & 63;                                       \
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;           \
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;                        \
-    const int nblk = ((nt) + 1) / 2;                                         \
-    for (int blk = wave; blk < nblk; blk += nwaves)
+// the workgroup's scene has stopped (converged, broken down): uniform branch
+__device__ __forceinline__ bool scene_stopped(const SceneCG* scg, int scene) {
+    __shared__ int sdone;
+    if (threadIdx.x == 0) sdone = scg[scene].done;
+    __syncthreads();
+    return sdone != 0;
+}
 
 // q = q + s D (page-tile variant; the block variant adds s D in k_cg_q);
 // partial of "dot"(s, q)
 __global__ __launch_bounds__(256) void k_cg_dot(DevParams dp, const float* __restrict__ grid,
                                                 const float* __restrict__ cgs, float* __restrict__ cgq,
                                                 float* __restrict__ part_sq, int add_diag, int64_t cap_tp,
+                                                const SceneCG* __restrict__ scg, const int* __restrict__ wg_scene,
                                                 const DevCounts* __restrict__ dc) {
     __shared__ float lds[4];
-    if (cg_stopped(dc)) return;
-    int nt = dc->num_total;
-    nt = nt < cap_tp ? nt : (int)cap_tp;
+    const SceneBlocks sb = scene_blocks(scg, wg_scene);
+    if (scene_stopped(scg, sb.scene)) return;
+    const int num_page = dc->num_page;
     const bool maxdot = dp.dot_mode == 0;
     float wacc = maxdot ? -INFINITY : 0.f;
-    FOR_128_BLOCKS(nt) {
-        const int n0 = blk * 128 + lane * 2;
+    for (int blk = sb.wave; blk < sb.nblk; blk += sb.nwaves) {
+        const int64_t n0 = scene_block_node(sb, num_page, cap_tp, blk);
         float part = 0.f;
-        if ((n0 >> 6) < nt) {
+        if (n0 >= 0) {
             const float2 sv = *(const float2*)(cgs + n0);
             float2 qv = *(const float2*)(cgq + n0);
             if (add_diag) {
-                const float* D = grid + (int64_t)(n0 >> 6) * kPageFloats + 7 * 64 + (n0 & 63);
+                const float* D = grid + (n0 >> 6) * kPageFloats + 7 * 64 + (n0 & 63);
                 qv.x += sv.x * D[0];
                 qv.y += sv.y * D[1];
                 *(float2*)(cgq + n0) = qv;
@@ -365,24 +376,26 @@ __global__ __launch_bounds__(256) void k_cg_update(DevParams dp, const float* __
                                                    float* __restrict__ cgx, float* __restrict__ cgr,
                                                    float* __restrict__ cgz, const float* __restrict__ cgs,
                                                    float* __restrict__ cgq, float* __restrict__ part, int nb, int k,
-                                                   int64_t cap_tp, const DevCounts* __restrict__ dc) {
+                                                   int64_t cap_tp, const SceneCG* __restrict__ scg,
+                                                   const int* __restrict__ wg_scene,
+                                                   const DevCounts* __restrict__ dc) {
     __shared__ float lds[4];
-    if (cg_stopped(dc)) return;
-    const float sq = reduce_dot(dp, part, nb, lds);
-    const float rho_old = reduce_dot(dp, part + (1 + (k & 1)) * nb, nb, lds);
+    const SceneBlocks sb = scene_blocks(scg, wg_scene);
+    if (scene_stopped(scg, sb.scene)) return;
+    const float sq = reduce_dot(dp, part + sb.wg0, sb.nwg, lds);
+    const float rho_old = reduce_dot(dp, part + (1 + (k & 1)) * nb + sb.wg0, sb.nwg, lds);
     const float alpha = rho_old / sq;
-    int nt = dc->num_total;
-    nt = nt < cap_tp ? nt : (int)cap_tp;
+    const int num_page = dc->num_page;
     const bool maxdot = dp.dot_mode == 0;
     float wacc = maxdot ? -INFINITY : 0.f, wmax = 0.f;
-    FOR_128_BLOCKS(nt) {
-        const int n0 = blk * 128 + lane * 2;
+    for (int blk = sb.wave; blk < sb.nblk; blk += sb.nwaves) {
+        const int64_t n0 = scene_block_node(sb, num_page, cap_tp, blk);
         float rz = 0.f, rmax = 0.f;
-        if ((n0 >> 6) < nt) {
-            const float* gp = grid + (int64_t)(n0 >> 6) * kPageFloats + (n0 & 63);
+        if (n0 >= 0) {
+            const float* gp = grid + (n0 >> 6) * kPageFloats + (n0 & 63);
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const int nn = n0 + h;
+                const int64_t nn = n0 + h;
                 if (gp[5 * 64 + h] > 0.f) {
                     const float D = gp[7 * 64 + h];
                     cgx[nn] += cgs[nn] * alpha;
@@ -411,34 +424,43 @@ __global__ __launch_bounds__(256) void k_cg_update(DevParams dp, const float* __
     }
 }
 
-// convergence / breakdown test, then s = s beta + z
+// convergence / breakdown test of the workgroup's scene, then s = s beta + z.
+// The scene's first workgroup records the scene's iteration count and norm
+// and, when it stops, takes it off the batch's active count; the last scene
+// to stop sets the batch flag that turns every later PCG launch into a no-op.
 __global__ __launch_bounds__(256) void k_cg_direction(DevParams dp, const float* __restrict__ grid,
                                                       const float* __restrict__ cgz, float* __restrict__ cgs,
                                                       const float* __restrict__ part, int nb, int k,
-                                                      int64_t cap_tp, DevCounts* dc) {
+                                                      int64_t cap_tp, SceneCG* __restrict__ scg,
+                                                      const int* __restrict__ wg_scene, DevCounts* dc) {
     __shared__ float lds[4];
-    if (cg_stopped(dc)) return;
-    const float norm = reduce_parts<true>(part + 3 * nb, nb, lds);
-    const float rho_old = reduce_dot(dp, part + (1 + (k & 1)) * nb, nb, lds);
-    const float rho = reduce_dot(dp, part + (1 + ((k + 1) & 1)) * nb, nb, lds);
-    const bool converged = norm < dp.cg_tol;
+    const SceneBlocks sb = scene_blocks(scg, wg_scene);
+    if (scene_stopped(scg, sb.scene)) return;
+    const float norm = reduce_parts<true>(part + 3 * nb + sb.wg0, sb.nwg, lds);
+    const float rho_old = reduce_dot(dp, part + (1 + (k & 1)) * nb + sb.wg0, sb.nwg, lds);
+    const float rho = reduce_dot(dp, part + (1 + ((k + 1) & 1)) * nb + sb.wg0, sb.nwg, lds);
+    const bool converged = norm < dp.scenes[sb.scene].cg_tol;
     const bool stop = converged || rho_old == 0.f;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        dc->cg_iters = k + 1;
-        dc->cg_norm = norm;
+    if ((int)blockIdx.x == sb.wg0 && threadIdx.x == 0) {
+        scg[sb.scene].iters = k + 1;
+        scg[sb.scene].norm = norm;
+        atomicMax(&dc->cg_iters, k + 1);
         if (stop) {
-            dc->cg_done = 1;
-            if (!converged) dc->cg_breakdown = 1;
+            scg[sb.scene].done = 1;
+            if (!converged) {
+                scg[sb.scene].breakdown = 1;
+                atomicAdd(&dc->cg_breakdown, 1);
+            }
+            if (atomicSub(&dc->cg_active, 1) == 1) dc->cg_done = 1;
         }
     }
     if (stop) return;
     const float beta = rho / rho_old;
-    int nt = dc->num_total;
-    nt = nt < cap_tp ? nt : (int)cap_tp;
-    FOR_128_BLOCKS(nt) {
-        const int n0 = blk * 128 + lane * 2;
-        if ((n0 >> 6) < nt) {
-            const float* gp = grid + (int64_t)(n0 >> 6) * kPageFloats + (n0 & 63);
+    const int num_page = dc->num_page;
+    for (int blk = sb.wave; blk < sb.nblk; blk += sb.nwaves) {
+        const int64_t n0 = scene_block_node(sb, num_page, cap_tp, blk);
+        if (n0 >= 0) {
+            const float* gp = grid + (n0 >> 6) * kPageFloats + (n0 & 63);
 #pragma unroll
             for (int h = 0; h < 2; ++h)
                 if (gp[5 * 64 + h] > 0.f) cgs[n0 + h] = cgs[n0 + h] * beta + cgz[n0 + h];
@@ -467,9 +489,9 @@ static void apply_blocks(mpmx_ctx* c, ST st) {
     const BlockTables t = block_tables_of(c);
     const int g1 = block_grid(c), g2 = g1;
     MPMX_TIMED(c, "cg_col", (k_cg_col<ST><<<(int)g1, 256, 0, c->stream>>>(c->dp, st, c->vol, t, c->cg_s,
-                                                                         c->cg_col, (int)c->n, c->dc)));
+                                                                         c->cg_col, (int)c->n, c->scg, c->dc)));
     MPMX_TIMED(c, "cg_q", (k_cg_q<ST><<<(int)g2, 256, 0, c->stream>>>(c->dp, st, t, c->cg_col, c->grid, c->cg_s,
-                                                                     c->cg_q, (int)c->n, c->dc)));
+                                                                     c->cg_q, (int)c->n, c->scg, c->dc)));
 }
 
 static hipError_t pcg_iteration(mpmx_ctx* c, int k) {
@@ -483,11 +505,12 @@ static hipError_t pcg_iteration(mpmx_ctx* c, int k) {
     const int nb = c->cg_nb;
     MPMX_TIMED(c, "cg_vector", ({
                    k_cg_dot<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_s, c->cg_q, c->cg_part,
-                                                       blocks ? 0 : 1, c->cap_tp, c->dc);
+                                                       blocks ? 0 : 1, c->cap_tp, c->scg, c->wg_scene, c->dc);
                    k_cg_update<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_x, c->cg_r, c->cg_z, c->cg_s,
-                                                          c->cg_q, c->cg_part, nb, k, c->cap_tp, c->dc);
+                                                          c->cg_q, c->cg_part, nb, k, c->cap_tp, c->scg,
+                                                          c->wg_scene, c->dc);
                    k_cg_direction<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_z, c->cg_s, c->cg_part, nb, k,
-                                                             c->cap_tp, c->dc);
+                                                             c->cap_tp, c->scg, c->wg_scene, c->dc);
                }));
     return hipGetLastError();
 }

@@ -160,22 +160,33 @@ hipError_t scan_exclusive(mpmx_ctx* c, const int* in, int* out, int64_t n_cap, c
 }
 
 // ---------------------------------------------------------------- keys
-// cal_cellId (Particle_kernal.cu:158-165): key of the quadratic base node
-__global__ void k_keys(const float* __restrict__ x, uint32_t* __restrict__ keys, int n, int N,
-                       DevCounts* dc) {
+// cal_cellId (Particle_kernal.cu:158-165): key of the quadratic base node.
+// Particles are in upload order here: scene s holds [scene_start[s],
+// scene_start[s + 1]); the key adds the scene's slot origin.
+__global__ void k_keys(DevParams dp, const float* __restrict__ x, uint32_t* __restrict__ keys, int n,
+                       const int* __restrict__ scene_start, DevCounts* dc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    int lo = 0, hi = dp.num_scenes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (scene_start[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const DevParams& sp = dp.scenes[lo];
+    const int N = sp.N;
     const float x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
-    const float lo = 2.0f / N, hi = 1.0f - 2.0f / N;
-    if (!(x0 >= lo && x0 <= hi && x1 >= lo && x1 <= hi && x2 >= lo && x2 <= hi))
+    const float lo_x = 2.0f / N, hi_x = 1.0f - 2.0f / N;
+    if (!(x0 >= lo_x && x0 <= hi_x && x1 >= lo_x && x1 <= hi_x && x2 >= lo_x && x2 <= hi_x))
         atomicOr(&dc->err, kErrDomain);
-    keys[i] = key_of(base_node(x0, N), base_node(x1, N), base_node(x2, N));
+    keys[i] = key_of(base_node(x0, N) + sp.origin[0], base_node(x1, N) + sp.origin[1],
+                     base_node(x2, N) + sp.origin[2]);
 }
 
 hipError_t compute_keys(mpmx_ctx* c) {
     const int n = (int)c->n;
     const float* x = c->cfg.precision == MPMX_MIXED ? c->ms[c->cur].x : c->ps[c->cur].x;
-    k_keys<<<(n + 255) / 256, 256, 0, c->stream>>>(x, c->keys, n, c->P.N, c->dc);
+    k_keys<<<(n + 255) / 256, 256, 0, c->stream>>>(c->dp, x, c->keys, n, c->scene_start, c->dc);
     return hipGetLastError();
 }
 

@@ -293,13 +293,8 @@ __device__ __forceinline__ void count_fallbacks(bool fb, int* counter) {
     if (m && (int)(threadIdx.x & 63) == __ffsll((long long)act) - 1) atomicAdd(counter, (int)__popcll(m));
 }
 
-template <bool ALL64>
-__global__ __launch_bounds__(256) void k_stress_fp32(DevParams dp, PState s, float* __restrict__ sigma,
-                                                     float* __restrict__ FP, float* __restrict__ vol, int n,
-                                                     DevCounts* dc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    StressIn in;
+// state access of the two storage forms
+__device__ __forceinline__ void stress_load(const PState& s, int i, int n, StressIn& in) {
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -308,32 +303,8 @@ __global__ __launch_bounds__(256) void k_stress_fp32(DevParams dp, PState s, flo
     in.g = s.g[i];
     in.alpha = s.alpha[i];
     in.psi = s.psi[i];
-    StressOut out;
-    count_fallbacks(stress_one<ALL64>(dp, in, out), &dc->svd_fallbacks);
-    bool finite = true;
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
-            s.F[(int64_t)(r + 3 * cc) * n + i] = out.F[r][cc];
-            sigma[(int64_t)(r + 3 * cc) * n + i] = out.sigma[r][cc];
-            finite = finite && isfinite(out.sigma[r][cc]);
-        }
-    s.g[i] = out.g;
-    s.alpha[i] = out.alpha;
-    s.psi[i] = out.psi;
-    FP[i] = out.FP;
-    vol[i] = out.vol;
-    if (!finite) atomicOr(&dc->err, kErrNonFinite);
 }
-
-template <bool ALL64>
-__global__ __launch_bounds__(256) void k_stress_mixed(DevParams dp, MState s, float* __restrict__ sigma,
-                                                      float* __restrict__ FP, float* __restrict__ vol, int n,
-                                                      DevCounts* dc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    StressIn in;
+__device__ __forceinline__ void stress_load(const MState& s, int i, int n, StressIn& in) {
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -343,20 +314,49 @@ __global__ __launch_bounds__(256) void k_stress_mixed(DevParams dp, MState s, fl
     in.g = bf2f(s.g[i]);
     in.alpha = bf2f(s.alpha[i]);
     in.psi = bf2f(s.psi[i]);
+}
+__device__ __forceinline__ void stress_store(const PState& s, int i, int n, const StressOut& out) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) s.F[(int64_t)(r + 3 * cc) * n + i] = out.F[r][cc];
+    s.g[i] = out.g;
+    s.alpha[i] = out.alpha;
+    s.psi[i] = out.psi;
+}
+__device__ __forceinline__ void stress_store(const MState& s, int i, int n, const StressOut& out) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+            s.F[(int64_t)(r + 3 * cc) * n + i] = f2h(out.F[r][cc] - (r == cc ? 1.0f : 0.0f));
+    s.g[i] = f2bf(out.g);
+    s.alpha[i] = f2bf(out.alpha);
+    s.psi[i] = f2bf(out.psi);
+}
+
+// one thread per particle; every wave holds particles of one scene, whose
+// constants it reads once
+template <bool ALL64, typename ST>
+__global__ __launch_bounds__(256) void k_stress(DevParams dp, ST s, WaveMap wm, float* __restrict__ sigma,
+                                                float* __restrict__ FP, float* __restrict__ vol, int n,
+                                                DevCounts* dc) {
+    int i, scene;
+    if (!wave_particle(wm, n, i, scene)) return;
+    const DevParams& sp = dp.scenes[scene];
+    StressIn in;
+    stress_load(s, i, n, in);
     StressOut out;
-    count_fallbacks(stress_one<ALL64>(dp, in, out), &dc->svd_fallbacks);
+    count_fallbacks(stress_one<ALL64>(sp, in, out), &dc->svd_fallbacks);
     bool finite = true;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) {
-            s.F[(int64_t)(r + 3 * cc) * n + i] = f2h(out.F[r][cc] - (r == cc ? 1.0f : 0.0f));
             sigma[(int64_t)(r + 3 * cc) * n + i] = out.sigma[r][cc];
             finite = finite && isfinite(out.sigma[r][cc]);
         }
-    s.g[i] = f2bf(out.g);
-    s.alpha[i] = f2bf(out.alpha);
-    s.psi[i] = f2bf(out.psi);
+    stress_store(s, i, n, out);
     FP[i] = out.FP;
     vol[i] = out.vol;
     if (!finite) atomicOr(&dc->err, kErrNonFinite);
@@ -366,11 +366,11 @@ template <bool ALL64>
 static void stress_dispatch(mpmx_ctx* c) {
     const int n = (int)c->n;
     if (c->cfg.precision == MPMX_MIXED)
-        k_stress_mixed<ALL64><<<(n + 255) / 256, 256, 0, c->stream>>>(c->dp, c->ms[c->cur], c->sigma, c->FP,
-                                                                      c->vol, n, c->dc);
+        k_stress<ALL64, MState><<<particle_blocks(c), 256, 0, c->stream>>>(c->dp, c->ms[c->cur], c->wmap, c->sigma,
+                                                                           c->FP, c->vol, n, c->dc);
     else
-        k_stress_fp32<ALL64><<<(n + 255) / 256, 256, 0, c->stream>>>(c->dp, c->ps[c->cur], c->sigma, c->FP,
-                                                                     c->vol, n, c->dc);
+        k_stress<ALL64, PState><<<particle_blocks(c), 256, 0, c->stream>>>(c->dp, c->ps[c->cur], c->wmap, c->sigma,
+                                                                           c->FP, c->vol, n, c->dc);
 }
 
 hipError_t launch_stress(mpmx_ctx* c) {

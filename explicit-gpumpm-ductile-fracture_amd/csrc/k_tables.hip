@@ -52,15 +52,28 @@ __device__ __forceinline__ uint64_t dilate(uint64_t m, uint64_t prev) {
     return m | ((m & lo0) << (1 << A)) | ((m & lo1hi0) << (7 << A)) | ((prev & top) >> (9 << A));
 }
 
-__global__ void k_neighbour_mask(const uint64_t* __restrict__ pmask, uint64_t* __restrict__ nmask, int64_t words) {
+// Each scene's dilation stays inside its slot and below its own page limit
+// (the page coordinates its domain alone would hold), so a scene of a batch
+// gets the neighbour pages it would get alone, less the ones past the slot,
+// which no particle of the domain reaches.
+__global__ void k_neighbour_mask(DevParams dp, const uint64_t* __restrict__ pmask, uint64_t* __restrict__ nmask,
+                                 int64_t words) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= words) return;
+    const int wshift = dp.slot_shift - 6;   // 2^wshift words per slot
+    const int64_t local = w & ((1ll << wshift) - 1);
+    const int lim = dp.scenes[(int)(w >> wshift)].page_limit >> 2;   // words per axis
     int cx, cy, cz;
-    unmorton((uint32_t)w, cx, cy, cz);
+    unmorton((uint32_t)local, cx, cy, cz);
+    if (cx >= lim || cy >= lim || cz >= lim) {
+        nmask[w] = 0ull;
+        return;
+    }
+    const int64_t slot_base = w - local;
     auto P = [&](int dx, int dy, int dz) -> uint64_t {
         const int x = cx - dx, y = cy - dy, z = cz - dz;
         if (x < 0 || y < 0 || z < 0) return 0ull;
-        return pmask[morton((uint32_t)x, (uint32_t)y, (uint32_t)z)];
+        return pmask[slot_base + morton((uint32_t)x, (uint32_t)y, (uint32_t)z)];
     };
     // dilate along z the four words the y and x steps need, then along y, then x
     const uint64_t a00 = dilate<0>(P(0, 0, 0), P(0, 0, 1)), a01 = dilate<0>(P(0, 1, 0), P(0, 1, 1));
@@ -185,19 +198,42 @@ __global__ void k_block_keys(const uint64_t* __restrict__ pmask, const uint64_t*
         }
 }
 
+// page ranges of every scene: its particle pages and neighbour-only pages are
+// the ranks of its slot's words in the two prefix scans
+__global__ void k_scene_counts(DevParams dp, const int* __restrict__ wpre, const int* __restrict__ spre,
+                               SceneCG* __restrict__ scg, const DevCounts* __restrict__ dc) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= dp.num_scenes) return;
+    const int wshift = dp.slot_shift - 6;
+    const int64_t w0 = (int64_t)s << wshift, w1 = (int64_t)(s + 1) << wshift;
+    const bool last = s + 1 == dp.num_scenes;
+    const int p0 = wpre[w0], p1 = last ? dc->num_page : wpre[w1];
+    const int n0 = spre[w0], n1 = last ? dc->num_supp : spre[w1];
+    scg[s].pfirst = p0;
+    scg[s].np = p1 - p0;
+    scg[s].nfirst = n0;
+    scg[s].nn = n1 - n0;
+}
+
+hipError_t launch_scene_counts(mpmx_ctx* c) {
+    k_scene_counts<<<(c->num_scenes + 255) / 256, 256, 0, c->stream>>>(c->dp, c->wcount, c->scount, c->scg, c->dc);
+    return hipGetLastError();
+}
+
 hipError_t build_tables(mpmx_ctx* c) {
     const int n = (int)c->n;
     const int64_t words = c->mask_words;
     hipError_t e;
     if ((e = fill_zero(c, c->pmask, words * 8)) != hipSuccess) return e;
     k_mark_pages<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask);
-    k_neighbour_mask<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pmask, c->nmask, words);
+    k_neighbour_mask<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->dp, c->pmask, c->nmask, words);
     k_word_counts<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pmask, c->nmask, c->wcount,
                                                                       c->scount, words);
     if ((e = scan_exclusive(c, c->wcount, c->wcount, words, nullptr, words, &c->dc->num_page)) != hipSuccess)
         return e;
     if ((e = scan_exclusive(c, c->scount, c->scount, words, nullptr, words, &c->dc->num_supp)) != hipSuccess)
         return e;
+    if ((e = launch_scene_counts(c)) != hipSuccess) return e;
     k_page_keys<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pmask, c->nmask, c->wcount, c->scount,
                                                                     words, c->page_key, c->cap_p, c->cap_tp,
                                                                     c->dc);

@@ -120,9 +120,10 @@ __global__ __launch_bounds__(BLOCK) void k_p2g(DevParams dp, ST s, const float* 
 }
 
 // G2P of particle i from a staged node region of R^3 nodes whose node (0,0,0)
-// is node org: channel ch of region node (a, b, c) at tv[ch R^3 + (a R + b) R + c]
-// (v0, v1, v2 and the damage increment).  MLS velocity clamp, c update,
-// F = (I + dt grad v) F, B, x, the next cell key and the error flags.
+// is node org (scene-local cells): channel ch of region node (a, b, c) at
+// tv[ch R^3 + (a R + b) R + c] (v0, v1, v2 and the damage increment).  MLS
+// velocity clamp, c update, F = (I + dt grad v) F, B, x, the next cell key
+// (with the scene's slot origin) and the error flags.
 template <int TRANSFER, int R, typename ST>
 __device__ __forceinline__ void g2p_particle(const DevParams& dp, const ST& s, int i, int n, float dt,
                                              const int (&org)[3], const float* tv, uint32_t* __restrict__ keys,
@@ -213,7 +214,8 @@ __device__ __forceinline__ void g2p_particle(const DevParams& dp, const ST& s, i
     if (!(xn[0] >= lo && xn[0] <= hi && xn[1] >= lo && xn[1] <= hi && xn[2] >= lo && xn[2] <= hi))
         atomicOr(&dc->err, isfinite(xn[0] + xn[1] + xn[2]) ? kErrDomain : kErrNonFinite);
     if (!finite) atomicOr(&dc->err, kErrNonFinite);
-    keys[i] = key_of(base_node(xn[0], dp.N), base_node(xn[1], dp.N), base_node(xn[2], dp.N));
+    keys[i] = key_of(base_node(xn[0], dp.N) + dp.origin[0], base_node(xn[1], dp.N) + dp.origin[1],
+                     base_node(xn[2], dp.N) + dp.origin[2]);
 }
 
 // the grid values G2P reads at global node gn (page * 64 + cell): v0..v2 and
@@ -276,7 +278,7 @@ constexpr int kBR3 = kBR * kBR * kBR;
 template <int TRANSFER, typename ST>
 __global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTables t, const float* __restrict__ grid,
                                                    const float* __restrict__ cgx, uint32_t* __restrict__ keys,
-                                                   float dt, int n, DevCounts* dc) {
+                                                   float dt, int n, const SceneCG* __restrict__ scg, DevCounts* dc) {
     __shared__ float tv[4 * kBR3];
     __shared__ int spg[27];
     __shared__ int prange[2];
@@ -286,18 +288,19 @@ __global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTabl
     nt = nt < t.cap_tp ? nt : (int)t.cap_tp;
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
-    const bool breakdown = dc->cg_breakdown != 0;
     FOR_LOGICAL_ITEMS(blk, nb) {
         const uint32_t bkey = t.block_key[blk];
+        const int scene = __builtin_amdgcn_readfirstlane(scene_of_page(dp, bkey << 3));
+        const DevParams sp = dp.scenes[scene];   // loaded once per block, before the particle loop
+        const bool breakdown = scg[scene].breakdown != 0;
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
         const int tid = threadIdx.x;
         if (tid < 27) {
             const int px = 2 * bc[0] + tid / 9, py = 2 * bc[1] + (tid / 3) % 3, pz = 2 * bc[2] + tid % 3;
             int id = -1;
-            if (px < t.axis_pages && py < t.axis_pages && pz < t.axis_pages)
-                id = page_rank(morton((uint32_t)px, (uint32_t)py, (uint32_t)pz), t.pmask, t.nmask, t.wpre, t.spre,
-                               np_raw);
+            const uint32_t key = scene_page_key(t, px, py, pz, scene);
+            if (key != ~0u) id = page_rank(key, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
             spg[tid] = (id >= 0 && id < nt) ? id : -1;
         }
         if (tid >= 64 && tid < 128) {
@@ -332,7 +335,7 @@ __global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTabl
                 const int G = spg[(X >> 2) * 9 + (Y >> 2) * 3 + (Z >> 2)];
                 float v0 = 0.f, v1 = 0.f, v2 = 0.f, dcn = 0.f;
                 if (G >= 0)
-                    g2p_node(dp, grid, cgx, breakdown, (int64_t)G * 64 + (((X & 3) << 4) | ((Y & 3) << 2) | (Z & 3)),
+                    g2p_node(sp, grid, cgx, breakdown, (int64_t)G * 64 + (((X & 3) << 4) | ((Y & 3) << 2) | (Z & 3)),
                              v0, v1, v2, dcn);
                 tv[q] = v0;
                 tv[kBR3 + q] = v1;
@@ -340,8 +343,8 @@ __global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTabl
                 tv[3 * kBR3 + q] = dcn;
             }
             __syncthreads();
-            const int org[3] = {8 * bc[0], 8 * bc[1], 8 * bc[2]};
-            for (int i = pa + tid; i < pb; i += 256) g2p_particle<TRANSFER, kBR>(dp, s, i, n, dt, org, tv, keys, dc);
+            const int org[3] = {8 * bc[0] - sp.origin[0], 8 * bc[1] - sp.origin[1], 8 * bc[2] - sp.origin[2]};
+            for (int i = pa + tid; i < pb; i += 256) g2p_particle<TRANSFER, kBR>(sp, s, i, n, dt, org, tv, keys, dc);
         }
         __syncthreads();
     }
@@ -430,10 +433,10 @@ static void g2p_block_kernel(mpmx_ctx* c, ST st, float dt) {
     MPMX_TIMED(c, "g2p_block", ({
                    if (c->P.transfer == 0)
                        k_g2p_block<0, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys, dt,
-                                                                         (int)c->n, c->dc);
+                                                                         (int)c->n, c->scg, c->dc);
                    else
                        k_g2p_block<1, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys, dt,
-                                                                         (int)c->n, c->dc);
+                                                                         (int)c->n, c->scg, c->dc);
                }));
 }
 

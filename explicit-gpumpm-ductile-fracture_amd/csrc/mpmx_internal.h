@@ -19,6 +19,14 @@
 //  PCG        x, r, z, s, q float[cap_tp*64] page-major (the reference's d_grid_*);
 //             partials float[4][cg_nb]: s.q, r.z (two, by iteration parity), max|r|
 //  blocks     block_key uint32[tp_bound]: occupied 2x2x2-page blocks (page key >> 3)
+//  scenes     a context holds one scene or a batch of independent scenes.  Scene
+//             s occupies slot s of the Morton page space: a cube of 2^slot_log2
+//             pages per axis whose page keys are [s << slot_shift, (s+1) <<
+//             slot_shift), so a page's scene is its key >> slot_shift, the
+//             sorted particles of a scene are contiguous, and every 2x2x2
+//             block belongs to one scene.  Positions stay in the scene's own
+//             units; keys add the slot origin.  Per-scene constants: scenes[s]
+//             (DevParams); per-scene PCG state and page ranges: SceneCG[s].
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -62,6 +70,29 @@ struct DevParams {
     float L0;
     float sq23f;   // sqrt(2/3) as float
     float svd_gap; // FP32 stress arithmetic: FP64 redo below this singular-value gap
+    // scenes: constants of scene s at scenes[s] (the context's own copy of
+    // this struct is scene 0 and carries the batch-wide fields)
+    const DevParams* scenes;
+    int num_scenes;
+    int slot_shift;     // scene of a page key = key >> slot_shift
+    int origin[3];      // this scene's cell (0,0,0) in the page space, per axis
+    int page_limit;     // neighbour pages exist below this page coordinate (per axis, scene-local)
+};
+
+// per-scene PCG state and page ranges (device array, one per scene).  The
+// scene's pages are its particle pages [pfirst, pfirst + np) and its
+// neighbour-only pages [num_page + nfirst, num_page + nfirst + nn) of the
+// context's page numbering; scene-local page l is the l-th of that list, as
+// the scene alone would number it.  Workgroups [wg0, wg0 + nwg) of every
+// node-space PCG kernel serve the scene.
+struct SceneCG {
+    int done;
+    int iters;
+    int breakdown;
+    float norm;
+    int pfirst, np, nfirst, nn;
+    int wg0, nwg;
+    int pad[6];
 };
 
 // device-resident counters and PCG state.  The PCG scalars themselves are not
@@ -81,7 +112,8 @@ struct DevCounts {
     int num_cell;
     int svd_fallbacks;
     int num_blocks;     // occupied 2x2x2-page blocks
-    int pad[20];
+    int cg_active;      // scenes whose PCG is still running
+    int pad[19];
 };
 
 struct PState {
@@ -111,6 +143,16 @@ struct MState {
     int* id = nullptr;
 };
 
+// Lanes -> particles of the per-particle kernels.  Every wave holds
+// particles of one scene: a batch pads each scene's (contiguous) particle
+// range to whole waves, so a kernel reads its scene's constants once per wave,
+// through the scalar cache.  One scene: wave w holds particles 64w .. 64w+63.
+struct WaveMap {
+    const int* first;   // per wave: first particle; nullptr for one scene
+    const int* count;   // per wave: particles in the wave (<= 64)
+    const int* scene;   // per wave: scene
+    int64_t waves;
+};
 struct TimerEntry {
     std::string name;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -122,6 +164,17 @@ struct mpmx_ctx {
     mpmx_params P;
     mpmx_config cfg;
     mpmx::DevParams dp;
+    // scenes (a single-scene context is a batch of one)
+    int num_scenes = 1;
+    std::vector<mpmx_params> scene_P;        // per-scene parameters as given
+    std::vector<int64_t> scene_n;            // particles per scene
+    mpmx::DevParams* scene_dp = nullptr;     // device [num_scenes]
+    mpmx::SceneCG* scg = nullptr;            // device [num_scenes]
+    int* wg_scene = nullptr;                 // device [cg_nb]: scene of each PCG workgroup
+    int* scene_start = nullptr;              // device [num_scenes + 1]: upload-order particle ranges
+    mpmx::WaveMap wmap{};                    // per-particle kernels' waves (batches only)
+    int* wave_buf = nullptr;                 // device storage of wmap
+    int slot_log2 = 0;                       // pages per slot axis = 2^slot_log2
     int device = 0;
     hipStream_t stream = nullptr;
     int64_t n = 0, cap_n = 0;
@@ -344,6 +397,64 @@ __device__ __forceinline__ void block_range(int count, int& begin, int& end) {
     end = begin + per < count ? begin + per : count;
 }
 
+// scene of page key pk
+__device__ __forceinline__ int scene_of_page(const DevParams& dp, uint32_t pk) { return (int)(pk >> dp.slot_shift); }
+
+// the calling lane's particle i and the wave's scene; false for idle lanes
+__device__ __forceinline__ bool wave_particle(const WaveMap& m, int n, int& i, int& scene) {
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (!m.first) {
+        i = (int)(w * 64 + lane);
+        scene = 0;
+        return i < n;
+    }
+    if (w >= m.waves) {
+        i = 0;
+        scene = 0;
+        return false;
+    }
+    i = m.first[w] + lane;
+    scene = __builtin_amdgcn_readfirstlane(m.scene[w]);
+    return lane < m.count[w];
+}
+
+// The 128-node blocks of the PCG's "dot" products, per scene: block b of a
+// scene is its scene-local pages 2b and 2b+1 (as the scene alone numbers its
+// pages); lane l takes nodes 2 (l & 31) and 2 (l & 31) + 1 of page
+// 2b + (l >> 5).  Workgroup blockIdx.x serves scene wg_scene[blockIdx.x]; the
+// waves of that scene's workgroups stride over its blocks, and each workgroup
+// writes its partial at blockIdx.x, so a scene's partials are [wg0, wg0 + nwg).
+struct SceneBlocks {
+    int scene, wg0, nwg, pfirst, np, nfirst, nn, nblk, wave, nwaves;
+};
+__device__ __forceinline__ SceneBlocks scene_blocks(const SceneCG* __restrict__ scg,
+                                                    const int* __restrict__ wg_scene) {
+    SceneBlocks b;
+    b.scene = wg_scene[blockIdx.x];
+    const SceneCG& g = scg[b.scene];
+    b.wg0 = g.wg0;
+    b.nwg = g.nwg;
+    b.pfirst = g.pfirst;
+    b.np = g.np;
+    b.nfirst = g.nfirst;
+    b.nn = g.nn;
+    b.nblk = (g.np + g.nn + 1) / 2;
+    const int wpb = (int)(blockDim.x >> 6);
+    b.wave = (int)(blockIdx.x - b.wg0) * wpb + (int)(threadIdx.x >> 6);
+    b.nwaves = b.nwg * wpb;
+    return b;
+}
+// first of the lane's two nodes in block blk of the scene, or -1
+__device__ __forceinline__ int64_t scene_block_node(const SceneBlocks& b, int num_page, int64_t cap_tp, int blk) {
+    const int lane = threadIdx.x & 63;
+    const int l = 2 * blk + (lane >> 5);
+    if (l >= b.np + b.nn) return -1;
+    const int pg = l < b.np ? b.pfirst + l : num_page + b.nfirst + (l - b.np);
+    if (pg < 0 || pg >= cap_tp) return -1;
+    return (int64_t)pg * 64 + 2 * (lane & 31);
+}
+
 }  // namespace mpmx
 
 // ---- host-side launch helpers (defined in the .hip files) -------------------
@@ -353,6 +464,8 @@ hipError_t scan_exclusive(mpmx_ctx* c, const int* in, int* out, int64_t n_cap, c
 hipError_t sort_and_gather(mpmx_ctx* c);
 hipError_t compute_keys(mpmx_ctx* c);
 hipError_t build_tables(mpmx_ctx* c);
+hipError_t launch_scene_counts(mpmx_ctx* c);
+int particle_blocks(const mpmx_ctx* c);   // 256-thread workgroups of the per-particle kernels
 hipError_t launch_stress(mpmx_ctx* c);
 hipError_t launch_p2g(mpmx_ctx* c, float dt);
 hipError_t launch_grid_update(mpmx_ctx* c, float dt);

@@ -12,6 +12,14 @@
 //   3  two-block impact: two 80^3-cell blocks x 8 ppc in a 512^3 grid, 8-cell
 //      gap along z, v_z = +-2.5 m/s
 //   4  sphere of radius 16 cells at the centre of a 128^3 grid, 8 ppc
+// and the batches (mpmx_scene_batch):
+//   4  64 scenes, particle counts log-uniform in [10k, 500k]
+//   5  256 scenes, particle counts log-uniform in [5k, 200k]
+// A batch scene is a cube, a notched bar (4:1:1, grips at +-0.5 m/s) or a
+// sphere (falling at 2 m/s), in turn, with E log-uniform in [1e4, 1e6] and
+// yield stress E / 100; particles are uniform-random in the shape at a
+// density of 1 to 8 per cell, and the grid (64^3, 128^3 or 256^3, uniform
+// among those in which the shape spans at most 60 % of an axis) follows.
 // Lattice scenes place 2x2x2 particles per cell at (0.25, 0.75) dx with a
 // uniform jitter of +-0.125 dx.  `scale` shrinks the geometry (lattice
 // scenes) or the particle count (scene 1).  Density follows
@@ -20,6 +28,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "../../include/mpmx.h"
@@ -168,7 +177,112 @@ bool build(int scene, uint64_t seed, double scale, Scene& s) {
     return true;
 }
 
+// scene k of a batch: parameters and particles appended to x, v
+void batch_scene(int config, uint64_t seed, int k, double scale, Scene& s) {
+    Rng rng{(seed * 0x2545F4914F6CDD1Dull) ^ (0x9E3779B97F4A7C15ull * (uint64_t)(k + 1))};
+    for (int q = 0; q < 4; ++q) rng.next();
+    mpmx_default_params(&s.p);
+    s.dt = 1e-4;
+    const double pmin = config == 4 ? 1e4 : 5e3, pmax = config == 4 ? 5e5 : 2e5;
+    int64_t P = (int64_t)std::llround(std::exp(std::log(pmin) + rng.u01() * (std::log(pmax) - std::log(pmin))) * scale);
+    if (P < 16) P = 16;
+    s.p.E = std::exp(std::log(1e4) + rng.u01() * (std::log(1e6) - std::log(1e4)));
+    s.p.nu = 0.3;
+    s.p.yield_stress = s.p.E / 100;
+    const int shape = k % 3;
+    const double ppc = 1.0 + 7.0 * rng.u01();
+    const double vol_cells = (double)P / ppc;
+    // linear size in cells: cube side, bar length (4:1:1), sphere diameter
+    const double L = shape == 0 ? std::cbrt(vol_cells) : shape == 1 ? std::cbrt(16.0 * vol_cells)
+                                                                    : std::cbrt(6.0 * vol_cells / M_PI);
+    const int grids[3] = {64, 128, 256};
+    int first = 0;
+    while (first < 2 && L > 0.6 * grids[first]) ++first;
+    const int N = grids[first + (int)(rng.next() % (uint64_t)(3 - first))];
+    s.p.N = N;
+    const double dx = 1.0 / N, c = 0.5;
+    const double h = std::max(L, 4.0) * dx;   // size in domain units
+    double lo[3], hi[3];
+    std::function<bool(const double*)> inside;
+    double v0[3] = {0, 0, 0};
+    if (shape == 0) {
+        for (int a = 0; a < 3; ++a) { lo[a] = c - h / 2; hi[a] = c + h / 2; }
+        inside = [](const double*) { return true; };
+    } else if (shape == 1) {
+        const double w = h / 4;
+        lo[0] = c - h / 2; hi[0] = c + h / 2;
+        for (int a = 1; a < 3; ++a) { lo[a] = c - w / 2; hi[a] = c + w / 2; }
+        const double ytop = hi[1], depth = 0.25 * w;
+        inside = [=](const double* q) { return !(q[1] > ytop - (depth - std::fabs(q[0] - c))); };
+        const int i0 = (int)std::floor(lo[0] * N), len = (int)std::ceil(h * N);
+        const int grip = std::max(1, len / 8);
+        s.p.grip_lo = i0 + grip;
+        s.p.grip_hi = i0 + len - grip;
+        s.p.grip_v = 0.5;
+        s.p.crit_stretch = 1.05;
+    } else {
+        for (int a = 0; a < 3; ++a) { lo[a] = c - h / 2; hi[a] = c + h / 2; }
+        const double r2 = h * h / 4;
+        inside = [=](const double* q) {
+            const double a0 = q[0] - c, a1 = q[1] - c, a2 = q[2] - c;
+            return a0 * a0 + a1 * a1 + a2 * a2 <= r2;
+        };
+        v0[1] = -2.0;
+    }
+    int64_t got = 0;
+    while (got < P) {
+        double q[3];
+        for (int a = 0; a < 3; ++a) q[a] = lo[a] + (hi[a] - lo[a]) * rng.u01();
+        if (!inside(q)) continue;
+        s.x.insert(s.x.end(), q, q + 3);
+        s.v.insert(s.v.end(), v0, v0 + 3);
+        ++got;
+    }
+    // shape volume by the acceptance of a fixed sample of the bounding box
+    double box = 1;
+    for (int a = 0; a < 3; ++a) box *= hi[a] - lo[a];
+    int acc = 0;
+    Rng vr{0x51ED270B27A5F3ull + (uint64_t)k};
+    for (int t = 0; t < 20000; ++t) {
+        double q[3];
+        for (int a = 0; a < 3; ++a) q[a] = lo[a] + (hi[a] - lo[a]) * vr.u01();
+        acc += inside(q) ? 1 : 0;
+    }
+    s.V0 = box * acc / 20000.0 / (double)P;
+    s.p.V0 = s.V0;
+}
+
 }  // namespace
+
+/* Batches of independent scenes (configs 4 and 5, described above).  Call with
+ * null arrays to size them: *num_scenes (in: at most this many scenes when
+ * > 0; out: the count) and the particle total (returned).  params[s],
+ * counts[s]; x, v, mass hold the scenes one after another. */
+extern "C" int64_t mpmx_scene_batch(int config, uint64_t seed, double scale, int* num_scenes, mpmx_params* params,
+                                    int64_t* counts, double* x, double* v, double* mass, double* dt) {
+    if ((config != 4 && config != 5) || !num_scenes) return -1;
+    if (scale <= 0) scale = 1;
+    int S = config == 4 ? 64 : 256;
+    if (*num_scenes > 0 && *num_scenes < S) S = *num_scenes;
+    *num_scenes = S;
+    int64_t off = 0;
+    for (int k = 0; k < S; ++k) {
+        Scene s;
+        batch_scene(config, seed, k, scale, s);
+        const int64_t n = (int64_t)(s.x.size() / 3);
+        if (params) params[k] = s.p;
+        if (counts) counts[k] = n;
+        if (x) std::memcpy(x + 3 * off, s.x.data(), sizeof(double) * 3 * n);
+        if (v) std::memcpy(v + 3 * off, s.v.data(), sizeof(double) * 3 * n);
+        if (mass) {
+            const double rho = density_rule(s.p, s.dt);
+            for (int64_t i = 0; i < n; ++i) mass[off + i] = rho * s.V0;
+        }
+        if (dt) *dt = s.dt;
+        off += n;
+    }
+    return off;
+}
 
 extern "C" int64_t mpmx_scene(int scene, uint64_t seed, double scale, mpmx_params* p, double* x, double* v,
                               double* mass, double* dt) {

@@ -27,7 +27,8 @@ EXPORTS = [
     "mpmx_substep", "mpmx_run", "mpmx_compute_dt", "mpmx_sync", "mpmx_get_field", "mpmx_set_field",
     "mpmx_get_order", "mpmx_get_tables", "mpmx_get_grid", "mpmx_get_grid_p2g", "mpmx_get_stats",
     "mpmx_timing_begin", "mpmx_timing_end", "mpmx_time_run", "mpmx_scene", "mpmx_get_device_pointers",
-    "mpmx_max_velocity_sq",
+    "mpmx_max_velocity_sq", "mpmx_create_batch", "mpmx_set_particles_batch", "mpmx_num_scenes",
+    "mpmx_get_scene_stats", "mpmx_scene_batch",
 ]
 
 
@@ -62,6 +63,13 @@ class Stats(C.Structure):
                 ("num_virtual_page", C.c_int), ("cg_iters", C.c_int), ("cg_breakdown", C.c_int),
                 ("substeps", C.c_int), ("cg_norm", C.c_double), ("mass", C.c_double),
                 ("momentum", C.c_double * 3), ("kinetic_energy", C.c_double)]
+
+
+class SceneStats(C.Structure):
+    _fields_ = [("num_particles", C.c_int64), ("first_particle", C.c_int64), ("num_page", C.c_int),
+                ("num_total_page", C.c_int), ("cg_iters", C.c_int), ("cg_breakdown", C.c_int),
+                ("cg_norm", C.c_double), ("mass", C.c_double), ("momentum", C.c_double * 3),
+                ("kinetic_energy", C.c_double)]
 
 
 class KernelTime(C.Structure):
@@ -129,6 +137,13 @@ def lib():
                         C.POINTER(C.c_double)], C.c_int64),
         "mpmx_get_device_pointers": ([vp, vp], C.c_int),
         "mpmx_max_velocity_sq": ([vp, C.POINTER(C.c_double)], C.c_int),
+        "mpmx_create_batch": ([C.c_int, C.POINTER(Params), C.POINTER(Config), C.POINTER(vp)], C.c_int),
+        "mpmx_set_particles_batch": ([vp, np.ctypeslib.ndpointer(np.int64, flags="C"), f64p, f64p, f64p],
+                                     C.c_int),
+        "mpmx_num_scenes": ([vp], C.c_int),
+        "mpmx_get_scene_stats": ([vp, C.c_int, C.POINTER(SceneStats)], C.c_int),
+        "mpmx_scene_batch": ([C.c_int, C.c_uint64, C.c_double, C.POINTER(C.c_int), vp, vp, vp, vp, vp,
+                              C.POINTER(C.c_double)], C.c_int64),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -177,14 +192,39 @@ def scene(scene_id, seed=1, scale=1.0):
     return p, x, v, m, dt.value
 
 
+def scene_batch(config, seed=1, scale=1.0, num_scenes=0):
+    """Seeded batch of scenes: returns (params list, counts[S], x[n,3], v[n,3], mass[n], dt)."""
+    L = lib()
+    S = C.c_int(num_scenes)
+    dt = C.c_double()
+    n = L.mpmx_scene_batch(config, seed, scale, C.byref(S), None, None, None, None, None, C.byref(dt))
+    if n < 0:
+        raise ValueError("unknown batch config %d" % config)
+    ps = (Params * S.value)()
+    counts = np.zeros(S.value, np.int64)
+    x = np.zeros((n, 3))
+    v = np.zeros((n, 3))
+    m = np.zeros(n)
+    L.mpmx_scene_batch(config, seed, scale, C.byref(S), C.cast(ps, C.c_void_p), counts.ctypes.data,
+                       x.ctypes.data, v.ctypes.data, m.ctypes.data, C.byref(dt))
+    return list(ps), counts, x, v, m, dt.value
+
+
 class Context:
-    """One simulation context (device memory + HIP stream) over the C ABI."""
+    """One simulation context (device memory + HIP stream) over the C ABI.
+    `params` is one Params (one scene) or a list of them (a batch of scenes)."""
 
     def __init__(self, params, config=None):
         self.L = lib()
         self.h = C.c_void_p()
         cfg = config if config is not None else default_config()
-        st = self.L.mpmx_create(C.byref(params), C.byref(cfg), C.byref(self.h))
+        if isinstance(params, (list, tuple)):
+            arr = (Params * len(params))(*params)
+            st = self.L.mpmx_create_batch(len(params), arr, C.byref(cfg), C.byref(self.h))
+            self.num_scenes = len(params)
+        else:
+            st = self.L.mpmx_create(C.byref(params), C.byref(cfg), C.byref(self.h))
+            self.num_scenes = 1
         if st != OK:
             raise MpmxError(st, self.L.mpmx_status_string(st).decode())
         self.params = params
@@ -212,6 +252,22 @@ class Context:
         mass = np.ascontiguousarray(mass, np.float64).reshape(-1)
         self.n = mass.shape[0]
         self._chk(self.L.mpmx_set_particles(self.h, self.n, x, v, mass))
+
+    def set_particles_batch(self, counts, x, v, mass):
+        counts = np.ascontiguousarray(counts, np.int64)
+        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        v = np.ascontiguousarray(v, np.float64).reshape(-1)
+        mass = np.ascontiguousarray(mass, np.float64).reshape(-1)
+        self.n = mass.shape[0]
+        self._chk(self.L.mpmx_set_particles_batch(self.h, counts, x, v, mass))
+
+    def scene_stats(self, s):
+        st = SceneStats()
+        self._chk(self.L.mpmx_get_scene_stats(self.h, s, C.byref(st)))
+        return {"num_particles": st.num_particles, "first_particle": st.first_particle,
+                "numPage": st.num_page, "numTotalPage": st.num_total_page, "cg_iters": st.cg_iters,
+                "cg_breakdown": st.cg_breakdown, "cg_norm": st.cg_norm, "mass": st.mass,
+                "momentum": list(st.momentum), "kinetic_energy": st.kinetic_energy}
 
     def bin_sort(self): self._chk(self.L.mpmx_bin_sort(self.h))
     def rebuild(self): self._chk(self.L.mpmx_rebuild(self.h))

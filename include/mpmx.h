@@ -117,6 +117,20 @@ typedef struct mpmx_stats {
     double  kinetic_energy;
 } mpmx_stats;
 
+/* one scene of a batch context (mpmx_get_scene_stats) */
+typedef struct mpmx_scene_stats {
+    int64_t num_particles;
+    int64_t first_particle;    /* the scene's particles are [first, first + num) of the upload order */
+    int     num_page;          /* the scene's particle pages                                     */
+    int     num_total_page;    /* + its neighbour-only pages                                     */
+    int     cg_iters;          /* iterations of its last PCG                                     */
+    int     cg_breakdown;
+    double  cg_norm;
+    double  mass;              /* FP64 sums over the scene's particles                           */
+    double  momentum[3];
+    double  kinetic_energy;
+} mpmx_scene_stats;
+
 /* per-kernel average durations of the last timed region (mpmx_timing_begin/end) */
 typedef struct mpmx_kernel_time {
     char   name[32];
@@ -143,6 +157,26 @@ mpmx_status mpmx_destroy(mpmx_ctx* ctx);
  * canonical sort are computed here. */
 mpmx_status mpmx_set_particles(mpmx_ctx* ctx, int64_t n, const double* x, const double* v,
                                const double* mass);
+
+/* Batched scenes: num_scenes independent scenes, each with its own parameters
+ * (grid N, material, yield, grips, walls, gravity, energy model), advanced
+ * together by every stage and substep call with one dt.  No counterpart in
+ * the reference, which runs one scene per process (Simulator.cpp:113-143).
+ * transfer, vpage, dot_mode and cg_max_iter must agree across scenes;
+ * p2g_variant 1 is single-scene only.  Each scene's PCG keeps its own
+ * scalars, iteration count and stopping test.  Positions stay in each
+ * scene's own unit cube.  The key space limits the batch:
+ * 6 + 3 ceil(log2(max N / 4)) + ceil(log2(num_scenes)) <= 32 (256 scenes of
+ * N <= 256).  A single-scene context is mpmx_create_batch(1, ...). */
+mpmx_status mpmx_create_batch(int num_scenes, const mpmx_params* params, const mpmx_config* cfg,
+                              mpmx_ctx** out);
+/* counts[num_scenes]; x, v (3 per particle) and mass hold the scenes one
+ * after another.  Fields come back in this order (mpmx_get_field). */
+mpmx_status mpmx_set_particles_batch(mpmx_ctx* ctx, const int64_t* counts, const double* x, const double* v,
+                                     const double* mass);
+int         mpmx_num_scenes(const mpmx_ctx* ctx);
+/* counts, PCG state and FP64 conservation sums of one scene */
+mpmx_status mpmx_get_scene_stats(mpmx_ctx* ctx, int scene, mpmx_scene_stats* out);
 
 /* ---- stages of one substep, in order (ExplicitTimeIntegrator.cu:881-995) ---- */
 /* Particles::reorder / sort_by_offsets (Particle_kernal.cu:484-573): canonical
@@ -277,6 +311,14 @@ mpmx_status mpmx_time_run(mpmx_ctx* ctx, int nsteps, double dt, double* ms);
  * Query the particle count with x == NULL; returns particle count or -1. */
 int64_t mpmx_scene(int scene, uint64_t seed, double scale, mpmx_params* p, double* x, double* v,
                    double* mass, double* dt);
+/* batches of independent scenes: config 4 = 64 scenes of 10k-500k particles,
+ * 5 = 256 scenes of 5k-200k (log-uniform; cube / notched bar / sphere in turn,
+ * grids 64^3-256^3, E log-uniform in [1e4, 1e6]); `scale` scales the counts.
+ * *num_scenes: in, at most this many scenes when > 0; out, the count.  Size
+ * the arrays with null pointers first; returns the particle total or -1.
+ * params[s], counts[s]; x, v, mass hold the scenes one after another. */
+int64_t mpmx_scene_batch(int config, uint64_t seed, double scale, int* num_scenes, mpmx_params* params,
+                         int64_t* counts, double* x, double* v, double* mass, double* dt);
 
 #ifdef __cplusplus
 }

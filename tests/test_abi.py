@@ -65,3 +65,47 @@ def test_no_device_is_reported(mpmx):
         assert e.status == mpmx.ERR_NO_DEVICE
     else:
         raise AssertionError("context created without a device")
+
+
+def test_batch_scenes_are_seeded_and_sized(mpmx):
+    """Batch configs: 64 scenes of 10k-500k particles (config 4) and 256 of
+    5k-200k (config 5), log-uniform; grids 64^3-256^3; every scene inside its
+    own unit cube; deterministic per seed."""
+    for cfg, S, lo_n, hi_n in ((4, 64, 1e4, 5e5), (5, 256, 5e3, 2e5)):
+        ps, counts, x, v, m, dt = mpmx.scene_batch(cfg, seed=cfg)
+        assert len(ps) == S and len(counts) == S and counts.sum() == len(m)
+        assert counts.min() >= 0.99 * lo_n and counts.max() <= 1.01 * hi_n
+        assert {p.N for p in ps} <= {64, 128, 256} and len({p.N for p in ps}) == 3
+        Es = np.array([p.E for p in ps])
+        assert Es.min() >= 1e4 and Es.max() <= 1e6
+        start = np.concatenate([[0], np.cumsum(counts)])
+        for s, p in enumerate(ps):
+            xs = x[start[s]:start[s + 1]]
+            assert xs.min() >= 2.0 / p.N and xs.max() <= 1 - 2.0 / p.N
+            assert p.V0 > 0 and m[start[s]] > 0
+        ps2, counts2, x2, *_ = mpmx.scene_batch(cfg, seed=cfg)
+        assert np.array_equal(counts, counts2) and np.array_equal(x, x2)
+    ps, counts, *_ = mpmx.scene_batch(4, seed=4, scale=0.01, num_scenes=5)
+    assert len(ps) == 5 and counts.sum() < 30000
+
+
+def test_batch_create_validates_arguments(mpmx):
+    """Batch-wide settings must agree, and the keys must fit 32 bits; these
+    are refused before any device is touched."""
+    L = mpmx.lib()
+    import ctypes as C
+
+    def create(params, cfg=None):
+        arr = (mpmx.Params * len(params))(*params)
+        h = C.c_void_p()
+        c = cfg if cfg is not None else mpmx.default_config()
+        return L.mpmx_create_batch(len(params), arr, C.byref(c), C.byref(h))
+
+    a = mpmx.default_params(N=64)
+    b = mpmx.default_params(N=64, transfer=1)
+    assert create([a, b]) == mpmx.ERR_ARG
+    big = [mpmx.default_params(N=512) for _ in range(64)]   # 6 + 21 + 6 > 32
+    assert create(big) == mpmx.ERR_ARG
+    assert create([a, a], mpmx.default_config(p2g_variant=1)) == mpmx.ERR_ARG
+    if mpmx.device_count() == 0:
+        assert create([a, a]) == mpmx.ERR_NO_DEVICE
