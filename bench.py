@@ -18,6 +18,12 @@ rocprofv3 counter runs (FETCH_SIZE + WRITE_SIZE, each scaled by what the
 counter reads for a plain copy of known size), read from the committed counter
 summary of this workload; null for other workloads.
 
+`batched` reports the batch forms on the same GPU: the 256-scene batch
+(5k-200k particles per scene, ~13M in all) with mixed-precision state and the
+64-scene batch (10k-500k, ~9.8M) in FP32, each advanced as one context, and
+the 256 scenes run one at a time through the plain FP32 single-scene path
+(the speed-up of the batched mixed form over that).  `--no-batch` skips it.
+
 `--copy-probe MIB` runs only a plain device copy (out = 1 * in over MIB MiB of
 float32, three times): the calibration kernel of those counter runs.
 """
@@ -108,6 +114,44 @@ def copy_probe(mib):
                       "launches": 3}))
 
 
+def batch_run(mpmx, config, precision, steps, warmup):
+    """One batch config advanced as one context: particle-substeps/s."""
+    ps, counts, x, v, m, dt = mpmx.scene_batch(config, seed=config)
+    cfg = mpmx.default_config(precision=mpmx.MIXED if precision == "mixed" else mpmx.FP32)
+    ctx = mpmx.Context(ps, cfg)
+    ctx.set_particles_batch(counts, x, v, m)
+    ctx.run(warmup, dt)
+    ctx.sync()
+    ms = ctx.time_run(steps, dt)
+    st = ctx.stats()
+    n = int(counts.sum())
+    ctx.close()
+    return {"config": "%d scenes, %d particles (%d-%d per scene), grids %s, %s" % (
+                len(ps), n, counts.min(), counts.max(), "/".join("%d^3" % g for g in sorted({q.N for q in ps})),
+                precision),
+            "value": n * steps / (ms / 1e3), "unit": "particle-substeps/s", "ms_per_step": ms / steps,
+            "steps": steps, "warmup": warmup, "cg_iters_last": st["cg_iters"], "cg_breakdown": st["cg_breakdown"]}
+
+
+def scenes_one_at_a_time(mpmx, config, steps, warmup):
+    """The batch's scenes through the plain FP32 single-scene path, one
+    context after another: particle-substeps/s over the summed GPU time."""
+    ps, counts, x, v, m, dt = mpmx.scene_batch(config, seed=config)
+    start = np.concatenate([[0], np.cumsum(counts)])
+    total_ms, work = 0.0, 0
+    for s, p in enumerate(ps):
+        a, b = start[s], start[s + 1]
+        ctx = mpmx.Context(p)
+        ctx.set_particles(x[a:b], v[a:b], m[a:b])
+        ctx.run(warmup, dt)
+        ctx.sync()
+        total_ms += ctx.time_run(steps, dt)
+        work += int(b - a) * steps
+        ctx.close()
+    return {"value": work / (total_ms / 1e3), "unit": "particle-substeps/s", "scenes": len(ps),
+            "steps_per_scene": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +164,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=30)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -200,6 +246,13 @@ def main():
                      "p2g_stage_gbs": p2g_stage_gbs, "g2p_gbs": g2p_gbs},
         "stage_ms_per_step": {k: v[1] / max(args.profile_steps, 1) for k, v in kt.items()},
     }
+    if not args.no_batch:
+        ctx.close()
+        c5 = batch_run(mpmx, 5, "mixed", args.batch_steps, 5)
+        c4 = batch_run(mpmx, 4, "fp32", args.batch_steps, 5)
+        alone = scenes_one_at_a_time(mpmx, 5, 3, 2)
+        out["batched"] = {"mixed_256": c5, "fp32_64": c4, "plain_fp32_one_scene_at_a_time_256": alone,
+                          "speedup_mixed_batch_vs_one_at_a_time": c5["value"] / alone["value"]}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.scene, args.cpu_steps)
     print(json.dumps(out))
