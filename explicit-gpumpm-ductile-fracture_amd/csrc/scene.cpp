@@ -25,10 +25,13 @@
 // scenes) or the particle count (scene 1).  Density follows
 // rho = max(1000, (lambda + 2 mu) (dt / (0.3 dx))^2) so the elastic wave
 // crosses at most 0.3 cells per substep.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <functional>
+#include <string>
 #include <vector>
 
 #include "../../include/mpmx.h"
@@ -297,4 +300,117 @@ extern "C" int64_t mpmx_scene(int scene, uint64_t seed, double scale, mpmx_param
     if (mass)
         for (int64_t i = 0; i < n; ++i) mass[i] = rho * s.V0;
     return n;
+}
+
+// ---- scene files ----------------------------------------------------------
+// Point-list OBJ (Sample::readObj, LoadSample.cpp:180-194): every line
+// "v x y z" is one particle; the reference's loader moves it down by 0.2 in
+// y.  Other lines (comments, faces) are skipped.
+extern "C" int64_t mpmx_read_obj_points(const char* path, double y_shift, double* x, int64_t cap) {
+    if (!path) return -1;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return -1;
+    char line[512];
+    int64_t n = 0;
+    while (std::fgets(line, sizeof(line), f)) {
+        double a, b, c;
+        if (line[0] != 'v' || (line[1] != ' ' && line[1] != '\t')) continue;
+        if (std::sscanf(line + 1, "%lf %lf %lf", &a, &b, &c) != 3) continue;
+        if (x && n < cap) {
+            x[3 * n] = a;
+            x[3 * n + 1] = b + y_shift;
+            x[3 * n + 2] = c;
+        }
+        ++n;
+    }
+    std::fclose(f);
+    return n;
+}
+
+namespace {
+
+// level set on an ni x nj x nk lattice, i fastest (Sample::LoadSDF,
+// LoadSample.cpp:3-32: "ni nj nk", "minx miny minz", "dx", then the values)
+struct LevelSet {
+    int ni = 0, nj = 0, nk = 0;
+    double min[3] = {0, 0, 0}, dx = 0;
+    std::vector<double> phi;
+    double at(int i, int j, int k) const { return phi[(size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * k)]; }
+    // trilinear value at lattice coordinates (Sample::fetchGridTrilinear, :39-60)
+    double trilinear(double x, double y, double z) const {
+        const int x0 = (int)std::floor(x), y0 = (int)std::floor(y), z0 = (int)std::floor(z);
+        const int x1 = (int)std::ceil(x), y1 = (int)std::ceil(y), z1 = (int)std::ceil(z);
+        const double fx = x - x0, fy = y - y0, fz = z - z0;
+        const double c00 = at(x0, y0, z0) * (1 - fx) + at(x1, y0, z0) * fx;
+        const double c01 = at(x0, y0, z1) * (1 - fx) + at(x1, y0, z1) * fx;
+        const double c10 = at(x0, y1, z0) * (1 - fx) + at(x1, y1, z0) * fx;
+        const double c11 = at(x0, y1, z1) * (1 - fx) + at(x1, y1, z1) * fx;
+        const double c0 = c00 * (1 - fy) + c10 * fy, c1 = c01 * (1 - fy) + c11 * fy;
+        return c0 * (1 - fz) + c1 * fz;
+    }
+};
+
+bool load_level_set(const char* path, LevelSet& ls) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) return false;
+    bool ok = std::fscanf(f, "%d %d %d", &ls.ni, &ls.nj, &ls.nk) == 3 &&
+              std::fscanf(f, "%lf %lf %lf", &ls.min[0], &ls.min[1], &ls.min[2]) == 3 &&
+              std::fscanf(f, "%lf", &ls.dx) == 1 && ls.ni >= 4 && ls.nj >= 4 && ls.nk >= 4;
+    if (ok) {
+        ls.phi.resize((size_t)ls.ni * ls.nj * ls.nk);
+        for (auto& v : ls.phi)
+            if (std::fscanf(f, "%lf", &v) != 1) {
+                ok = false;
+                break;
+            }
+    }
+    std::fclose(f);
+    return ok;
+}
+
+}  // namespace
+
+// Sample::Initial_data + GenerateUniformSamples (LoadSample.cpp:62-160): the
+// level set's lattice (less one layer each side) is scaled uniformly into the
+// box res (cells) placed with its x and z centred on `center` and its low y at
+// center_y + res_y / 4, four cells in; samples_per_cell particles per grid cell
+// (the reference uses 20), i.e. validCells x samples_per_cell scale^3 in all,
+// where a valid lattice cell has a corner inside (phi < 0).  Samples are drawn
+// uniformly over the lattice and kept where the trilinear phi < 0, from a
+// SplitMix64 stream instead of rand().  Positions in domain units (dx = 1/N).
+// Returns the count (x may be null to size it) or -1.
+extern "C" int64_t mpmx_sample_sdf(const char* path, uint64_t seed, int N, const unsigned* center,
+                                   const unsigned* res, double samples_per_cell, double* x, int64_t cap) {
+    if (!path || !center || !res || N <= 0) return -1;
+    LevelSet ls;
+    if (!load_level_set(path, ls)) return -1;
+    int offset[3];
+    for (int a = 0; a < 3; ++a) offset[a] = (int)(center[a] - 0.5 * res[a]);
+    offset[1] = (int)(center[1] + 0.25 * res[1]);
+    const int lo = 1, hi[3] = {ls.ni - 2, ls.nj - 2, ls.nk - 2};
+    double scale = 1e300;
+    for (int a = 0; a < 3; ++a) scale = std::min(scale, (double)res[a] / (double)(hi[a] - lo));
+    const double per_lattice_cell = samples_per_cell * scale * scale * scale;
+    int64_t valid = 0;
+    for (int i = 0; i < ls.ni - 1; ++i)
+        for (int j = 0; j < ls.nj - 1; ++j)
+            for (int k = 0; k < ls.nk - 1; ++k) {
+                bool in = false;
+                for (int q = 0; q < 8 && !in; ++q) in = ls.at(i + (q >> 2), j + ((q >> 1) & 1), k + (q & 1)) < 0;
+                valid += in ? 1 : 0;
+            }
+    const int64_t count = (int64_t)((double)valid * per_lattice_cell);
+    if (!x) return count;
+    Rng rng{seed * 0x2545F4914F6CDD1Dull + 0x5DF5ull};
+    const double dxN = 1.0 / N;
+    for (int64_t s = 0; s < count && s < cap; ++s) {
+        double p[3];
+        do {
+            p[0] = rng.u01() * (ls.ni - 1);
+            p[1] = rng.u01() * (ls.nj - 1);
+            p[2] = rng.u01() * (ls.nk - 1);
+        } while (ls.trilinear(p[0], p[1], p[2]) >= 0);
+        for (int a = 0; a < 3; ++a) x[3 * s + a] = ((p[a] - lo) * scale + offset[a] + 4) * dxN;
+    }
+    return count;
 }

@@ -28,7 +28,8 @@ EXPORTS = [
     "mpmx_get_order", "mpmx_get_tables", "mpmx_get_grid", "mpmx_get_grid_p2g", "mpmx_get_stats",
     "mpmx_timing_begin", "mpmx_timing_end", "mpmx_time_run", "mpmx_scene", "mpmx_get_device_pointers",
     "mpmx_max_velocity_sq", "mpmx_create_batch", "mpmx_set_particles_batch", "mpmx_num_scenes",
-    "mpmx_get_scene_stats", "mpmx_scene_batch",
+    "mpmx_get_scene_stats", "mpmx_scene_batch", "mpmx_read_obj_points", "mpmx_sample_sdf",
+    "mpmx_write_particles", "mpmx_save_checkpoint", "mpmx_load_checkpoint",
 ]
 
 
@@ -144,6 +145,12 @@ def lib():
         "mpmx_get_scene_stats": ([vp, C.c_int, C.POINTER(SceneStats)], C.c_int),
         "mpmx_scene_batch": ([C.c_int, C.c_uint64, C.c_double, C.POINTER(C.c_int), vp, vp, vp, vp, vp,
                               C.POINTER(C.c_double)], C.c_int64),
+        "mpmx_read_obj_points": ([C.c_char_p, C.c_double, vp, C.c_int64], C.c_int64),
+        "mpmx_sample_sdf": ([C.c_char_p, C.c_uint64, C.c_int, C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.c_double,
+                             vp, C.c_int64], C.c_int64),
+        "mpmx_write_particles": ([vp, C.c_char_p], C.c_int),
+        "mpmx_save_checkpoint": ([vp, C.c_char_p], C.c_int),
+        "mpmx_load_checkpoint": ([C.c_char_p, C.c_int, C.POINTER(vp)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -210,6 +217,30 @@ def scene_batch(config, seed=1, scale=1.0, num_scenes=0):
     return list(ps), counts, x, v, m, dt.value
 
 
+def read_obj_points(path, y_shift=-0.2):
+    """Point-list OBJ as the reference's Sample::readObj loads it: x[n, 3]."""
+    L = lib()
+    n = L.mpmx_read_obj_points(path.encode(), y_shift, None, 0)
+    if n < 0:
+        raise OSError("cannot read %s" % path)
+    x = np.zeros((n, 3))
+    L.mpmx_read_obj_points(path.encode(), y_shift, x.ctypes.data, n)
+    return x
+
+
+def sample_sdf(path, N, center, res, samples_per_cell=20.0, seed=1):
+    """Particles inside a level-set file placed as Sample::Initial_data does: x[n, 3]."""
+    L = lib()
+    cen = (C.c_uint * 3)(*center)
+    rs = (C.c_uint * 3)(*res)
+    n = L.mpmx_sample_sdf(path.encode(), seed, N, cen, rs, samples_per_cell, None, 0)
+    if n < 0:
+        raise OSError("cannot read %s" % path)
+    x = np.zeros((n, 3))
+    L.mpmx_sample_sdf(path.encode(), seed, N, cen, rs, samples_per_cell, x.ctypes.data, n)
+    return x
+
+
 class Context:
     """One simulation context (device memory + HIP stream) over the C ABI.
     `params` is one Params (one scene) or a list of them (a batch of scenes)."""
@@ -252,6 +283,26 @@ class Context:
         mass = np.ascontiguousarray(mass, np.float64).reshape(-1)
         self.n = mass.shape[0]
         self._chk(self.L.mpmx_set_particles(self.h, self.n, x, v, mass))
+
+    @classmethod
+    def load_checkpoint(cls, path, device=0):
+        """A context restored from mpmx_save_checkpoint (continues bit-exactly)."""
+        self = cls.__new__(cls)
+        self.L = lib()
+        self.h = C.c_void_p()
+        st = self.L.mpmx_load_checkpoint(path.encode(), device, C.byref(self.h))
+        if st != OK:
+            raise MpmxError(st, self.L.mpmx_status_string(st).decode())
+        self.num_scenes = int(self.L.mpmx_num_scenes(self.h))
+        self.params = None
+        self.n = int(self.stats()["num_particles"])
+        return self
+
+    def save_checkpoint(self, path):
+        self._chk(self.L.mpmx_save_checkpoint(self.h, path.encode()))
+
+    def write_particles(self, path):
+        self._chk(self.L.mpmx_write_particles(self.h, path.encode()))
 
     def set_particles_batch(self, counts, x, v, mass):
         counts = np.ascontiguousarray(counts, np.int64)

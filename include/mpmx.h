@@ -320,6 +320,29 @@ int64_t mpmx_scene(int scene, uint64_t seed, double scale, mpmx_params* p, doubl
 int64_t mpmx_scene_batch(int config, uint64_t seed, double scale, int* num_scenes, mpmx_params* params,
                          int64_t* counts, double* x, double* v, double* mass, double* dt);
 
+/* ---- scene files and particle files (host; checkpoints also touch the GPU) ---- */
+/* Sample::readObj (LoadSample.cpp:180-194): a point-list OBJ, one particle per
+ * "v x y z" line, y moved by y_shift (the reference's loader uses -0.2).
+ * Returns the count (x may be NULL to size it) or -1. */
+int64_t mpmx_read_obj_points(const char* path, double y_shift, double* x, int64_t cap);
+/* Sample::LoadSDF + GenerateUniformSamples + Initial_data (LoadSample.cpp:3-160):
+ * particles uniform inside a level set (text: "ni nj nk", "minx miny minz",
+ * "dx", ni*nj*nk values, i fastest), scaled into the box res[3] (cells) at
+ * center[3] as the reference places it, samples_per_cell per grid cell (the
+ * reference uses 20); a seeded SplitMix64 stream replaces rand().  Positions
+ * in units of the N^3 domain.  Returns the count (x may be NULL) or -1. */
+int64_t mpmx_sample_sdf(const char* path, uint64_t seed, int N, const unsigned* center, const unsigned* res,
+                        double samples_per_cell, double* x, int64_t cap);
+/* MPMSimulator::writePartio (declared at Simulator.h:35, never defined in the
+ * reference): every particle in upload order as binary PLY: x y z vx vy vz c
+ * (float) and the scene index (int). */
+mpmx_status mpmx_write_particles(mpmx_ctx* ctx, const char* path);
+/* Checkpoint of the whole particle state and the context's parameters; a
+ * context loaded from it (on `device`) continues bit-exactly.  No reference
+ * counterpart (the reference keeps no checkpoints). */
+mpmx_status mpmx_save_checkpoint(mpmx_ctx* ctx, const char* path);
+mpmx_status mpmx_load_checkpoint(const char* path, int device, mpmx_ctx** out);
+
 #ifdef __cplusplus
 }
 #endif
