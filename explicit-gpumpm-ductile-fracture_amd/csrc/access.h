@@ -25,13 +25,14 @@ struct BlockTables {
     const int* page2particle;
     const uint32_t* keys;      // sorted particle keys
     const uint32_t* block_key;
+    const int* cell_start;     // [cap_p][65] first particle of each cell of each particle page
     int64_t cap_p, cap_tp, cap_blocks;
     int slot_shift;            // scene of a page key = key >> slot_shift
 };
 
 inline BlockTables block_tables_of(const mpmx_ctx* c) {
     return BlockTables{c->pmask, c->nmask, c->wcount, c->scount, c->page2particle, c->keys, c->block_key,
-                       c->cap_p, c->cap_tp, c->tp_bound, c->dp.slot_shift};
+                       c->cell_start, c->cap_p, c->cap_tp, c->tp_bound, c->dp.slot_shift};
 }
 
 // page key of page (px, py, pz) (non-negative) when that page lies in the
@@ -42,11 +43,15 @@ __device__ __forceinline__ uint32_t scene_page_key(const BlockTables& t, int px,
 }
 
 // ---- shared pieces of the block gathers ---------------------------------------
-// The particle pages 2b-1 .. 2b+1 (per axis) of block b, their particle ranges
-// and the cell filter that keeps the particles reaching the block; lanes 32..39
-// resolve the block's 8 grid pages.  Pages of another scene's slot are not
-// sources.  Called by one wavefront (the first of the workgroup); LDS arrays
-// owned by the caller.
+// A block gathers from the particle pages 2b-1 .. 2b+1 (per axis) of block b:
+// its own 8 pages, whose particles all reach it, and the 19 pages just below
+// it, whose particles reach it only from cells with index 2 or 3 along each
+// axis on which the page is below.
+
+// The 27 source pages in slot order with their whole particle ranges and the
+// cell filter per page, for gathers whose per-entry work is small (K s):
+// one wave resolves the pages, every thread walks a run of the concatenated
+// ranges and skips the particles that do not reach the block.
 struct BlockSources {
     int start[27], pre[28], filter[27];
     int page[27];          // particle page id of each source, or -1
@@ -112,6 +117,216 @@ __device__ __forceinline__ bool block_entry_simple(const BlockTables& t, const B
              ((f & 4) && !((cell >> 1) & 1u)));
 }
 
+// ---- class-ordered sources -----------------------------------------------------
+// The same 27 source pages, grouped by class: bit 2 / 1 / 0 of the class set
+// when the page lies below the block along x / y / z.  Only the particles in
+// cells with index 2 or 3 along each such axis reach the block, and their
+// base node lies one or two nodes below it, so a class knows at compile time
+// which node offsets can fall outside the block.  Each source page is listed
+// as runs of its passing cells (cell = i<<4 | j<<2 | k, i, j, k in 0..3), with
+// the particle ranges from the page's cell starts, so every listed particle
+// reaches the block: a workgroup runs one class at a time over exactly its
+// particles, and every wave of a class phase takes the same branches.
+// class:             0  1(z) 2(y) 3(yz) 4(x) 5(xz) 6(xy) 7(xyz)
+// pages:             8  4    4    2     4    2     2     1
+// runs per page:     1  16   4    8     1    8     2     4
+constexpr int kClassFirstSlot[9] = {0, 8, 12, 16, 18, 22, 24, 26, 27};
+constexpr int kClassRuns[8] = {1, 16, 4, 8, 1, 8, 2, 4};
+constexpr int kMaxRuns = 132;
+
+__host__ __device__ constexpr int class_first_run(int cls) {
+    int r = 0;
+    for (int c = 0; c < cls; ++c) r += (kClassFirstSlot[c + 1] - kClassFirstSlot[c]) * kClassRuns[c];
+    return r;
+}
+
+struct BlockClasses {
+    int start[kMaxRuns];     // first particle of run r
+    int pre[kMaxRuns + 1];   // particles listed before run r (all classes, in order)
+    int page[27];            // particle page id of source slot s, or -1 (slot order)
+    int out_page[8];
+    int thr0[9];             // threads [thr0[c], thr0[c + 1]) work on class c
+    int len[8];              // entries per thread of class c
+};
+
+// class and page offset (per axis, -1 below / 0 / 1) of source slot s
+__device__ __forceinline__ void source_slot(int s, int& cls, int (&d)[3]) {
+    cls = 0;
+#pragma unroll
+    for (int c = 1; c < 8; ++c) cls += s >= kClassFirstSlot[c] ? 1 : 0;
+    const int q = s - kClassFirstSlot[cls];
+    int bit = 0;
+#pragma unroll
+    for (int a = 2; a >= 0; --a) {
+        if ((cls >> (2 - a)) & 1) {
+            d[a] = -1;
+        } else {
+            d[a] = (q >> bit) & 1;
+            ++bit;
+        }
+    }
+}
+
+// run r of a page of class cls: cells [c0, c0 + len)
+__device__ __forceinline__ void class_run(int cls, int r, int& c0, int& len) {
+    const int fm = ((cls & 4) ? 32 : 0) | ((cls & 2) ? 8 : 0) | ((cls & 1) ? 2 : 0);
+    if (!fm) {
+        c0 = 0;
+        len = 64;
+        return;
+    }
+    const int low = __builtin_ctz(fm);
+    len = 1 << low;
+    const int free_bits = (0x3f & ~((2 << low) - 1)) & ~fm;
+    int dep = 0, src = 0;
+    for (int b = 0; b < 6; ++b)
+        if ((free_bits >> b) & 1) {
+            dep |= ((r >> src) & 1) << b;
+            ++src;
+        }
+    c0 = fm | dep;
+}
+
+// Three steps, with a workgroup barrier after each: block_pages (first
+// wave: lanes 0..26 one source slot each, lanes 32..39 the block's 8 grid
+// pages), block_cell_rows (every thread: the source pages' cell starts into
+// LDS, coalesced), block_runs (first wave: the runs, their prefix and the
+// threads of each class).  LDS owned by the caller.
+constexpr int kCellRows = 27 * 65;
+
+__device__ __forceinline__ void block_pages(const BlockTables& t, uint32_t bkey, const int (&bc)[3], int n,
+                                            int np_raw, int np, int nt, BlockClasses& S) {
+    const int lane = threadIdx.x;
+    const int scene = (int)((bkey << 3) >> t.slot_shift);
+    if (lane < 27) {
+        int cls, d[3];
+        source_slot(lane, cls, d);
+        int pg = -1;
+        const int px = 2 * bc[0] + d[0], py = 2 * bc[1] + d[1], pz = 2 * bc[2] + d[2];
+        const uint32_t key = (px >= 0 && py >= 0 && pz >= 0) ? scene_page_key(t, px, py, pz, scene) : ~0u;
+        if (key != ~0u) {
+            const int id = page_rank(key, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
+            if (id >= 0 && id < np) {
+                int a = t.page2particle[id], b = t.page2particle[id + 1];
+                a = a < 0 ? 0 : (a > n ? n : a);
+                b = b < a ? a : (b > n ? n : b);
+                if (b > a) pg = id;
+            }
+        }
+        S.page[lane] = pg;
+    } else if (lane >= 32 && lane < 40) {
+        const int o = lane - 32;
+        const int id = page_rank((bkey << 3) | (uint32_t)o, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
+        S.out_page[o] = (id >= 0 && id < nt) ? id : -1;
+    }
+}
+
+// rows[s * 65 + c]: first particle of cell c of source slot s (zeros for an
+// absent page), clamped to [0, n]
+__device__ __forceinline__ void block_cell_rows(const BlockTables& t, const BlockClasses& S, int n, int* rows) {
+    for (int q = threadIdx.x; q < kCellRows; q += blockDim.x) {
+        const int sl = q / 65, c = q - 65 * sl;
+        const int pg = S.page[sl];
+        int v = 0;
+        if (pg >= 0) {
+            v = t.cell_start[(int64_t)pg * 65 + c];
+            v = v < 0 ? 0 : (v > n ? n : v);
+        }
+        rows[q] = v;
+    }
+}
+
+__device__ __forceinline__ void block_runs(BlockClasses& S, const int* rows) {
+    const int lane = threadIdx.x;
+    int total = 0, r0 = 0, nr = 0, cls = 0;
+    if (lane < 27) {
+        int d[3];
+        source_slot(lane, cls, d);
+        nr = kClassRuns[cls];
+        r0 = class_first_run(cls) + (lane - kClassFirstSlot[cls]) * nr;
+        const int* row = rows + 65 * lane;
+        for (int r = 0; r < nr; ++r) {
+            int c0, len;
+            class_run(cls, r, c0, len);
+            const int st = row[c0], e = row[c0 + len];
+            const int cnt = e > st ? e - st : 0;
+            S.start[r0 + r] = st;
+            S.pre[r0 + r] = cnt;   // counts for now, prefix below
+            total += cnt;
+        }
+    }
+    // slots are in class order, so the exclusive prefix over lanes 0..26 is
+    // the prefix over runs
+    int incl = total;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        const int up = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += up;
+    }
+    if (lane < 27) {
+        int acc = incl - total;
+        for (int r = 0; r < nr; ++r) {
+            const int cnt = S.pre[r0 + r];
+            S.pre[r0 + r] = acc;
+            acc += cnt;
+        }
+    }
+    if (lane == 26) S.pre[kMaxRuns] = incl;
+    // threads per class in proportion to its entries (one at least), so that
+    // every thread takes about total / 256 entries of a single class and
+    // waves stay within one class except where two classes meet
+    const int all = __shfl(incl, 26, 64);
+    const int excl = incl - total;   // lanes 27.. hold `all`
+    const int c = lane < 8 ? lane : 0;
+    const int n_c = __shfl(excl, kClassFirstSlot[c + 1], 64) - __shfl(excl, kClassFirstSlot[c], 64);
+    int thr = 0;
+    if (lane < 8 && n_c > 0) thr = (int)((248ll * n_c) / all) + 1;
+    int tin = thr;
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        const int up = __shfl_up(tin, o, 64);
+        if (lane >= o) tin += up;
+    }
+    if (lane < 8) {
+        S.thr0[lane] = tin - thr;
+        S.len[lane] = thr > 0 ? (n_c + thr - 1) / thr : 0;
+    }
+    if (lane == 7) S.thr0[8] = tin;
+}
+
+// particles listed for class cls, and entry j of them
+__device__ __forceinline__ int class_count(const BlockClasses& S, int cls) {
+    return S.pre[cls == 7 ? kMaxRuns : class_first_run(cls + 1)] - S.pre[class_first_run(cls)];
+}
+__device__ __forceinline__ int class_entry(const BlockClasses& S, int cls, int j) {
+    const int lo0 = class_first_run(cls);
+    const int e = S.pre[lo0] + j;
+    int lo = lo0, hi = lo0 + (kClassFirstSlot[cls + 1] - kClassFirstSlot[cls]) * kClassRuns[cls] - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S.pre[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return S.start[lo] + (e - S.pre[lo]);
+}
+
+// the class of the calling thread and its entries [jb, je) of that class;
+// false for threads without entries
+__device__ __forceinline__ bool thread_class(const BlockClasses& S, int& cls, int& jb, int& je) {
+    const int t = (int)threadIdx.x;
+    cls = -1;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        if (t >= S.thr0[c] && t < S.thr0[c + 1]) cls = c;
+    if (cls < 0) return false;
+    const int len = S.len[cls];
+    jb = (t - S.thr0[cls]) * len;
+    je = jb + len;
+    const int n = class_count(S, cls);
+    je = je < n ? je : n;
+    return jb < je;
+}
+
 // ---- fixed-point accumulation -----------------------------------------------
 // LDS float atomics cost ~190 CU cycles per wave instruction on gfx950 and
 // 64-bit integer atomics ~7, so the block gathers add in fixed point: a
@@ -134,6 +349,10 @@ __device__ __forceinline__ int fix_exp(float b) {
 // x scaled by sc = 2^e (|x sc| <= 2^30), truncated, sign-extended to 64 bits
 __device__ __forceinline__ unsigned long long to_fix(float x, float sc) {
     return (unsigned long long)(long long)(int)(x * sc);
+}
+// a value already scaled, truncated, sign-extended
+__device__ __forceinline__ unsigned long long to_fix_scaled(float xs) {
+    return (unsigned long long)(long long)(int)xs;
 }
 __device__ __forceinline__ float from_fix(unsigned long long v, int e) {
     return (float)ldexp((double)(long long)v, -e);

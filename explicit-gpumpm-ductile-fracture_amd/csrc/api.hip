@@ -335,7 +335,7 @@ static void free_particles(mpmx_ctx* c) {
     dfree(c->sigma); dfree(c->FP); dfree(c->vol); dfree(c->cg_col); dfree(c->payload); dfree(c->pbound);
     dfree(c->keys); dfree(c->keys_alt); dfree(c->vals); dfree(c->vals_alt); dfree(c->hist);
     dfree(c->pmask); dfree(c->nmask); dfree(c->wcount); dfree(c->scount);
-    dfree(c->page_key); dfree(c->page2particle); dfree(c->adj); dfree(c->nvp); dfree(c->vpo); dfree(c->target); dfree(c->block_key); dfree(c->bcount);
+    dfree(c->page_key); dfree(c->page2particle); dfree(c->adj); dfree(c->nvp); dfree(c->vpo); dfree(c->target); dfree(c->block_key); dfree(c->bcount); dfree(c->cell_start);
     dfree(c->grid); dfree(c->grid_p2g);
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q); dfree(c->cg_part);
     dfree(c->scan_tmp);
@@ -658,6 +658,7 @@ mpmx_status mpmx_set_particles_batch(mpmx_ctx* c, const int64_t* counts, const d
     HIPCHK(c, dalloc(&c->pbound, 8 * (c->cap_p + 1)));
     HIPCHK(c, dalloc(&c->bcount, c->mask_words));
     HIPCHK(c, dalloc(&c->page2particle, c->cap_p + 1));
+    HIPCHK(c, dalloc(&c->cell_start, 65 * c->cap_p));
     HIPCHK(c, dalloc(&c->adj, 7 * c->cap_p));
     HIPCHK(c, dalloc(&c->nvp, c->cap_p + 1));
     HIPCHK(c, dalloc(&c->vpo, c->cap_p + 1));

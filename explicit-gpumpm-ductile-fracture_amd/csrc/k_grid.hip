@@ -153,9 +153,34 @@ __device__ __forceinline__ void p2g_bounds(const P2GTerms& T, float (&b)[kP2GCh]
     b[7] = fabsf(T.VF);
 }
 
-// add the particle's contributions to the block's 8^3 nodes in fixed point
+// the particle's terms times the channel scales (powers of two, so every
+// product below is the unscaled product times the scale, exactly)
 template <int TRANSFER>
-__device__ __forceinline__ void p2g_fix(const P2GTerms& T, const float (&sc)[kP2GCh], unsigned long long* acc) {
+__device__ __forceinline__ void scale_terms(P2GTerms& T, const float (&sc)[kP2GCh]) {
+    T.m *= sc[0];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        T.ar[r] *= sc[1 + r];
+        T.Qd[r] *= sc[1 + r];
+        T.Qd[r + 3] *= sc[1 + r];
+        T.Qd[r + 6] *= sc[1 + r];
+        if (TRANSFER == 1) {
+            T.sg[r] *= sc[1 + r];
+            T.sg[r + 3] *= sc[1 + r];
+            T.sg[r + 6] *= sc[1 + r];
+        }
+    }
+    T.cphase *= sc[4];
+    T.V *= sc[6];
+    T.VF *= sc[7];
+}
+
+// add the particle's contributions (terms scaled by scale_terms, sc5 the
+// scale of the weight channel) to the block's 8^3 nodes in fixed point.
+// Nodes outside the block are skipped; a wave whose lanes all hold particles
+// of one source class (see BlockClasses) skips them together.
+template <int TRANSFER>
+__device__ __forceinline__ void p2g_fix(const P2GTerms& T, float sc5, unsigned long long* acc) {
 #pragma unroll
     for (int ii = 0; ii < 3; ++ii) {
         const int la = T.l0[0] + ii;
@@ -182,9 +207,9 @@ __device__ __forceinline__ void p2g_fix(const P2GTerms& T, const float (&sc)[kP2
                     for (int r = 0; r < 3; ++r) pr[r] -= T.sg[r] * g0 + T.sg[r + 3] * g1 + T.sg[r + 6] * g2;
                 }
                 unsigned long long* a = acc + (la * 8 + lb) * kAccZ + lc;
-                const float add[kP2GCh] = {wt * T.m, pr[0], pr[1], pr[2], wt * T.cphase, wt, wt * T.V, wt * T.VF};
+                const float add[kP2GCh] = {wt * T.m, pr[0], pr[1], pr[2], wt * T.cphase, wt * sc5, wt * T.V, wt * T.VF};
 #pragma unroll
-                for (int k = 0; k < kP2GCh; ++k) atomicAdd(a + k * kAccStride, to_fix(add[k], sc[k]));
+                for (int k = 0; k < kP2GCh; ++k) atomicAdd(a + k * kAccStride, to_fix_scaled(add[k]));
             }
         }
     }
@@ -298,13 +323,30 @@ __global__ __launch_bounds__(256) void k_p2g_prep(DevParams dp, ST s, WaveMap wm
     }
 }
 
+// the calling thread's run [jb, je) of the entries of one class of the
+// block's sources: the particles of one cell (adjacent after the sort) go
+// through one lane in turn instead of hitting the same nodes in one
+// instruction
+template <int TRANSFER>
+__device__ __forceinline__ void p2g_class(const BlockClasses& S, const BlockTables& t, const float* __restrict__ pay,
+                                          const int (&org)[3], const float (&sc)[kP2GCh], unsigned long long* acc,
+                                          int cls, int jb, int je) {
+    for (int j = jb; j < je; ++j) {
+        const int i = class_entry(S, cls, j);
+        P2GTerms T;
+        payload_load<TRANSFER>(pay, i, t.keys[i], org, T);
+        scale_terms<TRANSFER>(T, sc);
+        p2g_fix<TRANSFER>(T, sc[5], acc);
+    }
+}
+
 template <int TRANSFER, typename ST>
 __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float* __restrict__ pay,
                                                    const unsigned* __restrict__ pbound, BlockTables t,
                                                    float* __restrict__ grid, float* __restrict__ grid_raw, CGVec cg,
                                                    float dt, int n, const DevCounts* __restrict__ dc) {
     __shared__ unsigned long long acc[kP2GCh * kAccStride];
-    __shared__ BlockSources S;
+    __shared__ BlockClasses S;
     __shared__ int sexp[kP2GCh];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int np_raw = dc->num_page;
@@ -318,13 +360,15 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
         const DevParams sp = dp.scenes[__builtin_amdgcn_readfirstlane(scene_of_page(dp, bkey << 3))];
-        if (wave == 0) block_sources(t, bkey, bc, n, np_raw, np, nt, S);
-        for (int q = tid; q < kP2GCh * kAccStride; q += 256) acc[q] = 0ull;
+        if (wave == 0) block_pages(t, bkey, bc, n, np_raw, np, nt, S);
         __syncthreads();
-        const int total = S.pre[27];
+        int* rows = reinterpret_cast<int*>(acc);   // cell starts, before the sums use the space
+        block_cell_rows(t, S, n, rows);
+        __syncthreads();
+        if (wave == 0) block_runs(S, rows);
         const int org[3] = {8 * bc[0], 8 * bc[1], 8 * bc[2]};
         // scales: the largest contribution per channel of the 27 source pages
-        if (wave == 0) {
+        if (wave == 1) {
             float bmax[kP2GCh];
 #pragma unroll
             for (int k = 0; k < kP2GCh; ++k) {
@@ -339,6 +383,8 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
             }
         }
         __syncthreads();
+        for (int q = tid; q < kP2GCh * kAccStride; q += 256) acc[q] = 0ull;
+        __syncthreads();
         int e[kP2GCh];
         float sc[kP2GCh];
 #pragma unroll
@@ -346,18 +392,8 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
             e[k] = sexp[k];
             sc[k] = ldexpf(1.0f, e[k]);
         }
-        // thread tid takes the run of entries [tid per, (tid + 1) per): the
-        // particles of one cell (adjacent after the sort) go through one lane
-        // in turn instead of hitting the same nodes in one instruction
-        const int per = (total + 255) / 256;
-        for (int k = 0; k < per; ++k) {
-            const int j = tid * per + k;
-            int i;
-            if (!block_entry_simple(t, S, j, total, i)) continue;
-            P2GTerms T;
-            payload_load<TRANSFER>(pay, i, t.keys[i], org, T);
-            p2g_fix<TRANSFER>(T, sc, acc);
-        }
+        int cls, jb, je;
+        if (thread_class(S, cls, jb, je)) p2g_class<TRANSFER>(S, t, pay, org, sc, acc, cls, jb, je);
         __syncthreads();
         // grid update of the block's pages, written page-major (coalesced)
         for (int q = tid; q < 512; q += 256) {

@@ -121,18 +121,38 @@ __global__ void k_page_keys(const uint64_t* __restrict__ pmask, const uint64_t* 
 }
 
 // markBlockOffset on page boundaries (Transformer_kernal.cu:22-36): first particle of each page
+// first particle of every cell of every particle page (cell2particle of
+// Transformer_kernal.cu:15-36, per page): cell_start[p * 65 + c] for c in
+// [0, 64], the first particle of the page with cell >= c (the page's end for
+// c = 64).  Particles of a page are sorted by cell, so particle i is that
+// first particle for the cells after its predecessor's cell up to its own;
+// the page's last particle also fills the cells after its own with the end.
+// Every entry is written once, without atomics.
+// The same pass writes page2particle (the first particle of each page,
+// markPageBoundary + the page head scan of Transformer_kernal.cu:38-47).
 __global__ void k_page_heads(const uint32_t* __restrict__ keys, int n, const uint64_t* __restrict__ pmask,
                              const uint64_t* __restrict__ nmask, const int* __restrict__ wpre,
-                             const int* __restrict__ spre, int* page2particle, int64_t cap_p,
-                             const DevCounts* dc) {
+                             const int* __restrict__ spre, int* __restrict__ page2particle,
+                             int* __restrict__ cell_start, int64_t cap_p, const DevCounts* __restrict__ dc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int num_page = dc->num_page;
     if (i == n - 1 && num_page <= cap_p) page2particle[num_page] = n;
-    const uint32_t pk = keys[i] >> 6;
-    if (i > 0 && (keys[i - 1] >> 6) == pk) return;
+    const uint32_t k = keys[i], pk = k >> 6;
+    const int cell = (int)(k & 63u);
+    const bool first = i == 0 || (keys[i - 1] >> 6) != pk;
+    const bool last = i == n - 1 || (keys[i + 1] >> 6) != pk;
+    const int prev = first ? -1 : (int)(keys[i - 1] & 63u);
+    if (prev == cell && !last) return;
+    const int np = num_page < cap_p ? num_page : (int)cap_p;
     const int id = page_rank(pk, pmask, nmask, wpre, spre, num_page);
-    if (id >= 0 && id < cap_p) page2particle[id] = i;
+    if (id < 0 || id >= cap_p) return;
+    if (first) page2particle[id] = i;
+    if (id >= np) return;
+    int* cs = cell_start + (int64_t)id * 65;
+    for (int c = prev + 1; c <= cell; ++c) cs[c] = i;
+    if (last)
+        for (int c = cell + 1; c <= 64; ++c) cs[c] = i + 1;
 }
 
 // establishPageTopology + markPageSize (Transformer_kernal.cu:98-120)
@@ -237,8 +257,8 @@ hipError_t build_tables(mpmx_ctx* c) {
     k_page_keys<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pmask, c->nmask, c->wcount, c->scount,
                                                                     words, c->page_key, c->cap_p, c->cap_tp,
                                                                     c->dc);
-    k_page_heads<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask, c->nmask, c->wcount,
-                                                          c->scount, c->page2particle, c->cap_p, c->dc);
+    k_page_heads<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask, c->nmask, c->wcount, c->scount,
+                                                          c->page2particle, c->cell_start, c->cap_p, c->dc);
     const int pblocks = (int)((c->cap_p + 255) / 256);
     k_page_topology<<<pblocks, 256, 0, c->stream>>>(c->page_key, c->page2particle, c->pmask, c->nmask,
                                                      c->wcount, c->scount, c->adj, c->nvp, c->cap_p,

@@ -209,6 +209,7 @@ struct mpmx_ctx {
     int64_t cap_p = 0, cap_tp = 0, cap_vp = 0;
     int64_t tp_bound = 0;    // total pages the domain can hold
     uint32_t* block_key = nullptr;   // [tp_bound] occupied 2x2x2-page blocks, Morton order
+    int* cell_start = nullptr;       // [cap_p][65] first particle of each cell of each particle page
     int* bcount = nullptr;           // [mask_words] per-word block counts, then their scan
     // grid + PCG
     float* grid = nullptr;
