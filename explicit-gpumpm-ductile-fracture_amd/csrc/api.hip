@@ -1012,6 +1012,7 @@ mpmx_status mpmx_get_tables(mpmx_ctx* c, uint64_t* page_key, int32_t* page2parti
                             int32_t* target, int32_t* particle2cell, int32_t* cell2particle, int32_t counts[4]) {
     if (!c || c->n == 0) return MPMX_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
+    if (!c->vpages_valid) HIPCHK(c, build_vpage_tables(c));
     mpmx_status s = read_counts(c);
     if (s != MPMX_OK) return s;
     const int np = std::min<int64_t>(c->hc->num_page, c->cap_p);
@@ -1124,6 +1125,7 @@ mpmx_status mpmx_get_stats(mpmx_ctx* c, mpmx_stats* out) {
         out->momentum[2] = h[3];
         out->kinetic_energy = h[4];
     }
+    if (c->n > 0 && !c->vpages_valid) HIPCHK(c, build_vpage_tables(c));
     mpmx_status s = read_counts(c);
     if (s != MPMX_OK) return s;
     out->num_page = c->hc->num_page;

@@ -45,11 +45,14 @@ hipError_t fill_zero_dev_count(mpmx_ctx* c, float* p, const int* count_dev, int6
 }
 
 // ---------------------------------------------------------------- scan
-// Exclusive scan of int32 in three phases: 1024-item tile scans, a single-block
-// scan of the tile sums, and the tile offsets added back.  The item count comes
-// either from the host (n_dev == nullptr) or from device memory, so the same
-// launches serve tables whose sizes are only known on the device.
-constexpr int kScanTile = 1024;
+// Exclusive scan of int32 in two launches: tile sums, then every tile adds up
+// the sums of the tiles before it (at most a few hundred) and scans its own
+// items from that offset; the last tile writes the total.  The item count
+// comes either from the host (n_dev == nullptr) or from device memory, so the
+// same launches serve tables whose sizes are only known on the device.
+// In-place scans (in == out) are allowed.
+constexpr int kScanPer = 4;
+constexpr int kScanTile = 256 * kScanPer;
 
 __device__ __forceinline__ int wave_incl_scan(int v) {
     const int lane = threadIdx.x & 63;
@@ -79,71 +82,53 @@ __device__ __forceinline__ int block_excl_scan256(int v, int& total, int* lds4) 
     return off + inc - v;
 }
 
-__global__ __launch_bounds__(256) void k_scan_tiles(const int* in, int* out, int* sums, const int* n_dev,
-                                                    int64_t n_host) {
-    __shared__ int lds4[4];
+__device__ __forceinline__ int64_t scan_count(const int* n_dev, int64_t n_host) {
     int64_t n = n_dev ? (int64_t)*n_dev : n_host;
-    n = n < n_host ? n : n_host;   // n_host is the capacity when n_dev is given
+    return n < n_host ? n : n_host;   // n_host is the capacity when n_dev is given
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const int* in, int* sums, const int* n_dev, int64_t n_host) {
+    __shared__ int lds4[4];
+    const int64_t n = scan_count(n_dev, n_host);
     const int64_t base = (int64_t)blockIdx.x * kScanTile;
-    int v[4];
     int local = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int64_t i = base + threadIdx.x * 4 + q;
+    for (int q = 0; q < kScanPer; ++q) {
+        const int64_t i = base + q * 256 + threadIdx.x;   // coalesced
+        local += (i < n) ? in[i] : 0;
+    }
+    int total;
+    (void)block_excl_scan256(local, total, lds4);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const int* in, int* out, const int* __restrict__ sums,
+                                                    const int* n_dev, int64_t n_host, int* total_dev) {
+    __shared__ int lds4[4];
+    const int64_t n = scan_count(n_dev, n_host);
+    // offset of this tile: the sums of the tiles before it
+    int pre = 0;
+    for (int t = threadIdx.x; t < (int)blockIdx.x; t += 256) pre += sums[t];
+    int offset;
+    (void)block_excl_scan256(pre, offset, lds4);
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPer;
+    int v[kScanPer];
+    int local = 0;
+#pragma unroll
+    for (int q = 0; q < kScanPer; ++q) {
+        const int64_t i = base + q;
         v[q] = (i < n) ? in[i] : 0;
         local += v[q];
     }
     int total;
-    int off = block_excl_scan256(local, total, lds4);
+    int off = block_excl_scan256(local, total, lds4) + offset;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int64_t i = base + threadIdx.x * 4 + q;
+    for (int q = 0; q < kScanPer; ++q) {
+        const int64_t i = base + q;
         if (i < n) out[i] = off;
         off += v[q];
     }
-    if (threadIdx.x == 0) sums[blockIdx.x] = total;
-}
-
-// single block: exclusive scan of tiles sums in place, total to *total_dev
-__global__ __launch_bounds__(256) void k_scan_sums(int* sums, int ntiles, int* total_dev) {
-    __shared__ int lds4[4];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < ntiles; base += 1024) {
-        int v[4];
-        int local = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int i = base + threadIdx.x * 4 + q;
-            v[q] = i < ntiles ? sums[i] : 0;
-            local += v[q];
-        }
-        int total;
-        int off = block_excl_scan256(local, total, lds4) + carry;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int i = base + threadIdx.x * 4 + q;
-            if (i < ntiles) sums[i] = off;
-            off += v[q];
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) carry += total;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0 && total_dev) *total_dev = carry;
-}
-
-__global__ __launch_bounds__(256) void k_scan_add(int* out, const int* sums, const int* n_dev, int64_t n_host) {
-    int64_t n = n_dev ? (int64_t)*n_dev : n_host;
-    n = n < n_host ? n : n_host;
-    const int add = sums[blockIdx.x];
-    const int64_t base = (int64_t)blockIdx.x * kScanTile;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int64_t i = base + threadIdx.x * 4 + q;
-        if (i < n) out[i] += add;
-    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0 && total_dev) *total_dev = offset + total;
 }
 
 hipError_t scan_exclusive(mpmx_ctx* c, const int* in, int* out, int64_t n_cap, const int* n_dev,
@@ -153,9 +138,8 @@ hipError_t scan_exclusive(mpmx_ctx* c, const int* in, int* out, int64_t n_cap, c
     if (tiles > c->scan_tmp_cap) return hipErrorInvalidValue;
     // with a device count, n_host carries the capacity bound
     const int64_t bound = n_dev ? n_cap : n_host;
-    k_scan_tiles<<<(int)tiles, 256, 0, c->stream>>>(in, out, c->scan_tmp, n_dev, bound);
-    k_scan_sums<<<1, 256, 0, c->stream>>>(c->scan_tmp, (int)tiles, total_dev);
-    k_scan_add<<<(int)tiles, 256, 0, c->stream>>>(out, c->scan_tmp, n_dev, bound);
+    k_scan_reduce<<<(int)tiles, 256, 0, c->stream>>>(in, c->scan_tmp, n_dev, bound);
+    k_scan_apply<<<(int)tiles, 256, 0, c->stream>>>(in, out, c->scan_tmp, n_dev, bound, total_dev);
     return hipGetLastError();
 }
 

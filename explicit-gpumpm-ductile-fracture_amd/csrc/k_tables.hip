@@ -167,9 +167,9 @@ __global__ void k_page_topology(const uint32_t* __restrict__ page_key, const int
     const uint32_t pk = page_key[p];
 #pragma unroll
     for (int q = 0; q < 7; ++q) {
-        const int id = page_rank(neighbor_key(pk, q), pmask, nmask, wpre, spre, dc->num_page);
-        if (id < 0) atomicOr(&dc->err, kErrCapacity);
-        adj[(int64_t)q * cap_p + p] = id < 0 ? 0 : id;
+        // -1 only past the edge of a batched scene's page space, which no
+        // node of the scene reaches
+        adj[(int64_t)q * cap_p + p] = page_rank(neighbor_key(pk, q), pmask, nmask, wpre, spre, dc->num_page);
     }
     const int cnt = page2particle[p + 1] - page2particle[p];
     nvp[p] = (cnt + vpage - 1) / vpage;
@@ -240,6 +240,23 @@ hipError_t launch_scene_counts(mpmx_ctx* c) {
     return hipGetLastError();
 }
 
+// page adjacency and virtual pages (establishPageTopology, markPageSize,
+// markVirtualPageOffset and the target pages, Transformer_kernal.cu:98-126):
+// the page-tile transfer kernels (p2g_variant 1) use them; the block kernels
+// do not, so for those they are built only when asked for
+hipError_t build_vpage_tables(mpmx_ctx* c) {
+    hipError_t e;
+    const int pblocks = (int)((c->cap_p + 255) / 256);
+    k_page_topology<<<pblocks, 256, 0, c->stream>>>(c->page_key, c->page2particle, c->pmask, c->nmask,
+                                                     c->wcount, c->scount, c->adj, c->nvp, c->cap_p,
+                                                     c->P.vpage, c->dc);
+    if ((e = scan_exclusive(c, c->nvp, c->vpo, c->cap_p, &c->dc->num_page, 0, &c->dc->num_vp)) != hipSuccess)
+        return e;
+    k_vtargets<<<pblocks, 256, 0, c->stream>>>(c->vpo, c->target, c->cap_p, c->cap_vp, c->dc);
+    c->vpages_valid = true;
+    return hipGetLastError();
+}
+
 hipError_t build_tables(mpmx_ctx* c) {
     const int n = (int)c->n;
     const int64_t words = c->mask_words;
@@ -259,13 +276,8 @@ hipError_t build_tables(mpmx_ctx* c) {
                                                                     c->dc);
     k_page_heads<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask, c->nmask, c->wcount, c->scount,
                                                           c->page2particle, c->cell_start, c->cap_p, c->dc);
-    const int pblocks = (int)((c->cap_p + 255) / 256);
-    k_page_topology<<<pblocks, 256, 0, c->stream>>>(c->page_key, c->page2particle, c->pmask, c->nmask,
-                                                     c->wcount, c->scount, c->adj, c->nvp, c->cap_p,
-                                                     c->P.vpage, c->dc);
-    if ((e = scan_exclusive(c, c->nvp, c->vpo, c->cap_p, &c->dc->num_page, 0, &c->dc->num_vp)) != hipSuccess)
-        return e;
-    k_vtargets<<<pblocks, 256, 0, c->stream>>>(c->vpo, c->target, c->cap_p, c->cap_vp, c->dc);
+    c->vpages_valid = false;
+    if (c->cfg.p2g_variant == 1 && (e = build_vpage_tables(c)) != hipSuccess) return e;
     const int wb = (int)((words + 255) / 256);
     k_block_counts<<<wb, 256, 0, c->stream>>>(c->pmask, c->nmask, c->bcount, words);
     if ((e = scan_exclusive(c, c->bcount, c->bcount, words, nullptr, words, &c->dc->num_blocks)) != hipSuccess)

@@ -210,6 +210,7 @@ struct mpmx_ctx {
     int64_t tp_bound = 0;    // total pages the domain can hold
     uint32_t* block_key = nullptr;   // [tp_bound] occupied 2x2x2-page blocks, Morton order
     int* cell_start = nullptr;       // [cap_p][65] first particle of each cell of each particle page
+    bool vpages_valid = false;       // adjacency and virtual pages match the current tables
     int* bcount = nullptr;           // [mask_words] per-word block counts, then their scan
     // grid + PCG
     float* grid = nullptr;
@@ -465,6 +466,7 @@ hipError_t scan_exclusive(mpmx_ctx* c, const int* in, int* out, int64_t n_cap, c
 hipError_t sort_and_gather(mpmx_ctx* c);
 hipError_t compute_keys(mpmx_ctx* c);
 hipError_t build_tables(mpmx_ctx* c);
+hipError_t build_vpage_tables(mpmx_ctx* c);
 hipError_t launch_scene_counts(mpmx_ctx* c);
 int particle_blocks(const mpmx_ctx* c);   // 256-thread workgroups of the per-particle kernels
 hipError_t launch_stress(mpmx_ctx* c);
