@@ -122,6 +122,7 @@ def batch_run(mpmx, config, precision, steps, warmup):
     ctx.set_particles_batch(counts, x, v, m)
     ctx.run(warmup, dt)
     ctx.sync()
+    tuning = ctx.autotune(dt, 3)
     ms = ctx.time_run(steps, dt)
     st = ctx.stats()
     n = int(counts.sum())
@@ -130,7 +131,8 @@ def batch_run(mpmx, config, precision, steps, warmup):
                 len(ps), n, counts.min(), counts.max(), "/".join("%d^3" % g for g in sorted({q.N for q in ps})),
                 precision),
             "value": n * steps / (ms / 1e3), "unit": "particle-substeps/s", "ms_per_step": ms / steps,
-            "steps": steps, "warmup": warmup, "cg_iters_last": st["cg_iters"], "cg_breakdown": st["cg_breakdown"]}
+            "steps": steps, "warmup": warmup, "cg_iters_last": st["cg_iters"], "cg_breakdown": st["cg_breakdown"],
+            "tuning": tuning}
 
 
 def scenes_one_at_a_time(mpmx, config, steps, warmup):
@@ -165,6 +167,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=30)
     args = ap.parse_args()
 
@@ -187,6 +190,9 @@ def main():
 
     ctx.run(args.warmup, dt)
     ctx.sync()
+    # launch shapes from the autotuner (trials on a restored snapshot; the
+    # shapes change no result)
+    tuning = ctx.tuning() if args.no_autotune else ctx.autotune(dt, 5)
     t0 = time.perf_counter()
     ms = ctx.time_run(args.steps, dt)   # events on the context stream, synchronised on both sides
     wall = time.perf_counter() - t0
@@ -237,7 +243,7 @@ def main():
         "config": {"workload": WORKLOADS.get(args.scene, "scene %d" % args.scene), "particles": n,
                    "grid": "%d^3" % p.N, "vpage": p.vpage, "precision": args.precision,
                    "pages_total": st["numTotalPage"], "active_nodes": G,
-                   "cg_iters_last": st["cg_iters"], "wall_s": wall},
+                   "cg_iters_last": st["cg_iters"], "wall_s": wall, "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(kname, args.scene, args.precision), "kernel": kname,

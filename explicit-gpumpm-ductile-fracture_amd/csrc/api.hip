@@ -115,7 +115,8 @@ __global__ void k_cell_ids(const uint32_t* __restrict__ keys, int n, const int* 
 
 int particle_blocks(const mpmx_ctx* c) {
     const int64_t threads = c->wmap.first ? c->wmap.waves * 64 : c->n;
-    return (int)std::max<int64_t>(1, (threads + 255) / 256);
+    const int wg = c->particle_wg;
+    return (int)std::max<int64_t>(1, (threads + wg - 1) / wg);
 }
 
 void timer_push(mpmx_ctx* c, const char* name, bool begin) {

@@ -1,6 +1,8 @@
 // io.hip — particle files of a context: a PLY dump (the reference declares
-// MPMSimulator::writePartio, Simulator.h:35, but never defines it) and a
-// checkpoint that resumes bit-exactly.  Host code over the context's buffers.
+// MPMSimulator::writePartio, Simulator.h:35, but never defines it), a
+// checkpoint that resumes bit-exactly, and the launch-shape autotuner, which
+// runs its trials on an in-memory snapshot of the same state.  Host code over
+// the context's buffers.
 //
 // Checkpoint layout (little-endian):
 //   "MPMXCKP1", int32 version 1, int32 num_scenes, int32 precision,
@@ -48,6 +50,48 @@ std::vector<std::pair<void*, size_t>> particle_buffers(mpmx_ctx* c) {
 mpmx_status io_fail(mpmx_ctx* c, mpmx_status s, const char* msg) {
     if (c) c->last_error = msg;
     return s;
+}
+
+// device copy of the particle state and the host flags that go with it
+struct Snapshot {
+    std::vector<std::pair<void*, size_t>> bufs;
+    char* mem = nullptr;
+    int cur = 0, substeps = 0, cg_spec = 1;
+    bool sorted = false;
+};
+
+hipError_t snapshot_take(mpmx_ctx* c, Snapshot& sn) {
+    sn.bufs = particle_buffers(c);
+    size_t total = 0;
+    for (auto& b : sn.bufs) total += (b.second + 255) / 256 * 256;
+    hipError_t e = hipMalloc((void**)&sn.mem, total);
+    if (e != hipSuccess) return e;
+    size_t off = 0;
+    for (auto& b : sn.bufs) {
+        if ((e = hipMemcpyAsync(sn.mem + off, b.first, b.second, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
+            return e;
+        off += (b.second + 255) / 256 * 256;
+    }
+    sn.cur = c->cur;
+    sn.sorted = c->sorted;
+    sn.substeps = c->substeps;
+    sn.cg_spec = c->cg_spec;
+    return hipStreamSynchronize(c->stream);
+}
+
+hipError_t snapshot_restore(mpmx_ctx* c, const Snapshot& sn) {
+    size_t off = 0;
+    hipError_t e = hipSuccess;
+    for (auto& b : sn.bufs) {
+        if ((e = hipMemcpyAsync(b.first, sn.mem + off, b.second, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
+            return e;
+        off += (b.second + 255) / 256 * 256;
+    }
+    c->cur = sn.cur;
+    c->sorted = sn.sorted;
+    c->substeps = sn.substeps;
+    c->cg_spec = sn.cg_spec;
+    return hipStreamSynchronize(c->stream);
 }
 
 }  // namespace
@@ -181,6 +225,79 @@ mpmx_status mpmx_write_particles(mpmx_ctx* c, const char* path) {
     }
     ok = (std::fclose(f) == 0) && ok;
     return ok ? MPMX_OK : io_fail(c, MPMX_ERR_HIP, "particle file write failed");
+}
+
+mpmx_status mpmx_get_tuning(mpmx_ctx* c, mpmx_tuning* out) {
+    if (!c || !out) return MPMX_ERR_ARG;
+    *out = mpmx_tuning{c->particle_wg, c->p2g_pct, c->g2p_pct, c->cg_pct};
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_set_tuning(mpmx_ctx* c, const mpmx_tuning* t) {
+    if (!c || !t) return MPMX_ERR_ARG;
+    if (t->particle_wg != 64 && t->particle_wg != 128 && t->particle_wg != 256) return MPMX_ERR_ARG;
+    for (int pct : {t->p2g_pct, t->g2p_pct, t->cg_pct})
+        if (pct < 10 || pct > 1000) return MPMX_ERR_ARG;
+    c->particle_wg = t->particle_wg;
+    c->p2g_pct = t->p2g_pct;
+    c->g2p_pct = t->g2p_pct;
+    c->cg_pct = t->cg_pct;
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_autotune(mpmx_ctx* c, double dt, int trial_steps, mpmx_tuning* best) {
+    if (!c || c->n == 0 || trial_steps < 1) return MPMX_ERR_ARG;
+    if (hipSetDevice(c->device) != hipSuccess) return io_fail(c, MPMX_ERR_HIP, "hipSetDevice failed");
+    Snapshot sn;
+    if (snapshot_take(c, sn) != hipSuccess) {
+        if (sn.mem) (void)hipFree(sn.mem);
+        return io_fail(c, MPMX_ERR_HIP, "snapshot failed");
+    }
+    const int wgs[3] = {64, 128, 256}, pcts[3] = {50, 100, 200};
+    // per candidate: ms of the kernel families over the trial
+    double t_part[3], t_p2g[3], t_g2p[3], t_cg[3];
+    mpmx_status st = MPMX_OK;
+    std::vector<mpmx_kernel_time> kt(64);
+    for (int k = 0; k < 3 && st == MPMX_OK; ++k) {
+        c->particle_wg = wgs[k];
+        c->p2g_pct = c->g2p_pct = c->cg_pct = pcts[k];
+        if (snapshot_restore(c, sn) != hipSuccess) {
+            st = io_fail(c, MPMX_ERR_HIP, "snapshot restore failed");
+            break;
+        }
+        int ne = 0;
+        if ((st = mpmx_timing_begin(c)) != MPMX_OK) break;
+        if ((st = mpmx_run(c, trial_steps, dt)) != MPMX_OK) break;
+        if ((st = mpmx_timing_end(c, kt.data(), (int)kt.size(), &ne)) != MPMX_OK) break;
+        auto ms = [&](const char* name) {
+            for (int q = 0; q < ne; ++q)
+                if (std::strcmp(kt[(size_t)q].name, name) == 0) return kt[(size_t)q].total_ms;
+            return 0.0;
+        };
+        t_part[k] = ms("stress") + ms("p2g_prep");
+        t_p2g[k] = ms("p2g_block");
+        t_g2p[k] = ms("g2p_block");
+        t_cg[k] = ms("cg_col") + ms("cg_q");
+    }
+    auto argmin = [](const double* v) {
+        int b = 0;
+        for (int k = 1; k < 3; ++k)
+            if (v[k] < v[b]) b = k;
+        return b;
+    };
+    if (st == MPMX_OK) {
+        c->particle_wg = wgs[argmin(t_part)];
+        c->p2g_pct = pcts[argmin(t_p2g)];
+        c->g2p_pct = pcts[argmin(t_g2p)];
+        c->cg_pct = pcts[argmin(t_cg)];
+    } else {
+        c->particle_wg = 256;
+        c->p2g_pct = c->g2p_pct = c->cg_pct = 100;
+    }
+    if (snapshot_restore(c, sn) != hipSuccess && st == MPMX_OK) st = io_fail(c, MPMX_ERR_HIP, "snapshot restore failed");
+    (void)hipFree(sn.mem);
+    if (st == MPMX_OK && best) mpmx_get_tuning(c, best);
+    return st;
 }
 
 }  // extern "C"

@@ -523,11 +523,11 @@ static hipError_t gather_dispatch_t(mpmx_ctx* c, ST st, float dt) {
     hipError_t e = fill_zero_dev_count(c, reinterpret_cast<float*>(c->pbound), &c->dc->num_page, kP2GCh, c->cap_p);
     if (e != hipSuccess) return e;
     const int n = (int)c->n;
-    MPMX_TIMED(c, "p2g_prep", (k_p2g_prep<TRANSFER, ST><<<particle_blocks(c), 256, 0, c->stream>>>(
+    MPMX_TIMED(c, "p2g_prep", (k_p2g_prep<TRANSFER, ST><<<particle_blocks(c), c->particle_wg, 0, c->stream>>>(
                                    c->dp, st, c->wmap, c->sigma, c->FP, c->vol, t, c->payload, c->pbound, dt, n,
                                    c->dc)));
     // about one workgroup per block: the hardware balances their uneven work
-    const int g = block_grid(c);
+    const int g = block_grid(c, c->p2g_pct);
     MPMX_TIMED(c, "p2g_block", (k_p2g_block<TRANSFER, ST><<<g, 256, 0, c->stream>>>(
                                     c->dp, c->payload, c->pbound, t, c->grid, raw, cg_vec(c), dt, n, c->dc)));
     if (c->dp.phase_field)

@@ -487,7 +487,7 @@ static void apply_block(mpmx_ctx* c, ST st) {
 template <typename ST>
 static void apply_blocks(mpmx_ctx* c, ST st) {
     const BlockTables t = block_tables_of(c);
-    const int g1 = block_grid(c), g2 = g1;
+    const int g1 = block_grid(c, c->cg_pct), g2 = g1;
     MPMX_TIMED(c, "cg_col", (k_cg_col<ST><<<(int)g1, 256, 0, c->stream>>>(c->dp, st, c->vol, t, c->cg_s,
                                                                          c->cg_col, (int)c->n, c->scg, c->dc)));
     MPMX_TIMED(c, "cg_q", (k_cg_q<ST><<<(int)g2, 256, 0, c->stream>>>(c->dp, st, t, c->cg_col, c->grid, c->cg_s,

@@ -366,10 +366,10 @@ template <bool ALL64>
 static void stress_dispatch(mpmx_ctx* c) {
     const int n = (int)c->n;
     if (c->cfg.precision == MPMX_MIXED)
-        k_stress<ALL64, MState><<<particle_blocks(c), 256, 0, c->stream>>>(c->dp, c->ms[c->cur], c->wmap, c->sigma,
+        k_stress<ALL64, MState><<<particle_blocks(c), c->particle_wg, 0, c->stream>>>(c->dp, c->ms[c->cur], c->wmap, c->sigma,
                                                                            c->FP, c->vol, n, c->dc);
     else
-        k_stress<ALL64, PState><<<particle_blocks(c), 256, 0, c->stream>>>(c->dp, c->ps[c->cur], c->wmap, c->sigma,
+        k_stress<ALL64, PState><<<particle_blocks(c), c->particle_wg, 0, c->stream>>>(c->dp, c->ps[c->cur], c->wmap, c->sigma,
                                                                            c->FP, c->vol, n, c->dc);
 }
 

@@ -354,12 +354,14 @@ static Tables tables_of(mpmx_ctx* c) {
     return Tables{c->page_key, c->page2particle, c->adj, c->vpo, c->target, c->cap_p, c->cap_tp, c->cap_vp};
 }
 
-// workgroups for the 2x2x2-block kernels: the last block count the host saw
-// (refreshed at every PCG check) plus a margin, a multiple of 8.  The kernels
-// stride over the device-side count, so a stale hint changes only the balance.
-int block_grid(mpmx_ctx* c) {
+// workgroups for the 2x2x2-block kernels: pct % of the last block count the
+// host saw (refreshed at every PCG check) plus a margin, a multiple of 8.  The
+// kernels stride over the device-side count, so a stale hint or any pct
+// changes only the balance, never the results.
+int block_grid(mpmx_ctx* c, int pct) {
     int64_t nb = c->hc->num_blocks;
     nb = nb + nb / 8 + 64;
+    nb = nb * (pct > 0 ? pct : 100) / 100;
     const int64_t cap = c->tp_bound / 8 + 8;
     nb = nb < cap ? nb : cap;
     nb = (nb + 7) / 8 * 8;
@@ -429,7 +431,7 @@ static void g2p_block(mpmx_ctx* c, ST st, float dt) {
 template <typename ST>
 static void g2p_block_kernel(mpmx_ctx* c, ST st, float dt) {
     const BlockTables t = block_tables_of(c);
-    const int g = block_grid(c);
+    const int g = block_grid(c, c->g2p_pct);
     MPMX_TIMED(c, "g2p_block", ({
                    if (c->P.transfer == 0)
                        k_g2p_block<0, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys, dt,

@@ -211,6 +211,9 @@ struct mpmx_ctx {
     uint32_t* block_key = nullptr;   // [tp_bound] occupied 2x2x2-page blocks, Morton order
     int* cell_start = nullptr;       // [cap_p][65] first particle of each cell of each particle page
     bool vpages_valid = false;       // adjacency and virtual pages match the current tables
+    // launch shapes (mpmx_tuning; mpmx_autotune picks them): none changes results
+    int particle_wg = 256;           // threads per workgroup of the per-particle kernels
+    int p2g_pct = 100, g2p_pct = 100, cg_pct = 100;   // block-kernel workgroups, % of the block count
     int* bcount = nullptr;           // [mask_words] per-word block counts, then their scan
     // grid + PCG
     float* grid = nullptr;
@@ -480,7 +483,7 @@ hipError_t fill_zero_dev_count(mpmx_ctx* c, float* p, const int* count_dev, int6
                                int64_t cap_units);
 void timer_push(mpmx_ctx* c, const char* name, bool begin);
 int transfer_grid_blocks(mpmx_ctx* c);
-int block_grid(mpmx_ctx* c);
+int block_grid(mpmx_ctx* c, int pct);
 }  // namespace mpmx
 
 #define MPMX_TIMED(ctx, name, stmt)          \

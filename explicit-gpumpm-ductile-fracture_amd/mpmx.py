@@ -29,7 +29,8 @@ EXPORTS = [
     "mpmx_timing_begin", "mpmx_timing_end", "mpmx_time_run", "mpmx_scene", "mpmx_get_device_pointers",
     "mpmx_max_velocity_sq", "mpmx_create_batch", "mpmx_set_particles_batch", "mpmx_num_scenes",
     "mpmx_get_scene_stats", "mpmx_scene_batch", "mpmx_read_obj_points", "mpmx_sample_sdf",
-    "mpmx_write_particles", "mpmx_save_checkpoint", "mpmx_load_checkpoint",
+    "mpmx_write_particles", "mpmx_save_checkpoint", "mpmx_load_checkpoint", "mpmx_get_tuning",
+    "mpmx_set_tuning", "mpmx_autotune",
 ]
 
 
@@ -71,6 +72,13 @@ class SceneStats(C.Structure):
                 ("num_total_page", C.c_int), ("cg_iters", C.c_int), ("cg_breakdown", C.c_int),
                 ("cg_norm", C.c_double), ("mass", C.c_double), ("momentum", C.c_double * 3),
                 ("kinetic_energy", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("particle_wg", C.c_int), ("p2g_pct", C.c_int), ("g2p_pct", C.c_int), ("cg_pct", C.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class KernelTime(C.Structure):
@@ -151,6 +159,9 @@ def lib():
         "mpmx_write_particles": ([vp, C.c_char_p], C.c_int),
         "mpmx_save_checkpoint": ([vp, C.c_char_p], C.c_int),
         "mpmx_load_checkpoint": ([C.c_char_p, C.c_int, C.POINTER(vp)], C.c_int),
+        "mpmx_get_tuning": ([vp, C.POINTER(Tuning)], C.c_int),
+        "mpmx_set_tuning": ([vp, C.POINTER(Tuning)], C.c_int),
+        "mpmx_autotune": ([vp, C.c_double, C.c_int, C.POINTER(Tuning)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -297,6 +308,24 @@ class Context:
         self.params = None
         self.n = int(self.stats()["num_particles"])
         return self
+
+    def autotune(self, dt, trial_steps=5):
+        """Pick the fastest launch shapes on the current state (which it restores)."""
+        t = Tuning()
+        self._chk(self.L.mpmx_autotune(self.h, dt, trial_steps, C.byref(t)))
+        return t.as_dict()
+
+    def tuning(self):
+        t = Tuning()
+        self._chk(self.L.mpmx_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        self._chk(self.L.mpmx_get_tuning(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._chk(self.L.mpmx_set_tuning(self.h, C.byref(t)))
 
     def save_checkpoint(self, path):
         self._chk(self.L.mpmx_save_checkpoint(self.h, path.encode()))

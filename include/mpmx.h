@@ -320,6 +320,25 @@ int64_t mpmx_scene(int scene, uint64_t seed, double scale, mpmx_params* p, doubl
 int64_t mpmx_scene_batch(int config, uint64_t seed, double scale, int* num_scenes, mpmx_params* params,
                          int64_t* counts, double* x, double* v, double* mass, double* dt);
 
+/* ---- launch shapes and their autotuner ----
+ * None of these changes any result: the block kernels stride over the
+ * device-side block count, and per-particle work does not depend on how it is
+ * grouped.  No reference counterpart (its launch shapes are fixed). */
+typedef struct mpmx_tuning {
+    int particle_wg;  /* threads per workgroup of the per-particle kernels (stress, P2G payload): 64/128/256 */
+    int p2g_pct;      /* workgroups of the block P2G, in % of the 2x2x2-block count (10..1000)      */
+    int g2p_pct;      /* ... of the block G2P                                                         */
+    int cg_pct;       /* ... of the block K s kernels of the phase-field solve                        */
+} mpmx_tuning;
+mpmx_status mpmx_get_tuning(mpmx_ctx* ctx, mpmx_tuning* out);
+mpmx_status mpmx_set_tuning(mpmx_ctx* ctx, const mpmx_tuning* t);
+/* Times trial_steps substeps of the context's current state for each
+ * candidate shape (particle_wg 64/128/256 with the block kernels at 50/100/200 %),
+ * restoring the state after every trial, so the simulation does not advance;
+ * keeps for each kernel family the fastest candidate and returns it in *best
+ * (may be NULL). */
+mpmx_status mpmx_autotune(mpmx_ctx* ctx, double dt, int trial_steps, mpmx_tuning* best);
+
 /* ---- scene files and particle files (host; checkpoints also touch the GPU) ---- */
 /* Sample::readObj (LoadSample.cpp:180-194): a point-list OBJ, one particle per
  * "v x y z" line, y moved by y_shift (the reference's loader uses -0.2).

@@ -1,6 +1,7 @@
-"""Particle files on the GPU path: checkpoint / resume continues bit-exactly
-(one scene and a batch, FP32 and mixed state), and the PLY particle dump holds
-the context's own state."""
+"""Particle files and state snapshots on the GPU path: checkpoint / resume
+continues bit-exactly (one scene and a batch, FP32 and mixed state), the PLY
+particle dump holds the context's own state, and the launch-shape autotuner
+leaves the state where it was and changes no result."""
 import numpy as np
 import pytest
 
@@ -56,3 +57,27 @@ def test_particle_dump(mpmx, tmp_path):
     assert np.array_equal(rec["v"], ctx.get("v").astype(np.float32))
     assert np.array_equal(rec["c"], ctx.get("c").astype(np.float32))
     assert (rec["s"] == 0).all()
+
+
+def test_autotune_changes_no_result(mpmx):
+    """The autotuner's trials run on a snapshot it restores; every launch
+    shape gives the same bits, so a tuned run equals an untuned one."""
+    p, x, v, m, dt = mpmx.scene(3, seed=1, scale=0.1)
+    a = mpmx.Context(p)
+    a.set_particles(x, v, m)
+    a.run(3, dt)
+    before = state(a)
+    best = a.autotune(dt, 2)
+    assert best["particle_wg"] in (64, 128, 256)
+    after = state(a)
+    for k in FIELDS:
+        assert np.array_equal(before[k], after[k]), k
+    assert a.stats()["substeps"] == 3
+    a.run(6, dt)
+    b = mpmx.Context(p)
+    b.set_particles(x, v, m)
+    b.set_tuning(particle_wg=64, p2g_pct=200, g2p_pct=50, cg_pct=50)
+    b.run(9, dt)
+    got, want = state(a), state(b)
+    for k in FIELDS:
+        assert np.array_equal(got[k], want[k]), k
