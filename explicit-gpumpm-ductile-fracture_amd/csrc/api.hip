@@ -119,6 +119,21 @@ int particle_blocks(const mpmx_ctx* c) {
     return (int)std::max<int64_t>(1, (threads + wg - 1) / wg);
 }
 
+// the captured substeps no longer match the context (buffers, capacities or
+// launch shapes changed)
+void destroy_graphs(mpmx_ctx* c) {
+    for (auto& g : c->graphs) {
+        if (g.a) (void)hipGraphExecDestroy(g.a);
+        if (g.b) (void)hipGraphExecDestroy(g.b);
+        g = mpmx_ctx::SubstepGraph{};
+    }
+}
+
+void note_cg_iters(mpmx_ctx* c, int iters) {
+    for (int q = 7; q > 0; --q) c->cg_recent[q] = c->cg_recent[q - 1];
+    c->cg_recent[0] = iters;
+}
+
 void timer_push(mpmx_ctx* c, const char* name, bool begin) {
     if (!c->timing) return;
     TimerEntry* t = nullptr;
@@ -325,6 +340,7 @@ static mpmx_status check_err(mpmx_ctx* c) {
 }
 
 static void free_particles(mpmx_ctx* c) {
+    destroy_graphs(c);
     for (int b = 0; b < 2; ++b) {
         PState& s = c->ps[b];
         dfree(s.x); dfree(s.v); dfree(s.B); dfree(s.F); dfree(s.c); dfree(s.alpha); dfree(s.psi); dfree(s.g);
@@ -343,6 +359,7 @@ static void free_particles(mpmx_ctx* c) {
 }
 
 static mpmx_status alloc_grid(mpmx_ctx* c, int64_t cap_tp) {
+    destroy_graphs(c);
     dfree(c->grid); dfree(c->grid_p2g);
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q); dfree(c->cg_part);
     c->cap_tp = cap_tp;
@@ -444,6 +461,7 @@ void mpmx_default_config(mpmx_config* c) {
     c->svd_gap = 1e-4;
     c->cg_check_every = 1;
     c->stress_fp64 = 1;
+    c->use_graph = 0;
 }
 
 const char* mpmx_status_string(mpmx_status s) {
@@ -760,14 +778,98 @@ mpmx_status mpmx_substep(mpmx_ctx* c, double dt) {
     if ((s = mpmx_p2g(c, dt)) != MPMX_OK) return s;
     if ((s = mpmx_grid_update(c, dt)) != MPMX_OK) return s;
     if ((s = mpmx_phase_field(c, dt)) != MPMX_OK) return s;
+    if (c->dp.phase_field) note_cg_iters(c, c->cg_spec);
     if ((s = mpmx_g2p(c, dt)) != MPMX_OK) return s;
     ++c->substeps;
     return MPMX_OK;
 }
 
+// ---- captured substeps ---------------------------------------------------
+// Captures the substep that starts from the current parity (particles
+// unsorted in buffer c->cur) as graph A (sort, tables, stress, P2G + grid
+// update, `iters` PCG iterations, the counter copy to the host) and graph B
+// (G2P).  The host-side state the stages update during capture is put back
+// afterwards; replays set it themselves.
+static mpmx_status capture_substep(mpmx_ctx* c, double dt, int iters) {
+    const int p = c->cur;
+    auto& G = c->graphs[p];
+    if (G.a) (void)hipGraphExecDestroy(G.a);
+    if (G.b) (void)hipGraphExecDestroy(G.b);
+    G = mpmx_ctx::SubstepGraph{};
+    hipGraph_t ga = nullptr, gb = nullptr;
+    mpmx_status s = MPMX_OK;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    if ((s = mpmx_bin_sort(c)) == MPMX_OK && (s = mpmx_rebuild(c)) == MPMX_OK && (s = mpmx_stress(c)) == MPMX_OK &&
+        (s = mpmx_p2g(c, dt)) == MPMX_OK && (s = mpmx_grid_update(c, dt)) == MPMX_OK) {
+        hipError_t e = c->dp.phase_field ? queue_pcg_iterations(c, 0, iters) : hipSuccess;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->hc, c->dc, sizeof(DevCounts), hipMemcpyDeviceToHost, c->stream);
+        if (e != hipSuccess) s = hipfail(c, e, "capture of the substep");
+    }
+    hipError_t ee = hipStreamEndCapture(c->stream, &ga);
+    if (s == MPMX_OK && ee != hipSuccess) s = hipfail(c, ee, "hipStreamEndCapture");
+    if (s == MPMX_OK) {
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        s = mpmx_g2p(c, dt);
+        ee = hipStreamEndCapture(c->stream, &gb);
+        if (s == MPMX_OK && ee != hipSuccess) s = hipfail(c, ee, "hipStreamEndCapture");
+    }
+    if (s == MPMX_OK) {
+        hipError_t e = hipGraphInstantiate(&G.a, ga, nullptr, nullptr, 0);
+        if (e == hipSuccess) e = hipGraphInstantiate(&G.b, gb, nullptr, nullptr, 0);
+        if (e != hipSuccess) s = hipfail(c, e, "hipGraphInstantiate");
+    }
+    if (ga) (void)hipGraphDestroy(ga);
+    if (gb) (void)hipGraphDestroy(gb);
+    c->cur = p;
+    c->sorted = false;
+    if (s != MPMX_OK) {
+        if (G.a) (void)hipGraphExecDestroy(G.a);
+        if (G.b) (void)hipGraphExecDestroy(G.b);
+        G = mpmx_ctx::SubstepGraph{};
+        return s;
+    }
+    G.valid = true;
+    G.dt = (float)dt;
+    G.iters = iters;
+    G.blocks = c->hc->num_blocks;
+    return MPMX_OK;
+}
+
+// one substep as two graph replays and one host check of the PCG
+static mpmx_status substep_graph(mpmx_ctx* c, double dt) {
+    const int p = c->cur;
+    int want = 1;
+    for (int q : c->cg_recent) want = std::max(want, q);
+    want = std::min(want, std::max(1, c->P.cg_max_iter));
+    auto& G = c->graphs[p];
+    if (!G.valid || G.dt != (float)dt || G.iters != want || c->hc->num_blocks > G.blocks + G.blocks / 4 + 64) {
+        mpmx_status s = capture_substep(c, dt, want);
+        if (s != MPMX_OK) return s;
+    }
+    HIPCHK(c, hipGraphLaunch(G.a, c->stream));
+    c->cur = 1 - p;
+    c->sorted = true;
+    c->grid_fused = c->cfg.p2g_variant != 1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // graph A copied the counters to the host
+    if (c->dp.phase_field) {
+        if (!c->hc->cg_done && G.iters < c->P.cg_max_iter)
+            HIPCHK(c, launch_pcg_from(c, G.iters, c->cfg.cg_check_every > 0 ? c->cfg.cg_check_every : 1));
+        c->cg_spec = c->hc->cg_iters > 0 ? c->hc->cg_iters : 1;
+        note_cg_iters(c, c->hc->cg_iters);
+    }
+    HIPCHK(c, hipGraphLaunch(G.b, c->stream));
+    c->sorted = false;
+    ++c->substeps;
+    return MPMX_OK;
+}
+
 mpmx_status mpmx_run(mpmx_ctx* c, int nsteps, double dt) {
+    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
     for (int k = 0; k < nsteps; ++k) {
-        mpmx_status s = mpmx_substep(c, dt);
+        // the first substep after an upload or a sort, and timed runs, go eagerly
+        const bool graph = c->cfg.use_graph && !c->timing && !c->sorted;
+        mpmx_status s = graph ? substep_graph(c, dt) : mpmx_substep(c, dt);
         if (s != MPMX_OK) return s;
     }
     return MPMX_OK;

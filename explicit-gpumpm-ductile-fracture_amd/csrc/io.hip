@@ -238,6 +238,7 @@ mpmx_status mpmx_set_tuning(mpmx_ctx* c, const mpmx_tuning* t) {
     if (t->particle_wg != 64 && t->particle_wg != 128 && t->particle_wg != 256) return MPMX_ERR_ARG;
     for (int pct : {t->p2g_pct, t->g2p_pct, t->cg_pct})
         if (pct < 10 || pct > 1000) return MPMX_ERR_ARG;
+    destroy_graphs(c);
     c->particle_wg = t->particle_wg;
     c->p2g_pct = t->p2g_pct;
     c->g2p_pct = t->g2p_pct;
@@ -253,6 +254,7 @@ mpmx_status mpmx_autotune(mpmx_ctx* c, double dt, int trial_steps, mpmx_tuning* 
         if (sn.mem) (void)hipFree(sn.mem);
         return io_fail(c, MPMX_ERR_HIP, "snapshot failed");
     }
+    destroy_graphs(c);
     const int wgs[3] = {64, 128, 256}, pcts[3] = {50, 100, 200};
     // per candidate: ms of the kernel families over the trial
     double t_part[3], t_p2g[3], t_g2p[3], t_cg[3];
@@ -294,6 +296,7 @@ mpmx_status mpmx_autotune(mpmx_ctx* c, double dt, int trial_steps, mpmx_tuning* 
         c->particle_wg = 256;
         c->p2g_pct = c->g2p_pct = c->cg_pct = 100;
     }
+    destroy_graphs(c);
     if (snapshot_restore(c, sn) != hipSuccess && st == MPMX_OK) st = io_fail(c, MPMX_ERR_HIP, "snapshot restore failed");
     (void)hipFree(sn.mem);
     if (st == MPMX_OK && best) mpmx_get_tuning(c, best);

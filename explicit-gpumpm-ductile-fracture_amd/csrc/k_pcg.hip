@@ -519,13 +519,11 @@ static hipError_t pcg_iteration(mpmx_ctx* c, int k) {
 // device flag says done.  The first chunk is as long as the previous substep's
 // solve (one host check per substep while the iteration count holds), later
 // chunks cg_check_every long.
-hipError_t launch_pcg(mpmx_ctx* c, float dt) {
-    (void)dt;
-    if (!c->dp.phase_field) return hipSuccess;
+hipError_t launch_pcg_from(mpmx_ctx* c, int it0, int first_chunk) {
     const int maxit = c->P.cg_max_iter;
     const int every = c->cfg.cg_check_every > 0 ? c->cfg.cg_check_every : 1;
-    int chunk = c->cg_spec > 0 ? c->cg_spec : 1;
-    int it = 0;
+    int chunk = first_chunk > 0 ? first_chunk : 1;
+    int it = it0;
     while (it < maxit) {
         chunk = (maxit - it) < chunk ? (maxit - it) : chunk;
         for (int k = 0; k < chunk; ++k) {
@@ -543,6 +541,21 @@ hipError_t launch_pcg(mpmx_ctx* c, float dt) {
         chunk = every;
     }
     c->cg_spec = c->hc->cg_iters > 0 ? c->hc->cg_iters : 1;
+    return hipSuccess;
+}
+
+hipError_t launch_pcg(mpmx_ctx* c, float dt) {
+    (void)dt;
+    if (!c->dp.phase_field) return hipSuccess;
+    return launch_pcg_from(c, 0, c->cg_spec);
+}
+
+// iterations k0 .. k1-1 queued without a host check (for graph capture)
+hipError_t queue_pcg_iterations(mpmx_ctx* c, int k0, int k1) {
+    for (int k = k0; k < k1; ++k) {
+        hipError_t e = pcg_iteration(c, k);
+        if (e != hipSuccess) return e;
+    }
     return hipSuccess;
 }
 

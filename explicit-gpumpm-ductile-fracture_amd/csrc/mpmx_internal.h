@@ -214,6 +214,17 @@ struct mpmx_ctx {
     // launch shapes (mpmx_tuning; mpmx_autotune picks them): none changes results
     int particle_wg = 256;           // threads per workgroup of the per-particle kernels
     int p2g_pct = 100, g2p_pct = 100, cg_pct = 100;   // block-kernel workgroups, % of the block count
+    // captured substeps (mpmx_config.use_graph), one pair per parity of the
+    // double-buffered particle state: A = sort .. PCG (k_graph iterations) +
+    // the counter copy, B = G2P
+    struct SubstepGraph {
+        hipGraphExec_t a = nullptr, b = nullptr;
+        bool valid = false;
+        float dt = 0.f;
+        int iters = 0;
+        int64_t blocks = 0;
+    } graphs[2];
+    int cg_recent[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // PCG iterations of the last substeps
     int* bcount = nullptr;           // [mask_words] per-word block counts, then their scan
     // grid + PCG
     float* grid = nullptr;
@@ -476,12 +487,15 @@ hipError_t launch_stress(mpmx_ctx* c);
 hipError_t launch_p2g(mpmx_ctx* c, float dt);
 hipError_t launch_grid_update(mpmx_ctx* c, float dt);
 hipError_t launch_pcg(mpmx_ctx* c, float dt);
+hipError_t launch_pcg_from(mpmx_ctx* c, int it0, int first_chunk);
+hipError_t queue_pcg_iterations(mpmx_ctx* c, int k0, int k1);
 hipError_t launch_g2p(mpmx_ctx* c, float dt);
 hipError_t launch_cg_reset(mpmx_ctx* c);
 hipError_t fill_zero(mpmx_ctx* c, void* p, int64_t bytes);
 hipError_t fill_zero_dev_count(mpmx_ctx* c, float* p, const int* count_dev, int64_t per_unit,
                                int64_t cap_units);
 void timer_push(mpmx_ctx* c, const char* name, bool begin);
+void destroy_graphs(mpmx_ctx* c);
 int transfer_grid_blocks(mpmx_ctx* c);
 int block_grid(mpmx_ctx* c, int pct);
 }  // namespace mpmx

@@ -99,6 +99,13 @@ typedef struct mpmx_config {
                                particle (default; state stays in its storage precision),
                                0 = FP32, redone in FP64 where the singular-value gap is
                                below svd_gap (0 = never)                              */
+    int     use_graph;      /* 1: mpmx_run / mpmx_time_run replay each substep as two
+                               captured HIP graphs (sort .. PCG, then G2P) instead of ~50
+                               launches; the PCG runs as many iterations in the graph as
+                               recent substeps needed and finishes any remaining ones
+                               eagerly.  Results are identical either way.  Default 0:
+                               on ROCm 7.2 the replays were 1-2 % slower than the direct
+                               launches for this kernel sequence                       */
 } mpmx_config;
 
 typedef struct mpmx_ctx mpmx_ctx;
