@@ -24,6 +24,9 @@ summary of this workload; null for other workloads.
 the 256 scenes run one at a time through the plain FP32 single-scene path
 (the speed-up of the batched mixed form over that).  `--no-batch` skips it.
 
+`impact_8M` reports the two-block impact (8,192,000 particles, 512^3) in
+FP32 and with mixed-precision state, each after the autotuner.
+
 `--copy-probe MIB` runs only a plain device copy (out = 1 * in over MIB MiB of
 float32, three times): the calibration kernel of those counter runs.
 """
@@ -133,6 +136,23 @@ def batch_run(mpmx, config, precision, steps, warmup):
             "value": n * steps / (ms / 1e3), "unit": "particle-substeps/s", "ms_per_step": ms / steps,
             "steps": steps, "warmup": warmup, "cg_iters_last": st["cg_iters"], "cg_breakdown": st["cg_breakdown"],
             "tuning": tuning}
+
+
+def scene_run(mpmx, scene_id, precision, steps, warmup):
+    """One scene through the single-scene path: particle-substeps/s."""
+    p, x, v, m, dt = mpmx.scene(scene_id, seed=1, scale=1.0)
+    cfg = mpmx.default_config(precision=mpmx.MIXED if precision == "mixed" else mpmx.FP32)
+    ctx = mpmx.Context(p, cfg)
+    ctx.set_particles(x, v, m)
+    ctx.run(warmup, dt)
+    ctx.sync()
+    ctx.autotune(dt, 3)
+    ms = ctx.time_run(steps, dt)
+    st = ctx.stats()
+    n = ctx.n
+    ctx.close()
+    return {"value": n * steps / (ms / 1e3), "unit": "particle-substeps/s", "ms_per_step": ms / steps,
+            "particles": n, "precision": precision, "steps": steps, "cg_iters_last": st["cg_iters"]}
 
 
 def scenes_one_at_a_time(mpmx, config, steps, warmup):
@@ -259,6 +279,8 @@ def main():
         alone = scenes_one_at_a_time(mpmx, 5, 3, 2)
         out["batched"] = {"mixed_256": c5, "fp32_64": c4, "plain_fp32_one_scene_at_a_time_256": alone,
                           "speedup_mixed_batch_vs_one_at_a_time": c5["value"] / alone["value"]}
+        out["impact_8M"] = {"workload": WORKLOADS[3], "fp32": scene_run(mpmx, 3, "fp32", 30, 5),
+                            "mixed": scene_run(mpmx, 3, "mixed", 30, 5)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.scene, args.cpu_steps)
     print(json.dumps(out))
