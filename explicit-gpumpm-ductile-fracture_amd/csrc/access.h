@@ -48,75 +48,6 @@ __device__ __forceinline__ uint32_t scene_page_key(const BlockTables& t, int px,
 // it, whose particles reach it only from cells with index 2 or 3 along each
 // axis on which the page is below.
 
-// The 27 source pages in slot order with their whole particle ranges and the
-// cell filter per page, for gathers whose per-entry work is small (K s):
-// one wave resolves the pages, every thread walks a run of the concatenated
-// ranges and skips the particles that do not reach the block.
-struct BlockSources {
-    int start[27], pre[28], filter[27];
-    int page[27];          // particle page id of each source, or -1
-    int out_page[8];
-};
-
-__device__ __forceinline__ void block_sources(const BlockTables& t, uint32_t bkey, const int (&bc)[3], int n,
-                                              int np_raw, int np, int nt, BlockSources& S) {
-    const int lane = threadIdx.x;
-    int cnt = 0;
-    const int scene = (int)((bkey << 3) >> t.slot_shift);
-    if (lane < 27) {
-        const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
-        const int px = 2 * bc[0] + dx, py = 2 * bc[1] + dy, pz = 2 * bc[2] + dz;
-        int st = 0, page_of_source = -1;
-        const uint32_t key = (px >= 0 && py >= 0 && pz >= 0) ? scene_page_key(t, px, py, pz, scene) : ~0u;
-        if (key != ~0u) {
-            const int id = page_rank(key, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
-            if (id >= 0 && id < np) {
-                page_of_source = id;
-                int a = t.page2particle[id], b = t.page2particle[id + 1];
-                a = a < 0 ? 0 : (a > n ? n : a);
-                b = b < a ? a : (b > n ? n : b);
-                st = a;
-                cnt = b - a;
-            }
-        }
-        S.start[lane] = st;
-        S.page[lane] = cnt > 0 ? page_of_source : -1;
-        S.filter[lane] = (dx < 0 ? 1 : 0) | (dy < 0 ? 2 : 0) | (dz < 0 ? 4 : 0);
-    } else if (lane >= 32 && lane < 40) {
-        const int o = lane - 32;
-        const int id = page_rank((bkey << 3) | (uint32_t)o, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
-        S.out_page[o] = (id >= 0 && id < nt) ? id : -1;
-    }
-    int incl = cnt;
-#pragma unroll
-    for (int o = 1; o < 32; o <<= 1) {
-        const int up = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += up;
-    }
-    if (lane < 27) S.pre[lane] = incl - cnt;
-    if (lane == 26) S.pre[27] = incl;
-}
-
-// entry j of the concatenated source ranges: particle index and whether it
-// reaches the block (any thread count)
-__device__ __forceinline__ bool block_entry_simple(const BlockTables& t, const BlockSources& S, int j, int total,
-                                                   int& i) {
-    i = 0;
-    if (j >= total) return false;
-    int lo = 0, hi = 26;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (S.pre[mid] <= j) lo = mid;
-        else hi = mid - 1;
-    }
-    i = S.start[lo] + (j - S.pre[lo]);
-    const int f = S.filter[lo];
-    if (!f) return true;
-    const uint32_t cell = t.keys[i] & 63u;
-    return !(((f & 1) && !((cell >> 5) & 1u)) || ((f & 2) && !((cell >> 3) & 1u)) ||
-             ((f & 4) && !((cell >> 1) & 1u)));
-}
-
 // ---- class-ordered sources -----------------------------------------------------
 // The same 27 source pages, grouped by class: bit 2 / 1 / 0 of the class set
 // when the page lies below the block along x / y / z.  Only the particles in
@@ -148,6 +79,19 @@ struct BlockClasses {
     int thr0[9];             // threads [thr0[c], thr0[c + 1]) work on class c
     int len[8];              // entries per thread of class c
 };
+
+// copy of a block's lists between LDS and HBM (every thread of the workgroup)
+constexpr int kClassWords = (int)(sizeof(BlockClasses) / sizeof(int));
+__device__ __forceinline__ void classes_store(const BlockClasses& S, BlockClasses* __restrict__ dst) {
+    const int* a = reinterpret_cast<const int*>(&S);
+    int* b = reinterpret_cast<int*>(dst);
+    for (int q = threadIdx.x; q < kClassWords; q += blockDim.x) b[q] = a[q];
+}
+__device__ __forceinline__ void classes_load(const BlockClasses* __restrict__ src, BlockClasses& S) {
+    const int* a = reinterpret_cast<const int*>(src);
+    int* b = reinterpret_cast<int*>(&S);
+    for (int q = threadIdx.x; q < kClassWords; q += blockDim.x) b[q] = a[q];
+}
 
 // class and page offset (per axis, -1 below / 0 / 1) of source slot s
 __device__ __forceinline__ void source_slot(int s, int& cls, int (&d)[3]) {

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <numeric>
 
+#include "access.h"
 #include "mpmx_internal.h"
 #include "precision.h"
 
@@ -353,6 +354,8 @@ static void free_particles(mpmx_ctx* c) {
     dfree(c->keys); dfree(c->keys_alt); dfree(c->vals); dfree(c->vals_alt); dfree(c->hist);
     dfree(c->pmask); dfree(c->nmask); dfree(c->wcount); dfree(c->scount);
     dfree(c->page_key); dfree(c->page2particle); dfree(c->adj); dfree(c->nvp); dfree(c->vpo); dfree(c->target); dfree(c->block_key); dfree(c->bcount); dfree(c->cell_start);
+    if (c->block_lists) (void)hipFree(c->block_lists);
+    c->block_lists = nullptr;
     dfree(c->grid); dfree(c->grid_p2g);
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q); dfree(c->cg_part);
     dfree(c->scan_tmp);
@@ -678,6 +681,7 @@ mpmx_status mpmx_set_particles_batch(mpmx_ctx* c, const int64_t* counts, const d
     HIPCHK(c, dalloc(&c->bcount, c->mask_words));
     HIPCHK(c, dalloc(&c->page2particle, c->cap_p + 1));
     HIPCHK(c, dalloc(&c->cell_start, 65 * c->cap_p));
+    HIPCHK(c, hipMalloc(&c->block_lists, sizeof(BlockClasses) * (size_t)tp_bound));
     HIPCHK(c, dalloc(&c->adj, 7 * c->cap_p));
     HIPCHK(c, dalloc(&c->nvp, c->cap_p + 1));
     HIPCHK(c, dalloc(&c->vpo, c->cap_p + 1));

@@ -343,8 +343,9 @@ __device__ __forceinline__ void p2g_class(const BlockClasses& S, const BlockTabl
 template <int TRANSFER, typename ST>
 __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float* __restrict__ pay,
                                                    const unsigned* __restrict__ pbound, BlockTables t,
-                                                   float* __restrict__ grid, float* __restrict__ grid_raw, CGVec cg,
-                                                   float dt, int n, const DevCounts* __restrict__ dc) {
+                                                   BlockClasses* __restrict__ lists, float* __restrict__ grid,
+                                                   float* __restrict__ grid_raw, CGVec cg, float dt, int n,
+                                                   const DevCounts* __restrict__ dc) {
     __shared__ unsigned long long acc[kP2GCh * kAccStride];
     __shared__ BlockClasses S;
     __shared__ int sexp[kP2GCh];
@@ -383,6 +384,8 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
             }
         }
         __syncthreads();
+        // the lists serve the K s gathers of this substep's PCG too
+        classes_store(S, lists + blk);
         for (int q = tid; q < kP2GCh * kAccStride; q += 256) acc[q] = 0ull;
         __syncthreads();
         int e[kP2GCh];
@@ -529,7 +532,8 @@ static hipError_t gather_dispatch_t(mpmx_ctx* c, ST st, float dt) {
     // about one workgroup per block: the hardware balances their uneven work
     const int g = block_grid(c, c->p2g_pct);
     MPMX_TIMED(c, "p2g_block", (k_p2g_block<TRANSFER, ST><<<g, 256, 0, c->stream>>>(
-                                    c->dp, c->payload, c->pbound, t, c->grid, raw, cg_vec(c), dt, n, c->dc)));
+                                    c->dp, c->payload, c->pbound, t, static_cast<BlockClasses*>(c->block_lists), c->grid, raw, cg_vec(c), dt, n,
+                                    c->dc)));
     if (c->dp.phase_field)
         k_cg_rho0<<<c->cg_nb, 256, 0, c->stream>>>(c->dp, c->cg_r, c->cg_z, c->cg_part + c->cg_nb, c->cap_tp,
                                                     c->scg, c->wg_scene, c->dc);

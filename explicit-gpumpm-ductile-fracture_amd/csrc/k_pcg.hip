@@ -228,26 +228,69 @@ __global__ __launch_bounds__(256) void k_cg_col(DevParams dp, ST s, const float*
     }
 }
 
+// (K s) contributions of the calling thread's run of one class of the
+// block's sources (BlockClasses): grad w_ip . col_p into the block's nodes in
+// fixed point; out-of-block nodes are skipped, together by the wave when its
+// lanes hold one class
+template <typename ST>
+__device__ __forceinline__ void cgq_class(const BlockClasses& S, const DevParams& sp, const ST& s,
+                                          const float* __restrict__ col, const int (&org)[3], float sc,
+                                          unsigned long long* acc, int cls, int jb, int je) {
+    for (int j = jb; j < je; ++j) {
+        const int i = class_entry(S, cls, j);
+        float w[3][3], gw[3][3], c3[3];
+        int l0[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float xa = ld_x(s, i, a);
+            const int b = base_node(xa, sp.N);
+            const float d0 = xa * sp.inv_dx - (float)b;
+            l0[a] = b - org[a];
+            bspline_grad(d0, w[a], gw[a]);
+            c3[a] = col[3 * (int64_t)i + a];
+        }
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+            const int la = l0[0] + ii;
+            if (la < 0 || la >= 8) continue;
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) {
+                const int lb = l0[1] + jj;
+                if (lb < 0 || lb >= 8) continue;
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) {
+                    const int lc = l0[2] + kk;
+                    if (lc < 0 || lc >= 8) continue;
+                    const float hw = gw[0][ii] * w[1][jj] * w[2][kk] * c3[0] +
+                                     w[0][ii] * gw[1][jj] * w[2][kk] * c3[1] +
+                                     w[0][ii] * w[1][jj] * gw[2][kk] * c3[2];
+                    atomicAdd(&acc[(la * 8 + lb) * kAccZ + lc], to_fix(hw, sc));
+                }
+            }
+        }
+    }
+}
+
 // q_i = sum_p grad w_ip . col_p + D_i s_i over the 8^3 nodes of a block, as a
 // gather with 64-bit fixed-point LDS atomics (|grad w . col| <= sum |col|
-// gives the scale); q is written whole (no zero fill, no global atomics)
+// gives the scale); q is written whole (no zero fill, no global atomics).
+// The block's source lists are the ones the block P2G of this substep built
+// (same block numbering), read from HBM.
 template <typename ST>
-__global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t, const float* __restrict__ col,
+__global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t,
+                                              const BlockClasses* __restrict__ lists, const float* __restrict__ col,
                                               const float* __restrict__ grid, const float* __restrict__ cgs,
                                               float* __restrict__ cgq, int n, const SceneCG* __restrict__ scg,
                                               DevCounts* dc) {
     __shared__ unsigned long long acc[kAccStride];
-    __shared__ BlockSources S;
+    __shared__ BlockClasses S;
     __shared__ float wmax[4];
     __shared__ int sexp;
     if (cg_stopped(dc)) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int np_raw = dc->num_page;
-    const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
-    int nt = dc->num_total;
-    nt = nt < t.cap_tp ? nt : (int)t.cap_tp;
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
+    (void)n;
     FOR_LOGICAL_ITEMS(blk, nb) {
         const uint32_t bkey = t.block_key[blk];
         const int scene = __builtin_amdgcn_readfirstlane(scene_of_page(dp, bkey << 3));
@@ -255,21 +298,19 @@ __global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t,
         const DevParams sp = dp.scenes[scene];
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
-        if (wave == 0) block_sources(t, bkey, bc, n, np_raw, np, nt, S);
+        classes_load(lists + blk, S);
         for (int q = tid; q < kAccStride; q += 256) acc[q] = 0ull;
         __syncthreads();
-        const int total = S.pre[27];
-        // the block's first node in scene-local cells
-        const int org[3] = {8 * bc[0] - sp.origin[0], 8 * bc[1] - sp.origin[1], 8 * bc[2] - sp.origin[2]};
+        // scale: the largest |col|_1 of the listed particles, over each thread's run
+        int cls, jb, je;
+        const bool mine = thread_class(S, cls, jb, je);
         float bm = 0.f;
-        const int per = (total + 255) / 256;   // runs of entries per thread (see k_p2g_block)
-        for (int k = 0; k < per; ++k) {
-            const int j = tid * per + k;
-            int i;
-            if (!block_entry_simple(t, S, j, total, i)) continue;
-            bm = fmaxf(bm, fabsf(col[3 * (int64_t)i]) + fabsf(col[3 * (int64_t)i + 1]) +
-                               fabsf(col[3 * (int64_t)i + 2]));
-        }
+        if (mine)
+            for (int j = jb; j < je; ++j) {
+                const int i = class_entry(S, cls, j);
+                bm = fmaxf(bm, fabsf(col[3 * (int64_t)i]) + fabsf(col[3 * (int64_t)i + 1]) +
+                                   fabsf(col[3 * (int64_t)i + 2]));
+            }
         bm = wave_max(bm);
         if (lane == 0) wmax[wave] = bm;
         __syncthreads();
@@ -277,41 +318,9 @@ __global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t,
         __syncthreads();
         const int e = sexp;
         const float sc = ldexpf(1.0f, e);
-        for (int k = 0; k < per; ++k) {
-            const int j = tid * per + k;
-            int i;
-            if (!block_entry_simple(t, S, j, total, i)) continue;
-            float w[3][3], gw[3][3], c3[3];
-            int l0[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const float xa = ld_x(s, i, a);
-                const int b = base_node(xa, sp.N);
-                const float d0 = xa * sp.inv_dx - (float)b;
-                l0[a] = b - org[a];
-                bspline_grad(d0, w[a], gw[a]);
-                c3[a] = col[3 * (int64_t)i + a];
-            }
-#pragma unroll
-            for (int ii = 0; ii < 3; ++ii) {
-                const int la = l0[0] + ii;
-                if (la < 0 || la >= 8) continue;
-#pragma unroll
-                for (int jj = 0; jj < 3; ++jj) {
-                    const int lb = l0[1] + jj;
-                    if (lb < 0 || lb >= 8) continue;
-#pragma unroll
-                    for (int kk = 0; kk < 3; ++kk) {
-                        const int lc = l0[2] + kk;
-                        if (lc < 0 || lc >= 8) continue;
-                        const float hw = gw[0][ii] * w[1][jj] * w[2][kk] * c3[0] +
-                                         w[0][ii] * gw[1][jj] * w[2][kk] * c3[1] +
-                                         w[0][ii] * w[1][jj] * gw[2][kk] * c3[2];
-                        atomicAdd(&acc[(la * 8 + lb) * kAccZ + lc], to_fix(hw, sc));
-                    }
-                }
-            }
-        }
+        // the block's first node in scene-local cells
+        const int org[3] = {8 * bc[0] - sp.origin[0], 8 * bc[1] - sp.origin[1], 8 * bc[2] - sp.origin[2]};
+        if (mine) cgq_class(S, sp, s, col, org, sc, acc, cls, jb, je);
         __syncthreads();
         for (int q = tid; q < 512; q += 256) {
             const int o = q >> 6, cell = q & 63;
@@ -490,8 +499,9 @@ static void apply_blocks(mpmx_ctx* c, ST st) {
     const int g1 = block_grid(c, c->cg_pct), g2 = g1;
     MPMX_TIMED(c, "cg_col", (k_cg_col<ST><<<(int)g1, 256, 0, c->stream>>>(c->dp, st, c->vol, t, c->cg_s,
                                                                          c->cg_col, (int)c->n, c->scg, c->dc)));
-    MPMX_TIMED(c, "cg_q", (k_cg_q<ST><<<(int)g2, 256, 0, c->stream>>>(c->dp, st, t, c->cg_col, c->grid, c->cg_s,
-                                                                     c->cg_q, (int)c->n, c->scg, c->dc)));
+    MPMX_TIMED(c, "cg_q", (k_cg_q<ST><<<(int)g2, 256, 0, c->stream>>>(
+                              c->dp, st, t, static_cast<const BlockClasses*>(c->block_lists), c->cg_col, c->grid,
+                              c->cg_s, c->cg_q, (int)c->n, c->scg, c->dc)));
 }
 
 static hipError_t pcg_iteration(mpmx_ctx* c, int k) {
