@@ -753,6 +753,17 @@ mpmx_status mpmx_p2g(mpmx_ctx* c, double dt) {
     return MPMX_OK;
 }
 
+// stress and P2G of a substep; with the gather P2G one pass computes the
+// stress and the P2G payload
+static mpmx_status stress_and_p2g(mpmx_ctx* c, double dt) {
+    if (c->cfg.p2g_variant == 1) {
+        mpmx_status s = mpmx_stress(c);
+        return s == MPMX_OK ? mpmx_p2g(c, dt) : s;
+    }
+    HIPCHK(c, launch_stress_p2g(c, (float)dt));
+    return MPMX_OK;
+}
+
 mpmx_status mpmx_grid_update(mpmx_ctx* c, double dt) {
     if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
     MPMX_TIMED(c, "grid_update", HIPCHK(c, launch_grid_update(c, (float)dt)));
@@ -778,8 +789,7 @@ mpmx_status mpmx_substep(mpmx_ctx* c, double dt) {
     mpmx_status s;
     if ((s = mpmx_bin_sort(c)) != MPMX_OK) return s;
     if ((s = mpmx_rebuild(c)) != MPMX_OK) return s;
-    if ((s = mpmx_stress(c)) != MPMX_OK) return s;
-    if ((s = mpmx_p2g(c, dt)) != MPMX_OK) return s;
+    if ((s = stress_and_p2g(c, dt)) != MPMX_OK) return s;
     if ((s = mpmx_grid_update(c, dt)) != MPMX_OK) return s;
     if ((s = mpmx_phase_field(c, dt)) != MPMX_OK) return s;
     if (c->dp.phase_field) note_cg_iters(c, c->cg_spec);
@@ -803,8 +813,8 @@ static mpmx_status capture_substep(mpmx_ctx* c, double dt, int iters) {
     hipGraph_t ga = nullptr, gb = nullptr;
     mpmx_status s = MPMX_OK;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    if ((s = mpmx_bin_sort(c)) == MPMX_OK && (s = mpmx_rebuild(c)) == MPMX_OK && (s = mpmx_stress(c)) == MPMX_OK &&
-        (s = mpmx_p2g(c, dt)) == MPMX_OK && (s = mpmx_grid_update(c, dt)) == MPMX_OK) {
+    if ((s = mpmx_bin_sort(c)) == MPMX_OK && (s = mpmx_rebuild(c)) == MPMX_OK &&
+        (s = stress_and_p2g(c, dt)) == MPMX_OK && (s = mpmx_grid_update(c, dt)) == MPMX_OK) {
         hipError_t e = c->dp.phase_field ? queue_pcg_iterations(c, 0, iters) : hipSuccess;
         if (e == hipSuccess)
             e = hipMemcpyAsync(c->hc, c->dc, sizeof(DevCounts), hipMemcpyDeviceToHost, c->stream);

@@ -20,11 +20,10 @@
 // global atomics, no zero fill of the grid, and sums that do not depend on
 // the order of the additions.  Each particle is read by the one to eight
 // blocks it reaches (about twice on average).
-#include "access.h"
+#include "payload.h"
 
 namespace mpmx {
 
-constexpr int kP2GCh = 8;   // m, p0, p1, p2, sum w c, sum w, sum V w, sum V FP w
 
 // grid update of one node with the raw P2G sums ch[] (undateGrid_kernal, the
 // grips of PullGrid_kernal and collideWithGround), then the node's PCG start
@@ -96,62 +95,6 @@ __global__ void k_cg_reset(DevParams dp, SceneCG* __restrict__ scg, DevCounts* d
     }
 }
 
-// per-particle P2G terms, computed once and kept in registers across rounds
-struct P2GTerms {
-    float w[3][3], gw[3][3];
-    int l0[3];
-    float m, cphase, V, VF;
-    float ar[3], Qd[9], sg[9];
-};
-
-template <int TRANSFER, typename ST>
-__device__ __forceinline__ void p2g_terms(const DevParams& dp, const ST& s, const float* __restrict__ sigma,
-                                          const float* __restrict__ FP, const float* __restrict__ vol, int i,
-                                          int n, float dt, const int (&org)[3], P2GTerms& T) {
-    float d0[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const float xa = ld_x(s, i, a);
-        const int b = base_node(xa, dp.N);
-        d0[a] = xa * dp.inv_dx - (float)b;
-        T.l0[a] = b - org[a];
-        if (TRANSFER == 0) bspline(d0[a], T.w[a]);
-        else bspline_grad(d0[a], T.w[a], T.gw[a]);
-    }
-    T.m = ld_m(s, i);
-    T.cphase = ld_c(s, i);
-    T.V = vol[i];
-    T.VF = T.V * FP[i];
-    float v[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) v[a] = ld_v(s, i, a);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const float Bk = ld_B(s, i, k, n);
-        const float sk = sigma[(int64_t)k * n + i];
-        if (TRANSFER == 0) T.Qd[k] = (Bk * T.m - sk * dt) * dp.Dinv * dp.dx;
-        else { T.Qd[k] = Bk * dp.Dinv * dp.dx * T.m; T.sg[k] = sk * dt * dp.inv_dx; }
-    }
-#pragma unroll
-    for (int r = 0; r < 3; ++r) T.ar[r] = T.m * v[r] - (T.Qd[r] * d0[0] + T.Qd[r + 3] * d0[1] + T.Qd[r + 6] * d0[2]);
-}
-
-// largest |contribution| of the particle to any node, per channel (weights <= 1,
-// gradients of the quadratic B-spline <= 1 in magnitude per axis)
-template <int TRANSFER>
-__device__ __forceinline__ void p2g_bounds(const P2GTerms& T, float (&b)[kP2GCh]) {
-    b[0] = fabsf(T.m);
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        float q = fabsf(T.ar[r]) + 2.f * (fabsf(T.Qd[r]) + fabsf(T.Qd[r + 3]) + fabsf(T.Qd[r + 6]));
-        if (TRANSFER == 1) q += fabsf(T.sg[r]) + fabsf(T.sg[r + 3]) + fabsf(T.sg[r + 6]);
-        b[1 + r] = q;
-    }
-    b[4] = fabsf(T.cphase);
-    b[5] = 1.f;
-    b[6] = fabsf(T.V);
-    b[7] = fabsf(T.VF);
-}
 
 // the particle's terms times the channel scales (powers of two, so every
 // product below is the unscaled product times the scale, exactly)
@@ -179,17 +122,27 @@ __device__ __forceinline__ void scale_terms(P2GTerms& T, const float (&sc)[kP2GC
 // scale of the weight channel) to the block's 8^3 nodes in fixed point.
 // Nodes outside the block are skipped; a wave whose lanes all hold particles
 // of one source class (see BlockClasses) skips them together.
+// The affine term ar + Qd (ii, jj, kk) is accumulated axis by axis with one
+// fused multiply-add per nonzero offset, shared by the nodes of a row; a zero
+// offset adds nothing (for finite terms it could only change the sign of a
+// zero, which converts to the same integer).
 template <int TRANSFER>
 __device__ __forceinline__ void p2g_fix(const P2GTerms& T, float sc5, unsigned long long* acc) {
 #pragma unroll
     for (int ii = 0; ii < 3; ++ii) {
         const int la = T.l0[0] + ii;
         if (la < 0 || la >= 8) continue;
+        float ax[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) ax[r] = ii == 0 ? T.ar[r] : fmaf(T.Qd[r], (float)ii, T.ar[r]);
 #pragma unroll
         for (int jj = 0; jj < 3; ++jj) {
             const int lb = T.l0[1] + jj;
             if (lb < 0 || lb >= 8) continue;
             const float wxy = T.w[0][ii] * T.w[1][jj];
+            float axy[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) axy[r] = jj == 0 ? ax[r] : fmaf(T.Qd[r + 3], (float)jj, ax[r]);
 #pragma unroll
             for (int kk = 0; kk < 3; ++kk) {
                 const int lc = T.l0[2] + kk;
@@ -197,8 +150,7 @@ __device__ __forceinline__ void p2g_fix(const P2GTerms& T, float sc5, unsigned l
                 const float wt = wxy * T.w[2][kk];
                 float pr[3];
 #pragma unroll
-                for (int r = 0; r < 3; ++r)
-                    pr[r] = wt * (T.ar[r] + T.Qd[r] * ii + T.Qd[r + 3] * jj + T.Qd[r + 6] * kk);
+                for (int r = 0; r < 3; ++r) pr[r] = wt * (kk == 0 ? axy[r] : fmaf(T.Qd[r + 6], (float)kk, axy[r]));
                 if (TRANSFER == 1) {
                     const float g0 = T.gw[0][ii] * T.w[1][jj] * T.w[2][kk];
                     const float g1 = T.w[0][ii] * T.gw[1][jj] * T.w[2][kk];
@@ -206,82 +158,24 @@ __device__ __forceinline__ void p2g_fix(const P2GTerms& T, float sc5, unsigned l
 #pragma unroll
                     for (int r = 0; r < 3; ++r) pr[r] -= T.sg[r] * g0 + T.sg[r + 3] * g1 + T.sg[r + 6] * g2;
                 }
-                unsigned long long* a = acc + (la * 8 + lb) * kAccZ + lc;
+                unsigned long long* a = acc + acc_node(la, lb, lc);
                 const float add[kP2GCh] = {wt * T.m, pr[0], pr[1], pr[2], wt * T.cphase, wt * sc5, wt * T.V, wt * T.VF};
+                // the weights are >= 0, so channel 5 needs no sign extension
 #pragma unroll
-                for (int k = 0; k < kP2GCh; ++k) atomicAdd(a + k * kAccStride, to_fix_scaled(add[k]));
+                for (int k = 0; k < kP2GCh; ++k)
+                    atomicAdd(a + k * kAccStride, k == 5 ? to_fix_scaled_nonneg(add[k]) : to_fix_scaled(add[k]));
             }
         }
     }
 }
 
-// ---- per-particle P2G payload ------------------------------------------------
-// The terms every node contribution of a particle needs, computed once per
-// substep in sorted order and stored as float4 records: d0 (the particle's
-// offset from its base node in cells), m, ar, Qd, c, V, V FP and, for APIC,
-// the stress term sg (MLS: 5 records of 4 floats; APIC: 7, sg[8] in the pad
-// slot of the fifth).  The block P2G reads one record per visit (a particle
-// is visited by the one to eight blocks it reaches) instead of recomputing
-// them from 26 scattered fields; the base node itself comes from the key.
-template <int TRANSFER> struct Payload { static constexpr int kFloats = TRANSFER ? 28 : 20; };
 
-template <int TRANSFER>
-__device__ __forceinline__ void payload_store(float* __restrict__ pay, int64_t i, const float (&d0)[3],
-                                              const P2GTerms& T) {
-    float4* q = reinterpret_cast<float4*>(pay + i * Payload<TRANSFER>::kFloats);
-    q[0] = make_float4(d0[0], d0[1], d0[2], T.m);
-    q[1] = make_float4(T.ar[0], T.ar[1], T.ar[2], T.cphase);
-    q[2] = make_float4(T.Qd[0], T.Qd[1], T.Qd[2], T.Qd[3]);
-    q[3] = make_float4(T.Qd[4], T.Qd[5], T.Qd[6], T.Qd[7]);
-    q[4] = make_float4(T.Qd[8], T.V, T.VF, TRANSFER == 1 ? T.sg[8] : 0.f);
-    if (TRANSFER == 1) {
-        q[5] = make_float4(T.sg[0], T.sg[1], T.sg[2], T.sg[3]);
-        q[6] = make_float4(T.sg[4], T.sg[5], T.sg[6], T.sg[7]);
-    }
-}
-
-// the terms of particle i (key k) relative to the block whose first node is org
-template <int TRANSFER>
-__device__ __forceinline__ void payload_load(const float* __restrict__ pay, int64_t i, uint32_t key,
-                                             const int (&org)[3], P2GTerms& T) {
-    const float4* q = reinterpret_cast<const float4*>(pay + i * Payload<TRANSFER>::kFloats);
-    const float4 a = q[0], b = q[1], c0 = q[2], c1 = q[3], c2 = q[4];
-    const float d0[3] = {a.x, a.y, a.z};
-    T.m = a.w;
-    T.ar[0] = b.x; T.ar[1] = b.y; T.ar[2] = b.z;
-    T.cphase = b.w;
-    T.Qd[0] = c0.x; T.Qd[1] = c0.y; T.Qd[2] = c0.z; T.Qd[3] = c0.w;
-    T.Qd[4] = c1.x; T.Qd[5] = c1.y; T.Qd[6] = c1.z; T.Qd[7] = c1.w;
-    T.Qd[8] = c2.x; T.V = c2.y; T.VF = c2.z;
-    if (TRANSFER == 1) {
-        const float4 s0 = q[5], s1 = q[6];
-        T.sg[0] = s0.x; T.sg[1] = s0.y; T.sg[2] = s0.z; T.sg[3] = s0.w;
-        T.sg[4] = s1.x; T.sg[5] = s1.y; T.sg[6] = s1.z; T.sg[7] = s1.w;
-        T.sg[8] = c2.w;
-    }
-    int px, py, pz;
-    unmorton(key >> 6, px, py, pz);
-    const int cell = (int)(key & 63u);
-    T.l0[0] = 4 * px + (cell >> 4) - org[0];
-    T.l0[1] = 4 * py + ((cell >> 2) & 3) - org[1];
-    T.l0[2] = 4 * pz + (cell & 3) - org[2];
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-        if (TRANSFER == 0) bspline(d0[ax], T.w[ax]);
-        else bspline_grad(d0[ax], T.w[ax], T.gw[ax]);
-    }
-}
-
-// payload of every particle and, per particle page, the largest contribution
-// per channel (a wave-segmented max over the page's contiguous particles, then
-// one integer atomicMax per page and channel on the float bits, which order
-// like the floats for non-negative values); pbound is zero on entry
+// payload of every particle and its page's channel bounds (page_bounds)
 template <int TRANSFER, typename ST>
 __global__ __launch_bounds__(256) void k_p2g_prep(DevParams dp, ST s, WaveMap wm, const float* __restrict__ sigma,
                                                   const float* __restrict__ FP, const float* __restrict__ vol,
                                                   BlockTables t, float* __restrict__ pay, unsigned* __restrict__ pbound,
                                                   float dt, int n, const DevCounts* __restrict__ dc) {
-    const int lane = threadIdx.x & 63;
     int i, scene;
     const bool valid = wave_particle(wm, n, i, scene);
     float b[kP2GCh];
@@ -289,38 +183,12 @@ __global__ __launch_bounds__(256) void k_p2g_prep(DevParams dp, ST s, WaveMap wm
     for (int k = 0; k < kP2GCh; ++k) b[k] = 0.f;
     const uint32_t pk = valid ? t.keys[i] >> 6 : 0xffffffffu;
     if (valid) {
-        const DevParams& sp = dp.scenes[scene];
-        const int org[3] = {0, 0, 0};
-        P2GTerms T;
-        p2g_terms<TRANSFER>(sp, s, sigma, FP, vol, i, n, dt, org, T);
-        float d0[3];
+        float sig[9];
 #pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            const float xa = ld_x(s, i, ax);
-            d0[ax] = xa * sp.inv_dx - (float)base_node(xa, sp.N);
-        }
-        payload_store<TRANSFER>(pay, i, d0, T);
-        p2g_bounds<TRANSFER>(T, b);
+        for (int k = 0; k < 9; ++k) sig[k] = sigma[(int64_t)k * n + i];
+        prep_particle<TRANSFER>(dp.scenes[scene], s, sig, FP[i], vol[i], i, n, dt, pay, b);
     }
-    const int np_raw = dc->num_page;
-    const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
-    unsigned long long todo = __ballot(valid);
-    while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const uint32_t lpk = __shfl(pk, leader, 64);
-        const bool mine = valid && pk == lpk;
-        todo &= ~__ballot(mine);
-        float m[kP2GCh];
-#pragma unroll
-        for (int k = 0; k < kP2GCh; ++k) m[k] = wave_max(mine ? b[k] : 0.f);
-        if (lane == leader) {
-            const int id = page_rank(lpk, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
-            if (id >= 0 && id < np) {
-#pragma unroll
-                for (int k = 0; k < kP2GCh; ++k) atomicMax(&pbound[(int64_t)id * kP2GCh + k], __float_as_uint(m[k]));
-            }
-        }
-    }
+    page_bounds(valid, pk, b, t, pbound, dc);
 }
 
 // the calling thread's run [jb, je) of the entries of one class of the
@@ -343,17 +211,13 @@ __device__ __forceinline__ void p2g_class(const BlockClasses& S, const BlockTabl
 template <int TRANSFER, typename ST>
 __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float* __restrict__ pay,
                                                    const unsigned* __restrict__ pbound, BlockTables t,
-                                                   BlockClasses* __restrict__ lists, float* __restrict__ grid,
+                                                   const BlockClasses* __restrict__ lists, float* __restrict__ grid,
                                                    float* __restrict__ grid_raw, CGVec cg, float dt, int n,
                                                    const DevCounts* __restrict__ dc) {
     __shared__ unsigned long long acc[kP2GCh * kAccStride];
     __shared__ BlockClasses S;
     __shared__ int sexp[kP2GCh];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int np_raw = dc->num_page;
-    const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
-    int nt = dc->num_total;
-    nt = nt < t.cap_tp ? nt : (int)t.cap_tp;
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
     FOR_LOGICAL_ITEMS(blk, nb) {
@@ -361,21 +225,18 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
         const DevParams sp = dp.scenes[__builtin_amdgcn_readfirstlane(scene_of_page(dp, bkey << 3))];
-        if (wave == 0) block_pages(t, bkey, bc, n, np_raw, np, nt, S);
-        __syncthreads();
-        int* rows = reinterpret_cast<int*>(acc);   // cell starts, before the sums use the space
-        block_cell_rows(t, S, n, rows);
-        __syncthreads();
-        if (wave == 0) block_runs(S, rows);
+        // the block's source lists, built with the page tables
+        const BlockClasses* L = lists + blk;
+        classes_load(L, S);
+        for (int q = tid; q < kP2GCh * kAccStride; q += 256) acc[q] = 0ull;
         const int org[3] = {8 * bc[0], 8 * bc[1], 8 * bc[2]};
         // scales: the largest contribution per channel of the 27 source pages
         if (wave == 1) {
+            const int pg = lane < 27 ? L->page[lane] : -1;
             float bmax[kP2GCh];
 #pragma unroll
-            for (int k = 0; k < kP2GCh; ++k) {
-                const int pg = lane < 27 ? S.page[lane] : -1;
+            for (int k = 0; k < kP2GCh; ++k)
                 bmax[k] = wave_max(pg >= 0 ? __uint_as_float(pbound[(int64_t)pg * kP2GCh + k]) : 0.f);
-            }
             if (lane < kP2GCh) {
                 float v = bmax[0];
 #pragma unroll
@@ -383,10 +244,6 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
                 sexp[lane] = fix_exp(v);
             }
         }
-        __syncthreads();
-        // the lists serve the K s gathers of this substep's PCG too
-        classes_store(S, lists + blk);
-        for (int q = tid; q < kP2GCh * kAccStride; q += 256) acc[q] = 0ull;
         __syncthreads();
         int e[kP2GCh];
         float sc[kP2GCh];
@@ -405,7 +262,7 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
             if (G < 0) continue;
             const int lx = ((o >> 2) & 1) * 4 + (cell >> 4), ly = ((o >> 1) & 1) * 4 + ((cell >> 2) & 3),
                       lz = (o & 1) * 4 + (cell & 3);
-            const int node = (lx * 8 + ly) * kAccZ + lz;
+            const int node = acc_node(lx, ly, lz);
             float ch[kP2GCh];
 #pragma unroll
             for (int k = 0; k < kP2GCh; ++k) ch[k] = from_fix(acc[k * kAccStride + node], e[k]);
@@ -510,6 +367,32 @@ __global__ __launch_bounds__(256) void k_grid_update(DevParams dp, const uint32_
     if (threadIdx.x == 0) part_rho[blockIdx.x] = r;
 }
 
+// source lists of every 2x2x2 block (block_lists_wave), one wave per block;
+// they depend on the page tables only and serve the block P2G and the K s
+// gathers of the substep's PCG
+__global__ __launch_bounds__(256) void k_block_lists(BlockTables t, BlockClasses* __restrict__ lists, int n,
+                                                     const DevCounts* __restrict__ dc) {
+    const int np_raw = dc->num_page;
+    const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
+    int nt = dc->num_total;
+    nt = nt < t.cap_tp ? nt : (int)t.cap_tp;
+    int nb = dc->num_blocks;
+    nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
+    const int per = (int)(blockDim.x >> 6);
+    const int nwaves = (int)gridDim.x * per;
+    for (int blk = xcd_logical_block() * per + (int)(threadIdx.x >> 6); blk < nb; blk += nwaves)
+        block_lists_wave(t, t.block_key[blk], n, np_raw, np, nt, lists + blk);
+}
+
+hipError_t launch_block_lists(mpmx_ctx* c) {
+    int g = (block_grid(c, 100) + 3) / 4;
+    g = (g + 7) / 8 * 8;
+    MPMX_TIMED(c, "block_lists", (k_block_lists<<<g, 256, 0, c->stream>>>(
+                                      block_tables_of(c), static_cast<BlockClasses*>(c->block_lists), (int)c->n,
+                                      c->dc)));
+    return hipGetLastError();
+}
+
 hipError_t launch_p2g_scatter(mpmx_ctx* c, float dt);
 
 hipError_t launch_cg_reset(mpmx_ctx* c) {
@@ -519,25 +402,31 @@ hipError_t launch_cg_reset(mpmx_ctx* c) {
 
 static CGVec cg_vec(mpmx_ctx* c) { return CGVec{c->cg_x, c->cg_r, c->cg_z, c->cg_s, c->cg_q}; }
 
+// block P2G (with grid update and PCG set-up) over the payload and page bounds
 template <int TRANSFER, typename ST>
-static hipError_t gather_dispatch_t(mpmx_ctx* c, ST st, float dt) {
+static hipError_t gather_blocks_t(mpmx_ctx* c, float dt) {
     const BlockTables t = block_tables_of(c);
     float* raw = (c->keep_p2g && c->grid_p2g) ? c->grid_p2g : nullptr;
-    hipError_t e = fill_zero_dev_count(c, reinterpret_cast<float*>(c->pbound), &c->dc->num_page, kP2GCh, c->cap_p);
-    if (e != hipSuccess) return e;
-    const int n = (int)c->n;
-    MPMX_TIMED(c, "p2g_prep", (k_p2g_prep<TRANSFER, ST><<<particle_blocks(c), c->particle_wg, 0, c->stream>>>(
-                                   c->dp, st, c->wmap, c->sigma, c->FP, c->vol, t, c->payload, c->pbound, dt, n,
-                                   c->dc)));
     // about one workgroup per block: the hardware balances their uneven work
     const int g = block_grid(c, c->p2g_pct);
     MPMX_TIMED(c, "p2g_block", (k_p2g_block<TRANSFER, ST><<<g, 256, 0, c->stream>>>(
-                                    c->dp, c->payload, c->pbound, t, static_cast<BlockClasses*>(c->block_lists), c->grid, raw, cg_vec(c), dt, n,
-                                    c->dc)));
+                                    c->dp, c->payload, c->pbound, t, static_cast<BlockClasses*>(c->block_lists),
+                                    c->grid, raw, cg_vec(c), dt, (int)c->n, c->dc)));
     if (c->dp.phase_field)
         k_cg_rho0<<<c->cg_nb, 256, 0, c->stream>>>(c->dp, c->cg_r, c->cg_z, c->cg_part + c->cg_nb, c->cap_tp,
                                                     c->scg, c->wg_scene, c->dc);
     return hipGetLastError();
+}
+
+template <int TRANSFER, typename ST>
+static hipError_t gather_dispatch_t(mpmx_ctx* c, ST st, float dt) {
+    const BlockTables t = block_tables_of(c);
+    hipError_t e = fill_zero_dev_count(c, reinterpret_cast<float*>(c->pbound), &c->dc->num_page, kP2GCh, c->cap_p);
+    if (e != hipSuccess) return e;
+    MPMX_TIMED(c, "p2g_prep", (k_p2g_prep<TRANSFER, ST><<<particle_blocks(c), c->particle_wg, 0, c->stream>>>(
+                                   c->dp, st, c->wmap, c->sigma, c->FP, c->vol, t, c->payload, c->pbound, dt,
+                                   (int)c->n, c->dc)));
+    return gather_blocks_t<TRANSFER, ST>(c, dt);
 }
 
 template <typename ST>
@@ -554,6 +443,23 @@ hipError_t launch_p2g(mpmx_ctx* c, float dt) {
         return launch_p2g_scatter(c, dt);
     }
     e = (c->cfg.precision == MPMX_MIXED) ? gather_dispatch(c, c->ms[c->cur], dt) : gather_dispatch(c, c->ps[c->cur], dt);
+    c->grid_fused = true;
+    return e;
+}
+
+// stress and gather P2G of a substep: the stress pass also writes the
+// payload and page bounds, so the P2G's own payload pass is skipped.  Same
+// results as launch_stress then launch_p2g, bit for bit.
+hipError_t launch_stress_p2g(mpmx_ctx* c, float dt) {
+    hipError_t e = launch_cg_reset(c);
+    if (e == hipSuccess)
+        e = fill_zero_dev_count(c, reinterpret_cast<float*>(c->pbound), &c->dc->num_page, kP2GCh, c->cap_p);
+    if (e != hipSuccess) return e;
+    MPMX_TIMED(c, "stress_prep", e = launch_stress_prep(c, dt));
+    if (e != hipSuccess) return e;
+    const bool mixed = c->cfg.precision == MPMX_MIXED;
+    if (c->P.transfer == 0) e = mixed ? gather_blocks_t<0, MState>(c, dt) : gather_blocks_t<0, PState>(c, dt);
+    else e = mixed ? gather_blocks_t<1, MState>(c, dt) : gather_blocks_t<1, PState>(c, dt);
     c->grid_fused = true;
     return e;
 }

@@ -264,7 +264,7 @@ __device__ __forceinline__ void cgq_class(const BlockClasses& S, const DevParams
                     const float hw = gw[0][ii] * w[1][jj] * w[2][kk] * c3[0] +
                                      w[0][ii] * gw[1][jj] * w[2][kk] * c3[1] +
                                      w[0][ii] * w[1][jj] * gw[2][kk] * c3[2];
-                    atomicAdd(&acc[(la * 8 + lb) * kAccZ + lc], to_fix(hw, sc));
+                    atomicAdd(&acc[acc_node(la, lb, lc)], to_fix(hw, sc));
                 }
             }
         }
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t,
                       lz = (o & 1) * 4 + (cell & 3);
             const int64_t nn = (int64_t)G * 64 + cell;
             const float D = grid[(int64_t)G * kPageFloats + 7 * 64 + cell];
-            cgq[nn] = from_fix(acc[(lx * 8 + ly) * kAccZ + lz], e) + cgs[nn] * D;
+            cgq[nn] = from_fix(acc[acc_node(lx, ly, lz)], e) + cgs[nn] * D;
         }
         __syncthreads();
     }

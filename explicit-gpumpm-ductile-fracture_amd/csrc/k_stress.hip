@@ -11,9 +11,8 @@
 // DevParams.
 //
 // One lane per particle; F, sigma component-major (coalesced across lanes).
-#include "mpmx_internal.h"
+#include "payload.h"
 #include "svd3.h"
-#include "precision.h"
 
 namespace mpmx {
 
@@ -335,18 +334,14 @@ __device__ __forceinline__ void stress_store(const MState& s, int i, int n, cons
     s.psi[i] = f2bf(out.psi);
 }
 
-// one thread per particle; every wave holds particles of one scene, whose
-// constants it reads once
+// stress of particle i: reads its F and phase-field state, writes back F,
+// g, alpha, psi and stores sigma (component r + 3 c), FP and V
 template <bool ALL64, typename ST>
-__global__ __launch_bounds__(256) void k_stress(DevParams dp, ST s, WaveMap wm, float* __restrict__ sigma,
-                                                float* __restrict__ FP, float* __restrict__ vol, int n,
-                                                DevCounts* dc) {
-    int i, scene;
-    if (!wave_particle(wm, n, i, scene)) return;
-    const DevParams& sp = dp.scenes[scene];
+__device__ __forceinline__ void stress_particle(const DevParams& sp, const ST& s, int i, int n,
+                                                float* __restrict__ sigma, float* __restrict__ FP,
+                                                float* __restrict__ vol, DevCounts* dc, StressOut& out) {
     StressIn in;
     stress_load(s, i, n, in);
-    StressOut out;
     count_fallbacks(stress_one<ALL64>(sp, in, out), &dc->svd_fallbacks);
     bool finite = true;
 #pragma unroll
@@ -360,6 +355,50 @@ __global__ __launch_bounds__(256) void k_stress(DevParams dp, ST s, WaveMap wm, 
     FP[i] = out.FP;
     vol[i] = out.vol;
     if (!finite) atomicOr(&dc->err, kErrNonFinite);
+}
+
+// one thread per particle; every wave holds particles of one scene, whose
+// constants it reads once
+template <bool ALL64, typename ST>
+__global__ __launch_bounds__(256) void k_stress(DevParams dp, ST s, WaveMap wm, float* __restrict__ sigma,
+                                                float* __restrict__ FP, float* __restrict__ vol, int n,
+                                                DevCounts* dc) {
+    int i, scene;
+    if (!wave_particle(wm, n, i, scene)) return;
+    StressOut out;
+    stress_particle<ALL64>(dp.scenes[scene], s, i, n, sigma, FP, vol, dc, out);
+}
+
+// the stress pass of a substep with the gather P2G: after the stress, the
+// particle's P2G payload record and its page's channel bounds, from the
+// stress still in registers (the same values k_p2g_prep reads back from
+// sigma, FP and V, so the records are bit-identical); pbound is zero on entry.
+// Three waves per SIMD (168 VGPRs, a 20-byte spill): 138 us against 161 us at
+// the two waves the unconstrained allocation (173 VGPRs) allows, 175 us at four.
+template <bool ALL64, int TRANSFER, typename ST>
+__global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, WaveMap wm, float* __restrict__ sigma,
+                                                     float* __restrict__ FP, float* __restrict__ vol, BlockTables t,
+                                                     float* __restrict__ pay, unsigned* __restrict__ pbound, float dt,
+                                                     int n, DevCounts* dc) {
+    int i, scene;
+    const bool valid = wave_particle(wm, n, i, scene);
+    float b[kP2GCh];
+#pragma unroll
+    for (int k = 0; k < kP2GCh; ++k) b[k] = 0.f;
+    uint32_t pk = 0xffffffffu;
+    if (valid) {
+        const DevParams& sp = dp.scenes[scene];
+        StressOut out;
+        stress_particle<ALL64>(sp, s, i, n, sigma, FP, vol, dc, out);
+        pk = t.keys[i] >> 6;
+        float sig[9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) sig[r + 3 * cc] = out.sigma[r][cc];
+        prep_particle<TRANSFER>(sp, s, sig, out.FP, out.vol, i, n, dt, pay, b);
+    }
+    page_bounds(valid, pk, b, t, pbound, dc);
 }
 
 template <bool ALL64>
@@ -376,6 +415,30 @@ static void stress_dispatch(mpmx_ctx* c) {
 hipError_t launch_stress(mpmx_ctx* c) {
     if (c->cfg.stress_fp64) stress_dispatch<true>(c);
     else stress_dispatch<false>(c);
+    return hipGetLastError();
+}
+
+template <bool ALL64, int TRANSFER, typename ST>
+static void stress_prep_dispatch(mpmx_ctx* c, ST st, float dt) {
+    k_stress_prep<ALL64, TRANSFER, ST><<<particle_blocks(c), c->particle_wg, 0, c->stream>>>(
+        c->dp, st, c->wmap, c->sigma, c->FP, c->vol, block_tables_of(c), c->payload, c->pbound, dt, (int)c->n, c->dc);
+}
+
+template <bool ALL64, typename ST>
+static void stress_prep_dispatch(mpmx_ctx* c, ST st, float dt) {
+    if (c->P.transfer == 0) stress_prep_dispatch<ALL64, 0>(c, st, dt);
+    else stress_prep_dispatch<ALL64, 1>(c, st, dt);
+}
+
+template <bool ALL64>
+static void stress_prep_dispatch(mpmx_ctx* c, float dt) {
+    if (c->cfg.precision == MPMX_MIXED) stress_prep_dispatch<ALL64>(c, c->ms[c->cur], dt);
+    else stress_prep_dispatch<ALL64>(c, c->ps[c->cur], dt);
+}
+
+hipError_t launch_stress_prep(mpmx_ctx* c, float dt) {
+    if (c->cfg.stress_fp64) stress_prep_dispatch<true>(c, dt);
+    else stress_prep_dispatch<false>(c, dt);
     return hipGetLastError();
 }
 

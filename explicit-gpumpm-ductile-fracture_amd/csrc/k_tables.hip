@@ -283,7 +283,8 @@ hipError_t build_tables(mpmx_ctx* c) {
     if ((e = scan_exclusive(c, c->bcount, c->bcount, words, nullptr, words, &c->dc->num_blocks)) != hipSuccess)
         return e;
     k_block_keys<<<wb, 256, 0, c->stream>>>(c->pmask, c->nmask, c->bcount, words, c->block_key, c->tp_bound);
-    return hipGetLastError();
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return c->cfg.p2g_variant == 1 ? hipSuccess : launch_block_lists(c);
 }
 
 }  // namespace mpmx

@@ -486,6 +486,9 @@ hipError_t launch_scene_counts(mpmx_ctx* c);
 int particle_blocks(const mpmx_ctx* c);   // 256-thread workgroups of the per-particle kernels
 hipError_t launch_stress(mpmx_ctx* c);
 hipError_t launch_p2g(mpmx_ctx* c, float dt);
+hipError_t launch_stress_prep(mpmx_ctx* c, float dt);
+hipError_t launch_stress_p2g(mpmx_ctx* c, float dt);
+hipError_t launch_block_lists(mpmx_ctx* c);
 hipError_t launch_grid_update(mpmx_ctx* c, float dt);
 hipError_t launch_pcg(mpmx_ctx* c, float dt);
 hipError_t launch_pcg_from(mpmx_ctx* c, int it0, int first_chunk);
