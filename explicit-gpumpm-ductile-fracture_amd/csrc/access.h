@@ -80,6 +80,15 @@ struct BlockClasses {
     int len[8];              // entries per thread of class c
 };
 
+// what the block G2P of a 2x2x2 block needs besides the grid: the grid pages
+// of its staged node region (pages 2b .. 2b+2 per axis, -1 where absent) and
+// the contiguous range [pa, pb) of its own particles
+struct BlockG2P {
+    int page[27];
+    int pa, pb;
+    int pad[3];
+};
+
 // copy of a block's lists between LDS and HBM (every thread of the workgroup)
 constexpr int kClassWords = (int)(sizeof(BlockClasses) / sizeof(int));
 __device__ __forceinline__ void classes_store(const BlockClasses& S, BlockClasses* __restrict__ dst) {
@@ -138,7 +147,7 @@ __device__ __forceinline__ void class_run(int cls, int r, int& c0, int& len) {
 // cell starts, the exclusive prefix over the runs, and the threads of each
 // class of the 256-thread gathers.
 __device__ __forceinline__ void block_lists_wave(const BlockTables& t, uint32_t bkey, int n, int np_raw, int np,
-                                                 int nt, BlockClasses* __restrict__ dst) {
+                                                 int nt, BlockClasses* __restrict__ dst, BlockG2P* __restrict__ g2p) {
     const int lane = threadIdx.x & 63;
     int bc[3];
     unmorton(bkey, bc[0], bc[1], bc[2]);
@@ -159,10 +168,41 @@ __device__ __forceinline__ void block_lists_wave(const BlockTables& t, uint32_t 
             }
         }
         dst->page[lane] = pg;
-    } else if (lane >= 32 && lane < 40) {
+    }
+    // the block's own 8 pages: grid pages, and the particle range of those
+    // that hold particles (consecutive page ids, so one range)
+    {
         const int o = lane - 32;
-        const int id = page_rank((bkey << 3) | (uint32_t)o, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
-        dst->out_page[o] = (id >= 0 && id < nt) ? id : -1;
+        int lo = 0x7fffffff, hi = -1;
+        if (o >= 0 && o < 8) {
+            const int id = page_rank((bkey << 3) | (uint32_t)o, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
+            dst->out_page[o] = (id >= 0 && id < nt) ? id : -1;
+            if (id >= 0 && id < np) lo = hi = id;
+        }
+#pragma unroll
+        for (int w = 32; w > 0; w >>= 1) {
+            lo = min(lo, __shfl_xor(lo, w, 64));
+            hi = max(hi, __shfl_xor(hi, w, 64));
+        }
+        if (lane == 0) {
+            int a = 0, b = 0;
+            if (hi >= 0) {
+                a = t.page2particle[lo];
+                b = t.page2particle[hi + 1];
+                a = a < 0 ? 0 : (a > n ? n : a);
+                b = b < a ? a : (b > n ? n : b);
+            }
+            g2p->pa = a;
+            g2p->pb = b;
+        }
+    }
+    // the G2P region's grid pages: 2b + (0, 1, 2) per axis
+    if (lane < 27) {
+        const int px = 2 * bc[0] + lane / 9, py = 2 * bc[1] + (lane / 3) % 3, pz = 2 * bc[2] + lane % 3;
+        int id = -1;
+        const uint32_t key = scene_page_key(t, px, py, pz, scene);
+        if (key != ~0u) id = page_rank(key, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
+        g2p->page[lane] = (id >= 0 && id < nt) ? id : -1;
     }
     // runs q = lane + 64 k in class order; an absent page's runs are empty
     int excl[3], carry = 0;

@@ -367,10 +367,11 @@ __global__ __launch_bounds__(256) void k_grid_update(DevParams dp, const uint32_
     if (threadIdx.x == 0) part_rho[blockIdx.x] = r;
 }
 
-// source lists of every 2x2x2 block (block_lists_wave), one wave per block;
-// they depend on the page tables only and serve the block P2G and the K s
-// gathers of the substep's PCG
-__global__ __launch_bounds__(256) void k_block_lists(BlockTables t, BlockClasses* __restrict__ lists, int n,
+// source lists and G2P ranges of every 2x2x2 block (block_lists_wave), one
+// wave per block; they depend on the page tables only and serve the block
+// P2G, the K s gathers of the substep's PCG and the block G2P
+__global__ __launch_bounds__(256) void k_block_lists(BlockTables t, BlockClasses* __restrict__ lists,
+                                                     BlockG2P* __restrict__ g2p, int n,
                                                      const DevCounts* __restrict__ dc) {
     const int np_raw = dc->num_page;
     const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
@@ -381,15 +382,15 @@ __global__ __launch_bounds__(256) void k_block_lists(BlockTables t, BlockClasses
     const int per = (int)(blockDim.x >> 6);
     const int nwaves = (int)gridDim.x * per;
     for (int blk = xcd_logical_block() * per + (int)(threadIdx.x >> 6); blk < nb; blk += nwaves)
-        block_lists_wave(t, t.block_key[blk], n, np_raw, np, nt, lists + blk);
+        block_lists_wave(t, t.block_key[blk], n, np_raw, np, nt, lists + blk, g2p + blk);
 }
 
 hipError_t launch_block_lists(mpmx_ctx* c) {
     int g = (block_grid(c, 100) + 3) / 4;
     g = (g + 7) / 8 * 8;
     MPMX_TIMED(c, "block_lists", (k_block_lists<<<g, 256, 0, c->stream>>>(
-                                      block_tables_of(c), static_cast<BlockClasses*>(c->block_lists), (int)c->n,
-                                      c->dc)));
+                                      block_tables_of(c), static_cast<BlockClasses*>(c->block_lists),
+                                      static_cast<BlockG2P*>(c->block_g2p), (int)c->n, c->dc)));
     return hipGetLastError();
 }
 

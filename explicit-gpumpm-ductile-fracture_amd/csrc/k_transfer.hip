@@ -146,7 +146,7 @@ __device__ __forceinline__ void g2p_particle(const DevParams& dp, const ST& s, i
 #pragma unroll
     for (int k = 0; k < 9; ++k) { Bn[k] = 0.f; G[k] = 0.f; }
 #pragma unroll
-    for (int ii = 0; ii < 3; ++ii)
+    for (int ii = 0; ii < 3; ++ii) {
 #pragma unroll
         for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
@@ -173,6 +173,18 @@ __device__ __forceinline__ void g2p_particle(const DevParams& dp, const ST& s, i
                         for (int cc = 0; cc < 3; ++cc) G[r + 3 * cc] += vi[r] * g3[cc];
                 }
             }
+        // one slab of nine nodes at a time: the scheduler would otherwise
+        // hoist all 108 staged-node reads and hold them in registers
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // B and the damage update first: stored before the velocity clamp's
+    // branch, their sums over the 27 nodes are not sunk past it (where their
+    // terms would hold over a hundred registers)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) st_B(s, i, k, n, Bn[k]);
+    const float pc0 = ld_c(s, i);
+    const float nc = pc0 + dcs;
+    st_c(s, i, fmaxf(fminf(pc0, nc), 0.f));
     if (TRANSFER == 0) {
         const float nv = sqrtf(vp3[0] * vp3[0] + vp3[1] * vp3[1] + vp3[2] * vp3[2]);
         if (nv > dp.vmax) {
@@ -180,9 +192,6 @@ __device__ __forceinline__ void g2p_particle(const DevParams& dp, const ST& s, i
             for (int r = 0; r < 3; ++r) vp3[r] = vp3[r] / nv * dp.vmax;
         }
     }
-    const float pc0 = ld_c(s, i);
-    const float nc = pc0 + dcs;
-    st_c(s, i, fmaxf(fminf(pc0, nc), 0.f));
     float xn[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -208,8 +217,6 @@ __device__ __forceinline__ void g2p_particle(const DevParams& dp, const ST& s, i
             finite = finite && isfinite(f);
             st_F(s, i, r + 3 * cc, n, f);
         }
-#pragma unroll
-    for (int k = 0; k < 9; ++k) st_B(s, i, k, n, Bn[k]);
     const float lo = 2.0f * dp.dx, hi = 1.0f - 2.0f * dp.dx;
     if (!(xn[0] >= lo && xn[0] <= hi && xn[1] >= lo && xn[1] <= hi && xn[2] >= lo && xn[2] <= hi))
         atomicOr(&dc->err, isfinite(xn[0] + xn[1] + xn[2]) ? kErrDomain : kErrNonFinite);
@@ -271,21 +278,17 @@ __global__ __launch_bounds__(BLOCK) void k_g2p(DevParams dp, ST s, Tables t, con
 // of one 2x2x2 page block reach (the block's 8^3 nodes and two layers of the
 // +x/+y/+z neighbours) and runs every particle of the block; the block's
 // particles are one contiguous range because its 8 pages are consecutive
-// Morton codes
+// Morton codes.  The region's grid pages and the particle range come with the
+// page tables (BlockG2P).
 constexpr int kBR = 10;                       // staged region edge
 constexpr int kBR3 = kBR * kBR * kBR;
 
 template <int TRANSFER, typename ST>
-__global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTables t, const float* __restrict__ grid,
+__global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTables t,
+                                                   const BlockG2P* __restrict__ info, const float* __restrict__ grid,
                                                    const float* __restrict__ cgx, uint32_t* __restrict__ keys,
                                                    float dt, int n, const SceneCG* __restrict__ scg, DevCounts* dc) {
     __shared__ float tv[4 * kBR3];
-    __shared__ int spg[27];
-    __shared__ int prange[2];
-    const int np_raw = dc->num_page;
-    const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
-    int nt = dc->num_total;
-    nt = nt < t.cap_tp ? nt : (int)t.cap_tp;
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
     FOR_LOGICAL_ITEMS(blk, nb) {
@@ -296,43 +299,12 @@ __global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTabl
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
         const int tid = threadIdx.x;
-        if (tid < 27) {
-            const int px = 2 * bc[0] + tid / 9, py = 2 * bc[1] + (tid / 3) % 3, pz = 2 * bc[2] + tid % 3;
-            int id = -1;
-            const uint32_t key = scene_page_key(t, px, py, pz, scene);
-            if (key != ~0u) id = page_rank(key, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
-            spg[tid] = (id >= 0 && id < nt) ? id : -1;
-        }
-        if (tid >= 64 && tid < 128) {
-            // own particle pages: ids of the block's pages below num_page, min and max
-            const int o = tid - 64;
-            int id = -1;
-            if (o < 8) id = page_rank((bkey << 3) | (uint32_t)o, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
-            const bool own = id >= 0 && id < np;
-            int lo = own ? id : 0x7fffffff, hi = own ? id : -1;
-#pragma unroll
-            for (int w = 32; w > 0; w >>= 1) {
-                lo = min(lo, __shfl_xor(lo, w, 64));
-                hi = max(hi, __shfl_xor(hi, w, 64));
-            }
-            if (o == 0) {
-                int a = 0, b = 0;
-                if (hi >= 0) {
-                    a = t.page2particle[lo];
-                    b = t.page2particle[hi + 1];
-                    a = a < 0 ? 0 : (a > n ? n : a);
-                    b = b < a ? a : (b > n ? n : b);
-                }
-                prange[0] = a;
-                prange[1] = b;
-            }
-        }
-        __syncthreads();
-        const int pa = prange[0], pb = prange[1];
+        const BlockG2P* B = info + blk;
+        const int pa = __builtin_amdgcn_readfirstlane(B->pa), pb = __builtin_amdgcn_readfirstlane(B->pb);
         if (pa < pb) {
             for (int q = tid; q < kBR3; q += 256) {
                 const int X = q / 100, Y = (q / 10) % 10, Z = q % 10;
-                const int G = spg[(X >> 2) * 9 + (Y >> 2) * 3 + (Z >> 2)];
+                const int G = B->page[(X >> 2) * 9 + (Y >> 2) * 3 + (Z >> 2)];
                 float v0 = 0.f, v1 = 0.f, v2 = 0.f, dcn = 0.f;
                 if (G >= 0)
                     g2p_node(sp, grid, cgx, breakdown, (int64_t)G * 64 + (((X & 3) << 4) | ((Y & 3) << 2) | (Z & 3)),
@@ -345,8 +317,8 @@ __global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTabl
             __syncthreads();
             const int org[3] = {8 * bc[0] - sp.origin[0], 8 * bc[1] - sp.origin[1], 8 * bc[2] - sp.origin[2]};
             for (int i = pa + tid; i < pb; i += 256) g2p_particle<TRANSFER, kBR>(sp, s, i, n, dt, org, tv, keys, dc);
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -434,10 +406,10 @@ static void g2p_block_kernel(mpmx_ctx* c, ST st, float dt) {
     const int g = block_grid(c, c->g2p_pct);
     MPMX_TIMED(c, "g2p_block", ({
                    if (c->P.transfer == 0)
-                       k_g2p_block<0, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys, dt,
+                       k_g2p_block<0, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, static_cast<const BlockG2P*>(c->block_g2p), c->grid, c->cg_x, c->keys, dt,
                                                                          (int)c->n, c->scg, c->dc);
                    else
-                       k_g2p_block<1, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys, dt,
+                       k_g2p_block<1, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, static_cast<const BlockG2P*>(c->block_g2p), c->grid, c->cg_x, c->keys, dt,
                                                                          (int)c->n, c->scg, c->dc);
                }));
 }

@@ -211,6 +211,7 @@ struct mpmx_ctx {
     uint32_t* block_key = nullptr;   // [tp_bound] occupied 2x2x2-page blocks, Morton order
     int* cell_start = nullptr;       // [cap_p][65] first particle of each cell of each particle page
     void* block_lists = nullptr;     // [tp_bound] per-block source lists of the block P2G (BlockClasses)
+    void* block_g2p = nullptr;       // [tp_bound] per-block G2P pages and particle range (BlockG2P)
     bool vpages_valid = false;       // adjacency and virtual pages match the current tables
     // launch shapes (mpmx_tuning; mpmx_autotune picks them): none changes results
     int particle_wg = 256;           // threads per workgroup of the per-particle kernels

@@ -11,9 +11,10 @@ namespace mpmx {
 __host__ __device__ __forceinline__ uint16_t f2bf(float f) {
     uint32_t u;
     __builtin_memcpy(&u, &f, 4);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    const uint32_t rne = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    const uint32_t qnan = (u >> 16) | 0x40u;
+    // a select, not a branch: kernels that store many values keep one block
+    return (uint16_t)(((u & 0x7fffffffu) > 0x7f800000u) ? qnan : rne);
 }
 __host__ __device__ __forceinline__ float bf2f(uint16_t h) {
     const uint32_t u = (uint32_t)h << 16;
