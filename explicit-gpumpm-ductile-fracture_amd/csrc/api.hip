@@ -352,7 +352,7 @@ static void free_particles(mpmx_ctx* c) {
     }
     dfree(c->sigma); dfree(c->FP); dfree(c->vol); dfree(c->cg_col); dfree(c->payload); dfree(c->pbound);
     dfree(c->keys); dfree(c->keys_alt); dfree(c->vals); dfree(c->vals_alt); dfree(c->hist);
-    dfree(c->pmask); dfree(c->nmask); dfree(c->wcount); dfree(c->scount);
+    dfree(c->pmask); dfree(c->nmask); dfree(c->pflag); dfree(c->wcount); dfree(c->scount);
     dfree(c->page_key); dfree(c->page2particle); dfree(c->adj); dfree(c->nvp); dfree(c->vpo); dfree(c->target); dfree(c->block_key); dfree(c->bcount); dfree(c->cell_start);
     if (c->block_lists) (void)hipFree(c->block_lists);
     c->block_lists = nullptr;
@@ -665,6 +665,8 @@ mpmx_status mpmx_set_particles_batch(mpmx_ctx* c, const int64_t* counts, const d
     // pages (mask_words from the scene geometry)
     HIPCHK(c, dalloc(&c->pmask, c->mask_words));
     HIPCHK(c, dalloc(&c->nmask, c->mask_words));
+    HIPCHK(c, dalloc(&c->pflag, 8 * c->mask_words));
+    HIPCHK(c, fill_zero(c, c->pflag, 64 * c->mask_words));
     HIPCHK(c, dalloc(&c->wcount, c->mask_words));
     HIPCHK(c, dalloc(&c->scount, c->mask_words));
     int64_t pages_domain = 0, pages_nb = 0;
@@ -690,7 +692,7 @@ mpmx_status mpmx_set_particles_batch(mpmx_ctx* c, const int64_t* counts, const d
     HIPCHK(c, dalloc(&c->vpo, c->cap_p + 1));
     HIPCHK(c, dalloc(&c->target, c->cap_vp));
     int64_t tiles = (256ll * c->sort_tiles + 1023) / 1024;
-    tiles = std::max(tiles, (c->mask_words + 1023) / 1024);
+    tiles = std::max(tiles, 3 * ((c->mask_words + 1023) / 1024));   // the page tables' three word scans
     tiles = std::max(tiles, (c->cap_p + 1023) / 1024);
     tiles = std::max(tiles, (n + 1023) / 1024);
     c->scan_tmp_cap = tiles + 1;

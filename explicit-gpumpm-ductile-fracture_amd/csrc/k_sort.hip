@@ -87,10 +87,9 @@ __device__ __forceinline__ int64_t scan_count(const int* n_dev, int64_t n_host) 
     return n < n_host ? n : n_host;   // n_host is the capacity when n_dev is given
 }
 
-__global__ __launch_bounds__(256) void k_scan_reduce(const int* in, int* sums, const int* n_dev, int64_t n_host) {
-    __shared__ int lds4[4];
-    const int64_t n = scan_count(n_dev, n_host);
-    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+// tile `tile` of a two-launch exclusive scan: its sum
+__device__ __forceinline__ void scan_reduce_tile(const int* in, int* sums, int64_t n, int tile, int* lds4) {
+    const int64_t base = (int64_t)tile * kScanTile;
     int local = 0;
 #pragma unroll
     for (int q = 0; q < kScanPer; ++q) {
@@ -99,19 +98,18 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const int* in, int* sums, c
     }
     int total;
     (void)block_excl_scan256(local, total, lds4);
-    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+    if (threadIdx.x == 0) sums[tile] = total;
 }
 
-__global__ __launch_bounds__(256) void k_scan_apply(const int* in, int* out, const int* __restrict__ sums,
-                                                    const int* n_dev, int64_t n_host, int* total_dev) {
-    __shared__ int lds4[4];
-    const int64_t n = scan_count(n_dev, n_host);
-    // offset of this tile: the sums of the tiles before it
+// tile `tile`: its prefix from the sums of the tiles before it, then the
+// scan of its entries; the last tile writes the total
+__device__ __forceinline__ void scan_apply_tile(const int* in, int* out, const int* __restrict__ sums, int64_t n,
+                                                int tile, int tiles, int* total_dev, int* lds4) {
     int pre = 0;
-    for (int t = threadIdx.x; t < (int)blockIdx.x; t += 256) pre += sums[t];
+    for (int t = threadIdx.x; t < tile; t += 256) pre += sums[t];
     int offset;
     (void)block_excl_scan256(pre, offset, lds4);
-    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPer;
+    const int64_t base = (int64_t)tile * kScanTile + (int64_t)threadIdx.x * kScanPer;
     int v[kScanPer];
     int local = 0;
 #pragma unroll
@@ -128,7 +126,18 @@ __global__ __launch_bounds__(256) void k_scan_apply(const int* in, int* out, con
         if (i < n) out[i] = off;
         off += v[q];
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0 && total_dev) *total_dev = offset + total;
+    if (tile == tiles - 1 && threadIdx.x == 0 && total_dev) *total_dev = offset + total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const int* in, int* sums, const int* n_dev, int64_t n_host) {
+    __shared__ int lds4[4];
+    scan_reduce_tile(in, sums, scan_count(n_dev, n_host), blockIdx.x, lds4);
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const int* in, int* out, const int* __restrict__ sums,
+                                                    const int* n_dev, int64_t n_host, int* total_dev) {
+    __shared__ int lds4[4];
+    scan_apply_tile(in, out, sums, scan_count(n_dev, n_host), blockIdx.x, gridDim.x, total_dev, lds4);
 }
 
 hipError_t scan_exclusive(mpmx_ctx* c, const int* in, int* out, int64_t n_cap, const int* n_dev,
@@ -140,6 +149,29 @@ hipError_t scan_exclusive(mpmx_ctx* c, const int* in, int* out, int64_t n_cap, c
     const int64_t bound = n_dev ? n_cap : n_host;
     k_scan_reduce<<<(int)tiles, 256, 0, c->stream>>>(in, c->scan_tmp, n_dev, bound);
     k_scan_apply<<<(int)tiles, 256, 0, c->stream>>>(in, out, c->scan_tmp, n_dev, bound, total_dev);
+    return hipGetLastError();
+}
+
+// up to three scans of n entries each in the same two launches (in place
+// allowed); array y uses the tile sums at sums + y * tiles
+__global__ __launch_bounds__(256) void k_scan_reduce3(ScanSet3 a, int* sums, int64_t n) {
+    __shared__ int lds4[4];
+    const int y = blockIdx.y;
+    scan_reduce_tile(a.in[y], sums + (int64_t)y * gridDim.x, n, blockIdx.x, lds4);
+}
+__global__ __launch_bounds__(256) void k_scan_apply3(ScanSet3 a, const int* sums, int64_t n) {
+    __shared__ int lds4[4];
+    const int y = blockIdx.y;
+    scan_apply_tile(a.in[y], a.out[y], sums + (int64_t)y * gridDim.x, n, blockIdx.x, gridDim.x, a.total[y], lds4);
+}
+
+hipError_t scan_exclusive3(mpmx_ctx* c, const ScanSet3& a, int k, int64_t n) {
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles < 1) return hipSuccess;
+    if (k * tiles > c->scan_tmp_cap) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)tiles, (unsigned)k);
+    k_scan_reduce3<<<grid, 256, 0, c->stream>>>(a, c->scan_tmp, n);
+    k_scan_apply3<<<grid, 256, 0, c->stream>>>(a, c->scan_tmp, n);
     return hipGetLastError();
 }
 

@@ -22,13 +22,31 @@ __device__ __forceinline__ uint32_t neighbor_key(uint32_t pk, int q) {
     return morton((uint32_t)(x + ((o >> 2) & 1)), (uint32_t)(y + ((o >> 1) & 1)), (uint32_t)(z + (o & 1)));
 }
 
-// markPageBoundary + buildHashMapFromPage + supplementAdjacentPages, as bits
-__global__ void k_mark_pages(const uint32_t* __restrict__ keys, int n, uint64_t* pmask) {
+// markPageBoundary + buildHashMapFromPage + supplementAdjacentPages, as bits:
+// the first particle of each page sets the page's byte flag (a plain store,
+// one writer per byte), then one thread per word packs its 64 flags into the
+// bitmap and clears them for the next build.  Setting the bits directly takes
+// an atomic per page on shared words, which costs more than both passes.
+__global__ void k_mark_pages(const uint32_t* __restrict__ keys, int n, uint8_t* __restrict__ pflag) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t pk = keys[i] >> 6;
     if (i > 0 && (keys[i - 1] >> 6) == pk) return;
-    atomicOr((unsigned long long*)&pmask[pk >> 6], 1ull << (pk & 63u));
+    pflag[pk] = 1;
+}
+__global__ void k_pack_pages(uint64_t* __restrict__ pflag, uint64_t* __restrict__ pmask, int64_t words) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    uint64_t* f = pflag + 8 * w;
+    uint64_t m = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const uint64_t b = f[q];   // flags of pages 8q .. 8q + 7 of the word, one byte each
+#pragma unroll
+        for (int j = 0; j < 8; ++j) m |= ((b >> (8 * j)) & 1ull) << (8 * q + j);
+        f[q] = 0ull;
+    }
+    pmask[w] = m;
 }
 
 // supplementAdjacentPages as a bit dilation: nmask = the particle pages P plus
@@ -82,18 +100,35 @@ __global__ void k_neighbour_mask(DevParams dp, const uint64_t* __restrict__ pmas
     nmask[w] = dilate<2>(b0, b1);
 }
 
+// per word: particle pages, neighbour-only pages and occupied 2x2x2 blocks
+// (one byte of the bitmaps each, see block_keys_of)
 __global__ void k_word_counts(const uint64_t* __restrict__ pmask, const uint64_t* __restrict__ nmask,
-                              int* wcount, int* scount, int64_t words) {
+                              int* wcount, int* scount, int* bcount, int64_t words) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= words) return;
-    const uint64_t P = pmask[w];
+    const uint64_t P = pmask[w], N = nmask[w];
     wcount[w] = __popcll(P);
-    scount[w] = __popcll(nmask[w] & ~P);
+    scount[w] = __popcll(N & ~P);
+    const uint64_t m = P | N;
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c += ((m >> (8 * k)) & 0xffull) != 0ull;
+    bcount[w] = c;
 }
 
-__global__ void k_page_keys(const uint64_t* __restrict__ pmask, const uint64_t* __restrict__ nmask,
-                            const int* __restrict__ wpre, const int* __restrict__ spre, int64_t words,
-                            uint32_t* page_key, int64_t cap_p, int64_t cap_tp, DevCounts* dc) {
+__device__ __forceinline__ void scene_counts_of(const DevParams& dp, int s, const int* __restrict__ wpre,
+                                                const int* __restrict__ spre, SceneCG* __restrict__ scg,
+                                                const DevCounts* __restrict__ dc);
+__device__ __forceinline__ void block_keys_of(uint64_t m, int64_t w, int id, uint32_t* __restrict__ block_key,
+                                              int64_t cap);
+
+// page keys in id order, the block keys (2x2x2 blocks in Morton order) and
+// each scene's page ranges, one thread per bitmap word
+__global__ void k_page_keys(DevParams dp, const uint64_t* __restrict__ pmask, const uint64_t* __restrict__ nmask,
+                            const int* __restrict__ wpre, const int* __restrict__ spre,
+                            const int* __restrict__ bpre, int64_t words, uint32_t* page_key, int64_t cap_p,
+                            int64_t cap_tp, uint32_t* __restrict__ block_key, int64_t cap_blocks,
+                            SceneCG* __restrict__ scg, DevCounts* dc) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int num_page = dc->num_page;
     if (w == 0) {
@@ -101,7 +136,9 @@ __global__ void k_page_keys(const uint64_t* __restrict__ pmask, const uint64_t* 
         dc->num_total = total;
         if (num_page > cap_p || total > cap_tp) atomicOr(&dc->err, kErrCapacity);
     }
+    if (w < dp.num_scenes) scene_counts_of(dp, (int)w, wpre, spre, scg, dc);
     if (w >= words) return;
+    block_keys_of(pmask[w] | nmask[w], w, bpre[w], block_key, cap_blocks);
     uint64_t P = pmask[w];
     int id = wpre[w];
     while (P) {
@@ -193,23 +230,8 @@ __global__ void k_vtargets(int* vpo, int* target, int64_t cap_p, int64_t cap_vp,
 // Morton codes 8b .. 8b+7, i.e. one byte of the page bitmaps, so a block is
 // occupied when that byte of pmask | nmask is non-zero; blocks are numbered in
 // ascending Morton order by a scan of the per-word counts.
-__global__ void k_block_counts(const uint64_t* __restrict__ pmask, const uint64_t* __restrict__ nmask,
-                               int* __restrict__ bcount, int64_t words) {
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= words) return;
-    const uint64_t m = pmask[w] | nmask[w];
-    int c = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) c += ((m >> (8 * k)) & 0xffull) != 0ull;
-    bcount[w] = c;
-}
-__global__ void k_block_keys(const uint64_t* __restrict__ pmask, const uint64_t* __restrict__ nmask,
-                             const int* __restrict__ bpre, int64_t words, uint32_t* __restrict__ block_key,
-                             int64_t cap) {
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= words) return;
-    const uint64_t m = pmask[w] | nmask[w];
-    int id = bpre[w];
+__device__ __forceinline__ void block_keys_of(uint64_t m, int64_t w, int id, uint32_t* __restrict__ block_key,
+                                              int64_t cap) {
 #pragma unroll
     for (int k = 0; k < 8; ++k)
         if ((m >> (8 * k)) & 0xffull) {
@@ -220,10 +242,9 @@ __global__ void k_block_keys(const uint64_t* __restrict__ pmask, const uint64_t*
 
 // page ranges of every scene: its particle pages and neighbour-only pages are
 // the ranks of its slot's words in the two prefix scans
-__global__ void k_scene_counts(DevParams dp, const int* __restrict__ wpre, const int* __restrict__ spre,
-                               SceneCG* __restrict__ scg, const DevCounts* __restrict__ dc) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= dp.num_scenes) return;
+__device__ __forceinline__ void scene_counts_of(const DevParams& dp, int s, const int* __restrict__ wpre,
+                                                const int* __restrict__ spre, SceneCG* __restrict__ scg,
+                                                const DevCounts* __restrict__ dc) {
     const int wshift = dp.slot_shift - 6;
     const int64_t w0 = (int64_t)s << wshift, w1 = (int64_t)(s + 1) << wshift;
     const bool last = s + 1 == dp.num_scenes;
@@ -233,6 +254,11 @@ __global__ void k_scene_counts(DevParams dp, const int* __restrict__ wpre, const
     scg[s].np = p1 - p0;
     scg[s].nfirst = n0;
     scg[s].nn = n1 - n0;
+}
+__global__ void k_scene_counts(DevParams dp, const int* __restrict__ wpre, const int* __restrict__ spre,
+                               SceneCG* __restrict__ scg, const DevCounts* __restrict__ dc) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < dp.num_scenes) scene_counts_of(dp, s, wpre, spre, scg, dc);
 }
 
 hipError_t launch_scene_counts(mpmx_ctx* c) {
@@ -261,28 +287,21 @@ hipError_t build_tables(mpmx_ctx* c) {
     const int n = (int)c->n;
     const int64_t words = c->mask_words;
     hipError_t e;
-    if ((e = fill_zero(c, c->pmask, words * 8)) != hipSuccess) return e;
-    k_mark_pages<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask);
+    k_mark_pages<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, reinterpret_cast<uint8_t*>(c->pflag));
+    k_pack_pages<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pflag, c->pmask, words);
     k_neighbour_mask<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->dp, c->pmask, c->nmask, words);
     k_word_counts<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pmask, c->nmask, c->wcount,
-                                                                      c->scount, words);
-    if ((e = scan_exclusive(c, c->wcount, c->wcount, words, nullptr, words, &c->dc->num_page)) != hipSuccess)
-        return e;
-    if ((e = scan_exclusive(c, c->scount, c->scount, words, nullptr, words, &c->dc->num_supp)) != hipSuccess)
-        return e;
-    if ((e = launch_scene_counts(c)) != hipSuccess) return e;
-    k_page_keys<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pmask, c->nmask, c->wcount, c->scount,
-                                                                    words, c->page_key, c->cap_p, c->cap_tp,
-                                                                    c->dc);
+                                                                      c->scount, c->bcount, words);
+    ScanSet3 sc{{c->wcount, c->scount, c->bcount}, {c->wcount, c->scount, c->bcount},
+                {&c->dc->num_page, &c->dc->num_supp, &c->dc->num_blocks}};
+    if ((e = scan_exclusive3(c, sc, 3, words)) != hipSuccess) return e;
+    k_page_keys<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(
+        c->dp, c->pmask, c->nmask, c->wcount, c->scount, c->bcount, words, c->page_key, c->cap_p, c->cap_tp,
+        c->block_key, c->tp_bound, c->scg, c->dc);
     k_page_heads<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask, c->nmask, c->wcount, c->scount,
                                                           c->page2particle, c->cell_start, c->cap_p, c->dc);
     c->vpages_valid = false;
     if (c->cfg.p2g_variant == 1 && (e = build_vpage_tables(c)) != hipSuccess) return e;
-    const int wb = (int)((words + 255) / 256);
-    k_block_counts<<<wb, 256, 0, c->stream>>>(c->pmask, c->nmask, c->bcount, words);
-    if ((e = scan_exclusive(c, c->bcount, c->bcount, words, nullptr, words, &c->dc->num_blocks)) != hipSuccess)
-        return e;
-    k_block_keys<<<wb, 256, 0, c->stream>>>(c->pmask, c->nmask, c->bcount, words, c->block_key, c->tp_bound);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return c->cfg.p2g_variant == 1 ? hipSuccess : launch_block_lists(c);
 }

@@ -198,6 +198,7 @@ struct mpmx_ctx {
     int64_t mask_words = 0;
     uint64_t* pmask = nullptr;
     uint64_t* nmask = nullptr;
+    uint64_t* pflag = nullptr;   // one byte per page, set by the page's first particle; zero between builds
     int* wcount = nullptr;   // per-word popcounts, then their exclusive scan
     int* scount = nullptr;
     uint32_t* page_key = nullptr;
@@ -477,6 +478,14 @@ __device__ __forceinline__ int64_t scene_block_node(const SceneBlocks& b, int nu
 
 // ---- host-side launch helpers (defined in the .hip files) -------------------
 namespace mpmx {
+// up to three exclusive scans of n entries at once; total[y] (may be null)
+// receives the sum of array y
+struct ScanSet3 {
+    const int* in[3];
+    int* out[3];
+    int* total[3];
+};
+hipError_t scan_exclusive3(mpmx_ctx* c, const ScanSet3& a, int k, int64_t n);
 hipError_t scan_exclusive(mpmx_ctx* c, const int* in, int* out, int64_t n_cap, const int* n_dev,
                           int64_t n_host, int* total_dev);
 hipError_t sort_and_gather(mpmx_ctx* c);
