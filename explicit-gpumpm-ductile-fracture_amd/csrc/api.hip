@@ -811,6 +811,8 @@ mpmx_status mpmx_substep(mpmx_ctx* c, double dt) {
 // afterwards; replays set it themselves.
 static mpmx_status capture_substep(mpmx_ctx* c, double dt, int iters) {
     const int p = c->cur;
+    uint32_t* const keys0 = c->keys;
+    uint32_t* const keys1 = c->keys_alt;
     auto& G = c->graphs[p];
     if (G.a) (void)hipGraphExecDestroy(G.a);
     if (G.b) (void)hipGraphExecDestroy(G.b);
@@ -841,6 +843,8 @@ static mpmx_status capture_substep(mpmx_ctx* c, double dt, int iters) {
     if (ga) (void)hipGraphDestroy(ga);
     if (gb) (void)hipGraphDestroy(gb);
     c->cur = p;
+    c->keys = keys0;
+    c->keys_alt = keys1;
     c->sorted = false;
     if (s != MPMX_OK) {
         if (G.a) (void)hipGraphExecDestroy(G.a);
@@ -868,6 +872,7 @@ static mpmx_status substep_graph(mpmx_ctx* c, double dt) {
     }
     HIPCHK(c, hipGraphLaunch(G.a, c->stream));
     c->cur = 1 - p;
+    if (sort_swaps_keys(c)) std::swap(c->keys, c->keys_alt);   // what the captured sort did
     c->sorted = true;
     c->grid_fused = c->cfg.p2g_variant != 1;
     HIPCHK(c, hipStreamSynchronize(c->stream));   // graph A copied the counters to the host

@@ -58,6 +58,8 @@ struct Snapshot {
     char* mem = nullptr;
     int cur = 0, substeps = 0, cg_spec = 1;
     bool sorted = false;
+    uint32_t* keys = nullptr;       // which key buffer is current (a sort may swap them)
+    uint32_t* keys_alt = nullptr;
 };
 
 hipError_t snapshot_take(mpmx_ctx* c, Snapshot& sn) {
@@ -73,6 +75,8 @@ hipError_t snapshot_take(mpmx_ctx* c, Snapshot& sn) {
         off += (b.second + 255) / 256 * 256;
     }
     sn.cur = c->cur;
+    sn.keys = c->keys;
+    sn.keys_alt = c->keys_alt;
     sn.sorted = c->sorted;
     sn.substeps = c->substeps;
     sn.cg_spec = c->cg_spec;
@@ -88,6 +92,8 @@ hipError_t snapshot_restore(mpmx_ctx* c, const Snapshot& sn) {
         off += (b.second + 255) / 256 * 256;
     }
     c->cur = sn.cur;
+    c->keys = sn.keys;
+    c->keys_alt = sn.keys_alt;
     c->sorted = sn.sorted;
     c->substeps = sn.substeps;
     c->cg_spec = sn.cg_spec;

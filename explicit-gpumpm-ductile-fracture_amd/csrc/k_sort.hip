@@ -8,6 +8,7 @@
 // is a stable radix sort of the Morton key, giving one canonical order: key
 // ascending, ties in previous sorted order.
 #include "mpmx_internal.h"
+#include <utility>
 
 namespace mpmx {
 
@@ -336,20 +337,21 @@ hipError_t sort_and_gather(mpmx_ctx* c) {
         ki = ko;
         vi = vo;
     }
-    // keep the sorted keys in c->keys and the permutation (new -> old) in c->vals
-    hipError_t e = hipSuccess;
-    if (ki != 0)
-        e = hipMemcpyAsync(c->keys, kb[ki], sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, c->stream);
-    if (e == hipSuccess && vi != 0)
-        e = hipMemcpyAsync(c->vals, vb[vi], sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, c->stream);
-    if (e != hipSuccess) return e;
+    // the sorted keys become c->keys (an odd pass count swaps the two buffers,
+    // as sort_swaps_keys says); the permutation (new -> old) feeds the gather
+    // from whichever buffer holds it
+    if (ki != 0) std::swap(c->keys, c->keys_alt);
+    const uint32_t* perm = vi < 0 ? nullptr : vb[vi];
+    if (!perm) return hipErrorInvalidValue;   // no pass ran: key_bits is at least 1
     const int nxt = 1 - c->cur;
     if (c->cfg.precision == MPMX_MIXED)
-        k_gather_mixed<<<(n + 255) / 256, 256, 0, c->stream>>>(c->vals, c->ms[c->cur], c->ms[nxt], n);
+        k_gather_mixed<<<(n + 255) / 256, 256, 0, c->stream>>>(perm, c->ms[c->cur], c->ms[nxt], n);
     else
-        k_gather_fp32<<<(n + 255) / 256, 256, 0, c->stream>>>(c->vals, c->ps[c->cur], c->ps[nxt], n);
+        k_gather_fp32<<<(n + 255) / 256, 256, 0, c->stream>>>(perm, c->ps[c->cur], c->ps[nxt], n);
     c->cur = nxt;
     return hipGetLastError();
 }
+
+bool sort_swaps_keys(const mpmx_ctx* c) { return ((c->key_bits + 7) / 8) % 2 == 1; }
 
 }  // namespace mpmx

@@ -486,6 +486,9 @@ struct ScanSet3 {
     int* total[3];
 };
 hipError_t scan_exclusive3(mpmx_ctx* c, const ScanSet3& a, int k, int64_t n);
+// whether a sort leaves the sorted keys in the other key buffer (swapping
+// c->keys and c->keys_alt)
+bool sort_swaps_keys(const mpmx_ctx* c);
 hipError_t scan_exclusive(mpmx_ctx* c, const int* in, int* out, int64_t n_cap, const int* n_dev,
                           int64_t n_host, int* total_dev);
 hipError_t sort_and_gather(mpmx_ctx* c);
