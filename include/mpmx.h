@@ -205,7 +205,10 @@ mpmx_status mpmx_phase_field(mpmx_ctx* ctx, double dt);
 mpmx_status mpmx_g2p(mpmx_ctx* ctx, double dt);
 
 /* ExplicitTimeIntegrator::integrate (ExplicitTimeIntegrator.cu:881-995):
- * bin_sort, rebuild, stress, p2g, grid_update, phase_field, g2p */
+ * bin_sort, rebuild, stress, p2g, grid_update, phase_field, g2p.  With the
+ * default gather P2G the stress pass also writes the P2G's per-particle
+ * terms (one pass instead of two); the results are bit-identical to calling
+ * the stages one by one. */
 mpmx_status mpmx_substep(mpmx_ctx* ctx, double dt);
 /* nsteps substeps with fixed dt (no host synchronisation in between unless the
  * PCG needs a convergence check) */
