@@ -249,6 +249,10 @@ __device__ __forceinline__ void cgq_class(const BlockClasses& S, const DevParams
             bspline_grad(d0, w[a], gw[a]);
             c3[a] = col[3 * (int64_t)i + a];
         }
+        // col times the block's power-of-two scale (exact); grad w . col is
+        // w_z (gw_x w_y c0 + w_x gw_y c1) + gw_z (w_x w_y c2), whose two
+        // brackets are shared by the three nodes of a row
+        const float cs[3] = {c3[0] * sc, c3[1] * sc, c3[2] * sc};
 #pragma unroll
         for (int ii = 0; ii < 3; ++ii) {
             const int la = l0[0] + ii;
@@ -257,14 +261,14 @@ __device__ __forceinline__ void cgq_class(const BlockClasses& S, const DevParams
             for (int jj = 0; jj < 3; ++jj) {
                 const int lb = l0[1] + jj;
                 if (lb < 0 || lb >= 8) continue;
+                const float rA = fmaf(gw[0][ii] * w[1][jj], cs[0], w[0][ii] * gw[1][jj] * cs[1]);
+                const float rB = w[0][ii] * w[1][jj] * cs[2];
 #pragma unroll
                 for (int kk = 0; kk < 3; ++kk) {
                     const int lc = l0[2] + kk;
                     if (lc < 0 || lc >= 8) continue;
-                    const float hw = gw[0][ii] * w[1][jj] * w[2][kk] * c3[0] +
-                                     w[0][ii] * gw[1][jj] * w[2][kk] * c3[1] +
-                                     w[0][ii] * w[1][jj] * gw[2][kk] * c3[2];
-                    atomicAdd(&acc[acc_node(la, lb, lc)], to_fix(hw, sc));
+                    const float hw = fmaf(w[2][kk], rA, gw[2][kk] * rB);
+                    atomicAdd(&acc[acc_node(la, lb, lc)], to_fix_scaled(hw));
                 }
             }
         }
