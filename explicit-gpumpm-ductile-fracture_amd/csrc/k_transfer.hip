@@ -124,16 +124,29 @@ __global__ __launch_bounds__(BLOCK) void k_p2g(DevParams dp, ST s, const float* 
 // tv[ch R^3 + (a R + b) R + c] (v0, v1, v2 and the damage increment).  MLS
 // velocity clamp, c update, F = (I + dt grad v) F, B, x, the next cell key
 // (with the scene's slot origin) and the error flags.
+// position and damage of a particle, which the block G2P loads ahead of the
+// node staging (F is read after the node loop, where it is needed: held
+// through the loop it would cost a wave per SIMD)
+struct G2PIn {
+    float x[3], c;
+};
+template <typename ST>
+__device__ __forceinline__ void g2p_load(const ST& s, int i, G2PIn& P) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) P.x[a] = ld_x(s, i, a);
+    P.c = ld_c(s, i);
+}
+
 template <int TRANSFER, int R, typename ST>
 __device__ __forceinline__ void g2p_particle(const DevParams& dp, const ST& s, int i, int n, float dt,
-                                             const int (&org)[3], const float* tv, uint32_t* __restrict__ keys,
-                                             DevCounts* dc) {
+                                             const int (&org)[3], const float* tv, const G2PIn& P,
+                                             uint32_t* __restrict__ keys, DevCounts* dc) {
     constexpr int R3 = R * R * R;
     float d0[3], w[3][3], gw[3][3];
     int l[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const float xa = ld_x(s, i, a);
+        const float xa = P.x[a];
         const int b = base_node(xa, dp.N);
         d0[a] = xa * dp.inv_dx - (float)b;
         const int la = b - org[a];
@@ -182,7 +195,7 @@ __device__ __forceinline__ void g2p_particle(const DevParams& dp, const ST& s, i
     // terms would hold over a hundred registers)
 #pragma unroll
     for (int k = 0; k < 9; ++k) st_B(s, i, k, n, Bn[k]);
-    const float pc0 = ld_c(s, i);
+    const float pc0 = P.c;
     const float nc = pc0 + dcs;
     st_c(s, i, fmaxf(fminf(pc0, nc), 0.f));
     if (TRANSFER == 0) {
@@ -196,7 +209,7 @@ __device__ __forceinline__ void g2p_particle(const DevParams& dp, const ST& s, i
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         st_v(s, i, r, vp3[r]);
-        xn[r] = ld_x(s, i, r) + vp3[r] * dt;
+        xn[r] = P.x[r] + vp3[r] * dt;
         st_x(s, i, r, xn[r]);
     }
     float A[9];
@@ -213,7 +226,9 @@ __device__ __forceinline__ void g2p_particle(const DevParams& dp, const ST& s, i
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) {
-            const float f = A[r] * F[3 * cc] + A[r + 3] * F[3 * cc + 1] + A[r + 6] * F[3 * cc + 2];
+            // (A F)_(r,cc) with the fused multiply-adds spelled out, so that
+            // the rounding does not depend on how the compiler pairs them
+            const float f = fmaf(A[r + 6], F[3 * cc + 2], fmaf(A[r], F[3 * cc], A[r + 3] * F[3 * cc + 1]));
             finite = finite && isfinite(f);
             st_F(s, i, r + 3 * cc, n, f);
         }
@@ -268,7 +283,9 @@ __global__ __launch_bounds__(BLOCK) void k_g2p(DevParams dp, ST s, Tables t, con
             int pc[3];
             unmorton(t.page_key[p], pc[0], pc[1], pc[2]);
             const int org[3] = {4 * pc[0], 4 * pc[1], 4 * pc[2]};
-            g2p_particle<TRANSFER, kTile>(dp, s, i, n, dt, org, tv, keys, dc);
+            G2PIn P;
+            g2p_load(s, i, P);
+            g2p_particle<TRANSFER, kTile>(dp, s, i, n, dt, org, tv, P, keys, dc);
         }
         __syncthreads();
     }
@@ -302,6 +319,9 @@ __global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTabl
         const BlockG2P* B = info + blk;
         const int pa = __builtin_amdgcn_readfirstlane(B->pa), pb = __builtin_amdgcn_readfirstlane(B->pb);
         if (pa < pb) {
+            // the thread's first particle, loaded while the region is staged
+            G2PIn P;
+            if (pa + tid < pb) g2p_load(s, pa + tid, P);
             for (int q = tid; q < kBR3; q += 256) {
                 const int X = q / 100, Y = (q / 10) % 10, Z = q % 10;
                 const int G = B->page[(X >> 2) * 9 + (Y >> 2) * 3 + (Z >> 2)];
@@ -316,7 +336,10 @@ __global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTabl
             }
             __syncthreads();
             const int org[3] = {8 * bc[0] - sp.origin[0], 8 * bc[1] - sp.origin[1], 8 * bc[2] - sp.origin[2]};
-            for (int i = pa + tid; i < pb; i += 256) g2p_particle<TRANSFER, kBR>(sp, s, i, n, dt, org, tv, keys, dc);
+            for (int i = pa + tid; i < pb; i += 256) {
+                if (i != pa + tid) g2p_load(s, i, P);
+                g2p_particle<TRANSFER, kBR>(sp, s, i, n, dt, org, tv, P, keys, dc);
+            }
             __syncthreads();
         }
     }
