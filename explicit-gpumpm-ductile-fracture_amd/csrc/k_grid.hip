@@ -74,9 +74,10 @@ struct CGVec {
 
 // PCG scalars start over every substep
 // (per scene: a scene whose energy has no fracture starts done)
-__global__ void k_cg_reset(DevParams dp, SceneCG* __restrict__ scg, DevCounts* dc) {
+// (run by the 64 lanes of one wave)
+__device__ __forceinline__ void cg_reset(const DevParams& dp, SceneCG* __restrict__ scg, DevCounts* dc) {
     int active = 0;
-    for (int s = threadIdx.x; s < dp.num_scenes; s += blockDim.x) {
+    for (int s = threadIdx.x; s < dp.num_scenes; s += 64) {
         const int on = dp.scenes[s].phase_field ? 1 : 0;
         scg[s].done = 1 - on;
         scg[s].iters = 0;
@@ -93,6 +94,19 @@ __global__ void k_cg_reset(DevParams dp, SceneCG* __restrict__ scg, DevCounts* d
         dc->cg_breakdown = 0;
         dc->cg_norm = 0.f;
     }
+}
+__global__ void k_cg_reset(DevParams dp, SceneCG* __restrict__ scg, DevCounts* dc) { cg_reset(dp, scg, dc); }
+
+// the PCG reset (first wave of the first workgroup) and the clear of the
+// per-page channel bounds, in one launch
+__global__ void k_cg_reset_bounds(DevParams dp, SceneCG* __restrict__ scg, DevCounts* dc,
+                                  unsigned* __restrict__ pbound, int64_t cap_p) {
+    if (blockIdx.x == 0 && threadIdx.x < 64) cg_reset(dp, scg, dc);
+    int64_t np = dc->num_page;
+    np = np < cap_p ? np : cap_p;
+    const int64_t m = np * kP2GCh;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (int64_t)gridDim.x * blockDim.x)
+        pbound[q] = 0u;
 }
 
 
@@ -452,9 +466,10 @@ hipError_t launch_p2g(mpmx_ctx* c, float dt) {
 // payload and page bounds, so the P2G's own payload pass is skipped.  Same
 // results as launch_stress then launch_p2g, bit for bit.
 hipError_t launch_stress_p2g(mpmx_ctx* c, float dt) {
-    hipError_t e = launch_cg_reset(c);
-    if (e == hipSuccess)
-        e = fill_zero_dev_count(c, reinterpret_cast<float*>(c->pbound), &c->dc->num_page, kP2GCh, c->cap_p);
+    int64_t blocks = (c->cap_p * kP2GCh + 255) / 256;
+    blocks = blocks < 2048 ? (blocks < 1 ? 1 : blocks) : 2048;
+    k_cg_reset_bounds<<<(int)blocks, 256, 0, c->stream>>>(c->dp, c->scg, c->dc, c->pbound, c->cap_p);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     MPMX_TIMED(c, "stress_prep", e = launch_stress_prep(c, dt));
     if (e != hipSuccess) return e;

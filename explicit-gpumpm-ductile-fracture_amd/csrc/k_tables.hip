@@ -74,7 +74,22 @@ __device__ __forceinline__ uint64_t dilate(uint64_t m, uint64_t prev) {
 // (the page coordinates its domain alone would hold), so a scene of a batch
 // gets the neighbour pages it would get alone, less the ones past the slot,
 // which no particle of the domain reaches.
+// per word: particle pages, neighbour-only pages and occupied 2x2x2 blocks
+// (one byte of the bitmaps each, see block_keys_of)
+__device__ __forceinline__ void word_counts_of(int64_t w, uint64_t P, uint64_t N, int* __restrict__ wcount,
+                                               int* __restrict__ scount, int* __restrict__ bcount) {
+    wcount[w] = __popcll(P);
+    scount[w] = __popcll(N & ~P);
+    const uint64_t m = P | N;
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c += ((m >> (8 * k)) & 0xffull) != 0ull;
+    bcount[w] = c;
+}
+
+// the neighbour bitmap word and the word's counts
 __global__ void k_neighbour_mask(DevParams dp, const uint64_t* __restrict__ pmask, uint64_t* __restrict__ nmask,
+                                 int* __restrict__ wcount, int* __restrict__ scount, int* __restrict__ bcount,
                                  int64_t words) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= words) return;
@@ -85,6 +100,7 @@ __global__ void k_neighbour_mask(DevParams dp, const uint64_t* __restrict__ pmas
     unmorton((uint32_t)local, cx, cy, cz);
     if (cx >= lim || cy >= lim || cz >= lim) {
         nmask[w] = 0ull;
+        word_counts_of(w, pmask[w], 0ull, wcount, scount, bcount);
         return;
     }
     const int64_t slot_base = w - local;
@@ -97,24 +113,11 @@ __global__ void k_neighbour_mask(DevParams dp, const uint64_t* __restrict__ pmas
     const uint64_t a00 = dilate<0>(P(0, 0, 0), P(0, 0, 1)), a01 = dilate<0>(P(0, 1, 0), P(0, 1, 1));
     const uint64_t a10 = dilate<0>(P(1, 0, 0), P(1, 0, 1)), a11 = dilate<0>(P(1, 1, 0), P(1, 1, 1));
     const uint64_t b0 = dilate<1>(a00, a01), b1 = dilate<1>(a10, a11);
-    nmask[w] = dilate<2>(b0, b1);
+    const uint64_t N = dilate<2>(b0, b1);
+    nmask[w] = N;
+    word_counts_of(w, pmask[w], N, wcount, scount, bcount);
 }
 
-// per word: particle pages, neighbour-only pages and occupied 2x2x2 blocks
-// (one byte of the bitmaps each, see block_keys_of)
-__global__ void k_word_counts(const uint64_t* __restrict__ pmask, const uint64_t* __restrict__ nmask,
-                              int* wcount, int* scount, int* bcount, int64_t words) {
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= words) return;
-    const uint64_t P = pmask[w], N = nmask[w];
-    wcount[w] = __popcll(P);
-    scount[w] = __popcll(N & ~P);
-    const uint64_t m = P | N;
-    int c = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) c += ((m >> (8 * k)) & 0xffull) != 0ull;
-    bcount[w] = c;
-}
 
 __device__ __forceinline__ void scene_counts_of(const DevParams& dp, int s, const int* __restrict__ wpre,
                                                 const int* __restrict__ spre, SceneCG* __restrict__ scg,
@@ -289,9 +292,8 @@ hipError_t build_tables(mpmx_ctx* c) {
     hipError_t e;
     k_mark_pages<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, reinterpret_cast<uint8_t*>(c->pflag));
     k_pack_pages<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pflag, c->pmask, words);
-    k_neighbour_mask<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->dp, c->pmask, c->nmask, words);
-    k_word_counts<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pmask, c->nmask, c->wcount,
-                                                                      c->scount, c->bcount, words);
+    k_neighbour_mask<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->dp, c->pmask, c->nmask, c->wcount,
+                                                                         c->scount, c->bcount, words);
     ScanSet3 sc{{c->wcount, c->scount, c->bcount}, {c->wcount, c->scount, c->bcount},
                 {&c->dc->num_page, &c->dc->num_supp, &c->dc->num_blocks}};
     if ((e = scan_exclusive3(c, sc, 3, words)) != hipSuccess) return e;
