@@ -534,11 +534,15 @@ mpmx_status mpmx_create_batch(int num_scenes, const mpmx_params* ps, const mpmx_
         return MPMX_ERR_HIP;
     }
     if (hipMalloc((void**)&c->dc, sizeof(DevCounts)) != hipSuccess ||
-        hipHostMalloc((void**)&c->hc, sizeof(DevCounts), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&c->hc, sizeof(DevCounts), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&c->pub, sizeof(DevCounts), hipHostMallocCoherent) != hipSuccess) {
+        if (c->dc) (void)hipFree(c->dc);
+        if (c->hc) (void)hipHostFree(c->hc);
         delete c;
         return MPMX_ERR_HIP;
     }
     std::memset(c->hc, 0, sizeof(DevCounts));
+    std::memset(c->pub, 0, sizeof(DevCounts));
     (void)hipMemcpy(c->dc, c->hc, sizeof(DevCounts), hipMemcpyHostToDevice);
     if (hipMalloc((void**)&c->scene_dp, sizeof(DevParams) * num_scenes) != hipSuccess ||
         hipMalloc((void**)&c->scg, sizeof(SceneCG) * num_scenes) != hipSuccess ||
@@ -568,6 +572,7 @@ mpmx_status mpmx_destroy(mpmx_ctx* c) {
     dfree(c->wave_buf);
     if (c->dc) (void)hipFree(c->dc);
     if (c->hc) (void)hipHostFree(c->hc);
+    if (c->pub) (void)hipHostFree(c->pub);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MPMX_OK;

@@ -29,6 +29,8 @@
 // solve: its own 128-node blocks, partials, scalars and stopping test
 // (SceneBlocks); the batch iterates until every scene has stopped.
 #include "access.h"
+#include <chrono>
+#include <cstring>
 
 namespace mpmx {
 
@@ -547,9 +549,7 @@ hipError_t launch_pcg_from(mpmx_ctx* c, int it0, int first_chunk) {
         it += chunk;
         // read the convergence flag and the counters (the block count sizes
         // the next launches of the block kernels)
-        hipError_t e = hipMemcpyAsync(c->hc, c->dc, sizeof(DevCounts), hipMemcpyDeviceToHost, c->stream);
-        if (e != hipSuccess) return e;
-        e = hipStreamSynchronize(c->stream);
+        hipError_t e = counts_to_host(c);
         if (e != hipSuccess) return e;
         if (c->hc->cg_done || it >= maxit) break;
         chunk = every;
@@ -570,6 +570,44 @@ hipError_t queue_pcg_iterations(mpmx_ctx* c, int k0, int k1) {
         hipError_t e = pcg_iteration(c, k);
         if (e != hipSuccess) return e;
     }
+    return hipSuccess;
+}
+
+// ---- counters to the host without a stream synchronisation -----------------
+// One wave copies the 31 counter words to coherent host memory, makes them
+// visible system-wide, then writes the sequence number the host waits for.
+__global__ void k_publish_counts(const DevCounts* __restrict__ dc, DevCounts* pub, unsigned seq) {
+    constexpr int kWords = (int)(sizeof(DevCounts) / sizeof(unsigned));
+    const unsigned* src = reinterpret_cast<const unsigned*>(dc);
+    unsigned* dst = reinterpret_cast<unsigned*>(pub);
+    const int lane = threadIdx.x;
+    if (lane < kWords - 1) __hip_atomic_store(&dst[lane], src[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __threadfence_system();
+    if (lane == 0) __hip_atomic_store(&pub->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+hipError_t counts_to_host(mpmx_ctx* c) {
+    const unsigned want = ++c->pub_seq;
+    k_publish_counts<<<1, 64, 0, c->stream>>>(c->dc, c->pub, want);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // the copy lands when every earlier kernel of the stream has finished;
+    // past two seconds (a stalled stream, or a launch that failed late) the
+    // stream synchronisation below reports what happened
+    const auto t0 = std::chrono::steady_clock::now();
+    bool seen = false;
+    for (int k = 0;; ++k) {
+        if (__atomic_load_n(&c->pub->seq, __ATOMIC_ACQUIRE) == want) {
+            seen = true;
+            break;
+        }
+        if ((k & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) break;
+    }
+    if (!seen) {
+        if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return e;
+        if (__atomic_load_n(&c->pub->seq, __ATOMIC_ACQUIRE) != want) return hipErrorUnknown;
+    }
+    std::memcpy(c->hc, c->pub, sizeof(DevCounts));
     return hipSuccess;
 }
 

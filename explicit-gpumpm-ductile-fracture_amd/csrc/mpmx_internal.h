@@ -113,7 +113,8 @@ struct DevCounts {
     int svd_fallbacks;
     int num_blocks;     // occupied 2x2x2-page blocks
     int cg_active;      // scenes whose PCG is still running
-    int pad[19];
+    int pad[18];
+    unsigned seq;       // host-published copies: written last, names the copy
 };
 
 struct PState {
@@ -248,6 +249,8 @@ struct mpmx_ctx {
     // counters
     mpmx::DevCounts* dc = nullptr;
     mpmx::DevCounts* hc = nullptr;   // pinned host mirror
+    mpmx::DevCounts* pub = nullptr;  // coherent pinned copy the device publishes to (counts_to_host)
+    unsigned pub_seq = 0;
     // scans
     int* scan_tmp = nullptr;
     int64_t scan_tmp_cap = 0;
@@ -502,6 +505,10 @@ hipError_t launch_p2g(mpmx_ctx* c, float dt);
 hipError_t launch_stress_prep(mpmx_ctx* c, float dt);
 hipError_t launch_stress_p2g(mpmx_ctx* c, float dt);
 hipError_t launch_block_lists(mpmx_ctx* c);
+// the device counters into c->hc once the work queued so far has finished:
+// a one-wave kernel copies them into coherent host memory, the host waits
+// for its sequence number (no stream synchronisation)
+hipError_t counts_to_host(mpmx_ctx* c);
 hipError_t launch_grid_update(mpmx_ctx* c, float dt);
 hipError_t launch_pcg(mpmx_ctx* c, float dt);
 hipError_t launch_pcg_from(mpmx_ctx* c, int it0, int first_chunk);
