@@ -359,14 +359,14 @@ static void free_particles(mpmx_ctx* c) {
     if (c->block_g2p) (void)hipFree(c->block_g2p);
     c->block_g2p = nullptr;
     dfree(c->grid); dfree(c->grid_p2g);
-    dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q); dfree(c->cg_part);
+    dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q); dfree(c->cg_part); dfree(c->cg_amask);
     dfree(c->scan_tmp);
 }
 
 static mpmx_status alloc_grid(mpmx_ctx* c, int64_t cap_tp) {
     destroy_graphs(c);
     dfree(c->grid); dfree(c->grid_p2g);
-    dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q); dfree(c->cg_part);
+    dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q); dfree(c->cg_part); dfree(c->cg_amask);
     c->cap_tp = cap_tp;
     // workgroups of the node-space kernels: about four 128-node blocks each at
     // capacity, at most 4 per CU, a multiple of 8 (one share per XCD)
@@ -418,6 +418,7 @@ static mpmx_status alloc_grid(mpmx_ctx* c, int64_t cap_tp) {
     HIPCHK(c, dalloc(&c->cg_z, cap_tp * 64));
     HIPCHK(c, dalloc(&c->cg_s, cap_tp * 64));
     HIPCHK(c, dalloc(&c->cg_q, cap_tp * 64));
+    HIPCHK(c, dalloc(&c->cg_amask, cap_tp));
     HIPCHK(c, fill_zero(c, c->grid, cap_tp * kPageFloats * 4));
     HIPCHK(c, fill_zero(c, c->cg_x, cap_tp * 64 * 4));
     return MPMX_OK;

@@ -70,6 +70,7 @@ struct CGVec {
     float* z;
     float* s;
     float* q;
+    uint64_t* amask;   // per grid page: bit c set when node c has sum w > 0 (block P2G only)
 };
 
 // PCG scalars start over every substep
@@ -297,7 +298,9 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
                 cg.r[nn] = r;
                 cg.z[nn] = z;
                 cg.s[nn] = z;
-                cg.q[nn] = 0.f;
+                // the wave holds the 64 nodes of page G: their activity bits
+                const uint64_t act = __ballot(ch[5] > 0.f);
+                if (cell == 0) cg.amask[G] = act;
             }
         }
         __syncthreads();
@@ -415,7 +418,7 @@ hipError_t launch_cg_reset(mpmx_ctx* c) {
     return hipGetLastError();
 }
 
-static CGVec cg_vec(mpmx_ctx* c) { return CGVec{c->cg_x, c->cg_r, c->cg_z, c->cg_s, c->cg_q}; }
+static CGVec cg_vec(mpmx_ctx* c) { return CGVec{c->cg_x, c->cg_r, c->cg_z, c->cg_s, c->cg_q, c->cg_amask}; }
 
 // block P2G (with grid update and PCG set-up) over the payload and page bounds
 template <int TRANSFER, typename ST>

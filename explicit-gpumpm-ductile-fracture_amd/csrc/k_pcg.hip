@@ -393,6 +393,7 @@ __global__ __launch_bounds__(256) void k_cg_update(DevParams dp, const float* __
                                                    float* __restrict__ cgq, float* __restrict__ part, int nb, int k,
                                                    int64_t cap_tp, const SceneCG* __restrict__ scg,
                                                    const int* __restrict__ wg_scene,
+                                                   const uint64_t* __restrict__ amask, int zero_q,
                                                    const DevCounts* __restrict__ dc) {
     __shared__ float lds[4];
     const SceneBlocks sb = scene_blocks(scg, wg_scene);
@@ -408,10 +409,11 @@ __global__ __launch_bounds__(256) void k_cg_update(DevParams dp, const float* __
         float rz = 0.f, rmax = 0.f;
         if (n0 >= 0) {
             const float* gp = grid + (n0 >> 6) * kPageFloats + (n0 & 63);
+            const uint64_t am = amask ? amask[n0 >> 6] >> (n0 & 63) : 0ull;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int64_t nn = n0 + h;
-                if (gp[5 * 64 + h] > 0.f) {
+                if (amask ? ((am >> h) & 1ull) != 0ull : gp[5 * 64 + h] > 0.f) {
                     const float D = gp[7 * 64 + h];
                     cgx[nn] += cgs[nn] * alpha;
                     const float r = cgr[nn] - cgq[nn] * alpha;
@@ -424,7 +426,7 @@ __global__ __launch_bounds__(256) void k_cg_update(DevParams dp, const float* __
                     rmax = fmaxf(rmax, fabsf(r));
                     rz += r * z;
                 }
-                cgq[nn] = 0.f;
+                if (zero_q) cgq[nn] = 0.f;   // the page-tile K s adds into q; the block one writes it whole
             }
         }
         const float bs = wave_sum(rz);
@@ -447,7 +449,8 @@ __global__ __launch_bounds__(256) void k_cg_direction(DevParams dp, const float*
                                                       const float* __restrict__ cgz, float* __restrict__ cgs,
                                                       const float* __restrict__ part, int nb, int k,
                                                       int64_t cap_tp, SceneCG* __restrict__ scg,
-                                                      const int* __restrict__ wg_scene, DevCounts* dc) {
+                                                      const int* __restrict__ wg_scene,
+                                                      const uint64_t* __restrict__ amask, DevCounts* dc) {
     __shared__ float lds[4];
     const SceneBlocks sb = scene_blocks(scg, wg_scene);
     if (scene_stopped(scg, sb.scene)) return;
@@ -476,9 +479,11 @@ __global__ __launch_bounds__(256) void k_cg_direction(DevParams dp, const float*
         const int64_t n0 = scene_block_node(sb, num_page, cap_tp, blk);
         if (n0 >= 0) {
             const float* gp = grid + (n0 >> 6) * kPageFloats + (n0 & 63);
+            const uint64_t am = amask ? amask[n0 >> 6] >> (n0 & 63) : 0ull;
 #pragma unroll
             for (int h = 0; h < 2; ++h)
-                if (gp[5 * 64 + h] > 0.f) cgs[n0 + h] = cgs[n0 + h] * beta + cgz[n0 + h];
+                if (amask ? ((am >> h) & 1ull) != 0ull : gp[5 * 64 + h] > 0.f)
+                    cgs[n0 + h] = cgs[n0 + h] * beta + cgz[n0 + h];
         }
     }
 }
@@ -519,14 +524,17 @@ static hipError_t pcg_iteration(mpmx_ctx* c, int k) {
         MPMX_TIMED(c, "cg_apply", (c->cfg.precision == MPMX_MIXED) ? apply_block(c, c->ms[c->cur])
                                                                    : apply_block(c, c->ps[c->cur]));
     const int nb = c->cg_nb;
+    // the block P2G records which nodes are active; the page-tile variant's
+    // grid update does not, and the kernels then test sum w themselves
+    const uint64_t* amask = blocks ? c->cg_amask : nullptr;
     MPMX_TIMED(c, "cg_vector", ({
                    k_cg_dot<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_s, c->cg_q, c->cg_part,
                                                        blocks ? 0 : 1, c->cap_tp, c->scg, c->wg_scene, c->dc);
                    k_cg_update<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_x, c->cg_r, c->cg_z, c->cg_s,
                                                           c->cg_q, c->cg_part, nb, k, c->cap_tp, c->scg,
-                                                          c->wg_scene, c->dc);
+                                                          c->wg_scene, amask, blocks ? 0 : 1, c->dc);
                    k_cg_direction<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_z, c->cg_s, c->cg_part, nb, k,
-                                                             c->cap_tp, c->scg, c->wg_scene, c->dc);
+                                                             c->cap_tp, c->scg, c->wg_scene, amask, c->dc);
                }));
     return hipGetLastError();
 }

@@ -240,6 +240,7 @@ struct mpmx_ctx {
     float* cg_s = nullptr;
     float* cg_q = nullptr;
     float* cg_part = nullptr;    // [4][cg_nb] per-workgroup partials
+    uint64_t* cg_amask = nullptr;   // [cap_tp] active-node bits per grid page (block P2G)
     float* cg_col = nullptr;     // [3n] per-particle M V dx^2 grad s (block K s)
     float* payload = nullptr;    // [28n] per-particle P2G terms (block P2G)
     unsigned* pbound = nullptr;  // [cap_p][8] per-page largest contribution per channel
