@@ -418,10 +418,12 @@ __global__ __launch_bounds__(256) void k_cg_update(DevParams dp, const float* __
                     cgx[nn] += cgs[nn] * alpha;
                     const float r = cgr[nn] - cgq[nn] * alpha;
                     cgr[nn] = r;
-                    float z = cgz[nn];
+                    float z;
                     if (!(fabsf(D) < 1e-30f)) {
                         z = r / D;
                         cgz[nn] = z;
+                    } else {
+                        z = cgz[nn];   // read only where it is kept
                     }
                     rmax = fmaxf(rmax, fabsf(r));
                     rz += r * z;
