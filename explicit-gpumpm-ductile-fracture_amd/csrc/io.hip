@@ -282,22 +282,26 @@ mpmx_status mpmx_autotune(mpmx_ctx* c, double dt, int trial_steps, mpmx_tuning* 
                 if (std::strcmp(kt[(size_t)q].name, name) == 0) return kt[(size_t)q].total_ms;
             return 0.0;
         };
-        t_part[k] = ms("stress") + ms("p2g_prep");
+        // per-particle kernels: the fused stress + payload pass of a substep
+        // (the stage-by-stage names cover contexts run stage by stage)
+        t_part[k] = ms("stress_prep") + ms("stress") + ms("p2g_prep");
         t_p2g[k] = ms("p2g_block");
         t_g2p[k] = ms("g2p_block");
         t_cg[k] = ms("cg_col") + ms("cg_q");
     }
-    auto argmin = [](const double* v) {
+    // a family none of whose timers ran keeps its default
+    auto argmin = [](const double* v, int dflt) {
+        if (!(v[0] > 0.0) && !(v[1] > 0.0) && !(v[2] > 0.0)) return dflt;
         int b = 0;
         for (int k = 1; k < 3; ++k)
             if (v[k] < v[b]) b = k;
         return b;
     };
     if (st == MPMX_OK) {
-        c->particle_wg = wgs[argmin(t_part)];
-        c->p2g_pct = pcts[argmin(t_p2g)];
-        c->g2p_pct = pcts[argmin(t_g2p)];
-        c->cg_pct = pcts[argmin(t_cg)];
+        c->particle_wg = wgs[argmin(t_part, 2)];   // default 256 threads
+        c->p2g_pct = pcts[argmin(t_p2g, 1)];       // default 100 %
+        c->g2p_pct = pcts[argmin(t_g2p, 1)];
+        c->cg_pct = pcts[argmin(t_cg, 1)];
     } else {
         c->particle_wg = 256;
         c->p2g_pct = c->g2p_pct = c->cg_pct = 100;
