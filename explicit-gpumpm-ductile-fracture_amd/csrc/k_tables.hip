@@ -75,7 +75,7 @@ __device__ __forceinline__ uint64_t dilate(uint64_t m, uint64_t prev) {
 // gets the neighbour pages it would get alone, less the ones past the slot,
 // which no particle of the domain reaches.
 // per word: particle pages, neighbour-only pages and occupied 2x2x2 blocks
-// (one byte of the bitmaps each, see block_keys_of)
+// (one byte of the bitmaps each: the 8 pages 8b .. 8b+7 of block b)
 __device__ __forceinline__ void word_counts_of(int64_t w, uint64_t P, uint64_t N, int* __restrict__ wcount,
                                                int* __restrict__ scount, int* __restrict__ bcount) {
     wcount[w] = __popcll(P);
@@ -122,8 +122,6 @@ __global__ void k_neighbour_mask(DevParams dp, const uint64_t* __restrict__ pmas
 __device__ __forceinline__ void scene_counts_of(const DevParams& dp, int s, const int* __restrict__ wpre,
                                                 const int* __restrict__ spre, SceneCG* __restrict__ scg,
                                                 const DevCounts* __restrict__ dc);
-__device__ __forceinline__ void block_keys_of(uint64_t m, int64_t w, int id, uint32_t* __restrict__ block_key,
-                                              int64_t cap);
 
 // page keys in id order, the block keys (2x2x2 blocks in Morton order) and
 // each scene's page ranges, one thread per bitmap word
@@ -132,31 +130,41 @@ __global__ void k_page_keys(DevParams dp, const uint64_t* __restrict__ pmask, co
                             const int* __restrict__ bpre, int64_t words, uint32_t* page_key, int64_t cap_p,
                             int64_t cap_tp, uint32_t* __restrict__ block_key, int64_t cap_blocks,
                             SceneCG* __restrict__ scg, DevCounts* dc) {
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // one lane per bit of a word (a wave per word): a set bit's id is the
+    // word's prefix plus the set bits below it.  The 2x2x2-page blocks of the
+    // gather transfer: the 8 pages of block b are the Morton codes 8b .. 8b+7,
+    // one byte of the bitmaps, so a block is occupied when that byte of
+    // pmask | nmask is non-zero, and blocks are numbered in ascending Morton
+    // order by the scan of the per-word block counts.
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t w = t >> 6;
+    const int b = (int)(t & 63);
     const int num_page = dc->num_page;
-    if (w == 0) {
+    if (t == 0) {
         const int total = num_page + dc->num_supp;
         dc->num_total = total;
         if (num_page > cap_p || total > cap_tp) atomicOr(&dc->err, kErrCapacity);
     }
-    if (w < dp.num_scenes) scene_counts_of(dp, (int)w, wpre, spre, scg, dc);
+    if (t < dp.num_scenes) scene_counts_of(dp, (int)t, wpre, spre, scg, dc);
     if (w >= words) return;
-    block_keys_of(pmask[w] | nmask[w], w, bpre[w], block_key, cap_blocks);
-    uint64_t P = pmask[w];
-    int id = wpre[w];
-    while (P) {
-        const int b = __ffsll((unsigned long long)P) - 1;
+    const uint64_t P = pmask[w], N = nmask[w];
+    const uint64_t below = b == 0 ? 0ull : (~0ull >> (64 - b));
+    if ((P >> b) & 1ull) {
+        const int id = wpre[w] + __popcll(P & below);
         if (id < cap_p && id < cap_tp) page_key[id] = (uint32_t)(w * 64 + b);
-        ++id;
-        P &= P - 1;
     }
-    uint64_t S = nmask[w] & ~pmask[w];
-    id = num_page + spre[w];
-    while (S) {
-        const int b = __ffsll((unsigned long long)S) - 1;
+    const uint64_t S = N & ~P;
+    if ((S >> b) & 1ull) {
+        const int id = num_page + spre[w] + __popcll(S & below);
         if (id < cap_tp) page_key[id] = (uint32_t)(w * 64 + b);
-        ++id;
-        S &= S - 1;
+    }
+    if (b < 8) {
+        const uint64_t m = P | N;
+        if ((m >> (8 * b)) & 0xffull) {
+            int id = bpre[w];
+            for (int k = 0; k < b; ++k) id += ((m >> (8 * k)) & 0xffull) != 0ull;
+            if (id < cap_blocks) block_key[id] = (uint32_t)(w * 8 + b);
+        }
     }
 }
 
@@ -229,20 +237,6 @@ __global__ void k_vtargets(int* vpo, int* target, int64_t cap_p, int64_t cap_vp,
     for (int v = b; v < e && v < cap_vp; ++v) target[v] = p;
 }
 
-// 2x2x2-page blocks for the gather transfer.  The 8 pages of block b are the
-// Morton codes 8b .. 8b+7, i.e. one byte of the page bitmaps, so a block is
-// occupied when that byte of pmask | nmask is non-zero; blocks are numbered in
-// ascending Morton order by a scan of the per-word counts.
-__device__ __forceinline__ void block_keys_of(uint64_t m, int64_t w, int id, uint32_t* __restrict__ block_key,
-                                              int64_t cap) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-        if ((m >> (8 * k)) & 0xffull) {
-            if (id < cap) block_key[id] = (uint32_t)(w * 8 + k);
-            ++id;
-        }
-}
-
 // page ranges of every scene: its particle pages and neighbour-only pages are
 // the ranks of its slot's words in the two prefix scans
 __device__ __forceinline__ void scene_counts_of(const DevParams& dp, int s, const int* __restrict__ wpre,
@@ -297,7 +291,7 @@ hipError_t build_tables(mpmx_ctx* c) {
     ScanSet3 sc{{c->wcount, c->scount, c->bcount}, {c->wcount, c->scount, c->bcount},
                 {&c->dc->num_page, &c->dc->num_supp, &c->dc->num_blocks}};
     if ((e = scan_exclusive3(c, sc, 3, words)) != hipSuccess) return e;
-    k_page_keys<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(
+    k_page_keys<<<(int)((words * 64 + 255) / 256), 256, 0, c->stream>>>(
         c->dp, c->pmask, c->nmask, c->wcount, c->scount, c->bcount, words, c->page_key, c->cap_p, c->cap_tp,
         c->block_key, c->tp_bound, c->scg, c->dc);
     k_page_heads<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask, c->nmask, c->wcount, c->scount,
