@@ -56,8 +56,9 @@ __device__ __forceinline__ uint32_t scene_page_key(const BlockTables& t, int px,
 // which node offsets can fall outside the block.  Each source page is listed
 // as runs of its passing cells (cell = i<<4 | j<<2 | k, i, j, k in 0..3), with
 // the particle ranges from the page's cell starts, so every listed particle
-// reaches the block: a workgroup runs one class at a time over exactly its
-// particles, and every wave of a class phase takes the same branches.
+// reaches the block; a workgroup's threads are split among the classes in
+// proportion to their entries, so that most waves hold particles of one class
+// and take the same branches.
 // class:             0  1(z) 2(y) 3(yz) 4(x) 5(xz) 6(xy) 7(xyz)
 // pages:             8  4    4    2     4    2     2     1
 // runs per page:     1  16   4    8     1    8     2     4
@@ -89,13 +90,8 @@ struct BlockG2P {
     int pad[3];
 };
 
-// copy of a block's lists between LDS and HBM (every thread of the workgroup)
+// copy of a block's lists from HBM into LDS (every thread of the workgroup)
 constexpr int kClassWords = (int)(sizeof(BlockClasses) / sizeof(int));
-__device__ __forceinline__ void classes_store(const BlockClasses& S, BlockClasses* __restrict__ dst) {
-    const int* a = reinterpret_cast<const int*>(&S);
-    int* b = reinterpret_cast<int*>(dst);
-    for (int q = threadIdx.x; q < kClassWords; q += blockDim.x) b[q] = a[q];
-}
 __device__ __forceinline__ void classes_load(const BlockClasses* __restrict__ src, BlockClasses& S) {
     const int* a = reinterpret_cast<const int*>(src);
     int* b = reinterpret_cast<int*>(&S);

@@ -13,13 +13,14 @@
 // nodes of P and its +1 neighbours, so block b receives from the 27 particle
 // pages 2b-1 .. 2b+1 (per axis); of a page below the block along an axis only
 // the particles whose cell has index & 3 >= 2 on that axis reach it.  The
-// workgroup walks the concatenated particle ranges of those pages twice: once
-// for the largest contribution per channel (the fixed-point scale), once to
-// add every contribution to the block's nodes in LDS with 64-bit integer
-// atomics.  It then updates its nodes in place and writes them once: no
-// global atomics, no zero fill of the grid, and sums that do not depend on
-// the order of the additions.  Each particle is read by the one to eight
-// blocks it reaches (about twice on average).
+// workgroup loads the block's source lists (runs of passing cells, built with
+// the page tables by k_block_lists), takes the fixed-point scale of each
+// channel from the per-page bounds of its 27 source pages (written with the
+// payload), and adds every listed particle's contributions to the block's
+// nodes in LDS with 64-bit integer atomics.  It then updates its nodes in
+// place and writes them once: no global atomics, no zero fill of the grid,
+// and sums that do not depend on the order of the additions.  Each particle
+// is read by the one to eight blocks it reaches (about twice on average).
 #include "payload.h"
 
 namespace mpmx {

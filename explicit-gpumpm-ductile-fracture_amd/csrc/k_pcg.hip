@@ -280,8 +280,8 @@ __device__ __forceinline__ void cgq_class(const BlockClasses& S, const DevParams
 // q_i = sum_p grad w_ip . col_p + D_i s_i over the 8^3 nodes of a block, as a
 // gather with 64-bit fixed-point LDS atomics (|grad w . col| <= sum |col|
 // gives the scale); q is written whole (no zero fill, no global atomics).
-// The block's source lists are the ones the block P2G of this substep built
-// (same block numbering), read from HBM.
+// The block's source lists are the ones the block P2G reads (built with the
+// page tables, same block numbering).
 template <typename ST>
 __global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t,
                                               const BlockClasses* __restrict__ lists, const float* __restrict__ col,
