@@ -99,17 +99,6 @@ __device__ __forceinline__ void cg_reset(const DevParams& dp, SceneCG* __restric
 }
 __global__ void k_cg_reset(DevParams dp, SceneCG* __restrict__ scg, DevCounts* dc) { cg_reset(dp, scg, dc); }
 
-// the PCG reset (first wave of the first workgroup) and the clear of the
-// per-page channel bounds, in one launch
-__global__ void k_cg_reset_bounds(DevParams dp, SceneCG* __restrict__ scg, DevCounts* dc,
-                                  unsigned* __restrict__ pbound, int64_t cap_p) {
-    if (blockIdx.x == 0 && threadIdx.x < 64) cg_reset(dp, scg, dc);
-    int64_t np = dc->num_page;
-    np = np < cap_p ? np : cap_p;
-    const int64_t m = np * kP2GCh;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (int64_t)gridDim.x * blockDim.x)
-        pbound[q] = 0u;
-}
 
 
 // the particle's terms times the channel scales (powers of two, so every
@@ -388,9 +377,20 @@ __global__ __launch_bounds__(256) void k_grid_update(DevParams dp, const uint32_
 // source lists and G2P ranges of every 2x2x2 block (block_lists_wave), one
 // wave per block; they depend on the page tables only and serve the block
 // P2G, the K s gathers of the substep's PCG and the block G2P
-__global__ __launch_bounds__(256) void k_block_lists(BlockTables t, BlockClasses* __restrict__ lists,
-                                                     BlockG2P* __restrict__ g2p, int n,
-                                                     const DevCounts* __restrict__ dc) {
+__global__ __launch_bounds__(256) void k_block_lists(DevParams dp, BlockTables t, BlockClasses* __restrict__ lists,
+                                                     BlockG2P* __restrict__ g2p, int n, SceneCG* __restrict__ scg,
+                                                     unsigned* __restrict__ pbound, DevCounts* dc) {
+    // the substep's PCG reset and the clear of the per-page channel bounds
+    // ride along (the next kernels, the stress pass and the block P2G, need
+    // both)
+    if (blockIdx.x == 0 && threadIdx.x < 64) cg_reset(dp, scg, dc);
+    {
+        int64_t npb = dc->num_page;
+        npb = npb < t.cap_p ? npb : t.cap_p;
+        const int64_t m = npb * kP2GCh;
+        for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (int64_t)gridDim.x * blockDim.x)
+            pbound[q] = 0u;
+    }
     const int np_raw = dc->num_page;
     const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
     int nt = dc->num_total;
@@ -407,8 +407,9 @@ hipError_t launch_block_lists(mpmx_ctx* c) {
     int g = (block_grid(c, 100) + 3) / 4;
     g = (g + 7) / 8 * 8;
     MPMX_TIMED(c, "block_lists", (k_block_lists<<<g, 256, 0, c->stream>>>(
-                                      block_tables_of(c), static_cast<BlockClasses*>(c->block_lists),
-                                      static_cast<BlockG2P*>(c->block_g2p), (int)c->n, c->dc)));
+                                      c->dp, block_tables_of(c), static_cast<BlockClasses*>(c->block_lists),
+                                      static_cast<BlockG2P*>(c->block_g2p), (int)c->n, c->scg, c->pbound,
+                                      c->dc)));
     return hipGetLastError();
 }
 
@@ -470,11 +471,9 @@ hipError_t launch_p2g(mpmx_ctx* c, float dt) {
 // payload and page bounds, so the P2G's own payload pass is skipped.  Same
 // results as launch_stress then launch_p2g, bit for bit.
 hipError_t launch_stress_p2g(mpmx_ctx* c, float dt) {
-    int64_t blocks = (c->cap_p * kP2GCh + 255) / 256;
-    blocks = blocks < 2048 ? (blocks < 1 ? 1 : blocks) : 2048;
-    k_cg_reset_bounds<<<(int)blocks, 256, 0, c->stream>>>(c->dp, c->scg, c->dc, c->pbound, c->cap_p);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    // the PCG reset and the zero page bounds come from k_block_lists, which
+    // the substep's table build has just run
+    hipError_t e = hipSuccess;
     MPMX_TIMED(c, "stress_prep", e = launch_stress_prep(c, dt));
     if (e != hipSuccess) return e;
     const bool mixed = c->cfg.precision == MPMX_MIXED;
