@@ -297,31 +297,6 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
     }
 }
 
-// partials of the PCG's first "dot"(r, z), per scene (the reference's
-// 128-entry dot blocks follow page order, not the 2x2x2 blocks)
-__global__ __launch_bounds__(256) void k_cg_rho0(DevParams dp, const float* __restrict__ cgr,
-                                                 const float* __restrict__ cgz, float* __restrict__ part_rho,
-                                                 int64_t cap_tp, const SceneCG* __restrict__ scg,
-                                                 const int* __restrict__ wg_scene, const DevCounts* __restrict__ dc) {
-    __shared__ float lds[4];
-    const SceneBlocks sb = scene_blocks(scg, wg_scene);
-    const int num_page = dc->num_page;
-    const bool maxdot = dp.dot_mode == 0;
-    float wacc = maxdot ? -INFINITY : 0.f;
-    for (int blk = sb.wave; blk < sb.nblk; blk += sb.nwaves) {
-        const int64_t n0 = scene_block_node(sb, num_page, cap_tp, blk);
-        float rz = 0.f;
-        if (n0 >= 0) {
-            const float2 r = *(const float2*)(cgr + n0);
-            const float2 z = *(const float2*)(cgz + n0);
-            rz = r.x * z.x + r.y * z.y;
-        }
-        const float bs = wave_sum(rz);
-        wacc = maxdot ? fmaxf(wacc, bs) : wacc + bs;
-    }
-    const float r = maxdot ? combine_waves<true>(wacc, lds) : combine_waves<false>(wacc, lds);
-    if (threadIdx.x == 0) part_rho[blockIdx.x] = r;
-}
 
 // the grid update after the scatter P2G: one wave per 128-node PCG block, two
 // nodes per lane
@@ -432,9 +407,7 @@ static hipError_t gather_blocks_t(mpmx_ctx* c, float dt) {
     MPMX_TIMED(c, "p2g_block", (k_p2g_block<TRANSFER, ST><<<g, 256, 0, c->stream>>>(
                                     c->dp, c->payload, c->pbound, t, static_cast<BlockClasses*>(c->block_lists),
                                     c->grid, raw, cg_vec(c), dt, (int)c->n, c->dc)));
-    if (c->dp.phase_field)
-        k_cg_rho0<<<c->cg_nb, 256, 0, c->stream>>>(c->dp, c->cg_r, c->cg_z, c->cg_part + c->cg_nb, c->cap_tp,
-                                                    c->scg, c->wg_scene, c->dc);
+    // the PCG's starting "dot"(r, z) comes with its first "dot"(s, q) (k_cg_dot)
     return hipGetLastError();
 }
 

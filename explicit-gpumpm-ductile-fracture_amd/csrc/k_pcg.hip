@@ -351,10 +351,15 @@ __device__ __forceinline__ bool scene_stopped(const SceneCG* scg, int scene) {
 }
 
 // q = q + s D (page-tile variant; the block variant adds s D in k_cg_q);
-// partial of "dot"(s, q)
+// partial of "dot"(s, q).  With rho0 set (first iteration of the block
+// variant) also the partial of the starting "dot"(r, z) into part_rho.  The
+// fused multiply-adds are spelled out so that the rounding does not depend
+// on how the compiler pairs the products.
 __global__ __launch_bounds__(256) void k_cg_dot(DevParams dp, const float* __restrict__ grid,
                                                 const float* __restrict__ cgs, float* __restrict__ cgq,
-                                                float* __restrict__ part_sq, int add_diag, int64_t cap_tp,
+                                                float* __restrict__ part_sq, int add_diag,
+                                                const float* __restrict__ cgr, const float* __restrict__ cgz,
+                                                float* __restrict__ part_rho, int rho0, int64_t cap_tp,
                                                 const SceneCG* __restrict__ scg, const int* __restrict__ wg_scene,
                                                 const DevCounts* __restrict__ dc) {
     __shared__ float lds[4];
@@ -362,10 +367,10 @@ __global__ __launch_bounds__(256) void k_cg_dot(DevParams dp, const float* __res
     if (scene_stopped(scg, sb.scene)) return;
     const int num_page = dc->num_page;
     const bool maxdot = dp.dot_mode == 0;
-    float wacc = maxdot ? -INFINITY : 0.f;
+    float wacc = maxdot ? -INFINITY : 0.f, racc = wacc;
     for (int blk = sb.wave; blk < sb.nblk; blk += sb.nwaves) {
         const int64_t n0 = scene_block_node(sb, num_page, cap_tp, blk);
-        float part = 0.f;
+        float part = 0.f, rz = 0.f;
         if (n0 >= 0) {
             const float2 sv = *(const float2*)(cgs + n0);
             float2 qv = *(const float2*)(cgq + n0);
@@ -375,13 +380,26 @@ __global__ __launch_bounds__(256) void k_cg_dot(DevParams dp, const float* __res
                 qv.y += sv.y * D[1];
                 *(float2*)(cgq + n0) = qv;
             }
-            part = sv.x * qv.x + sv.y * qv.y;
+            part = fmaf(sv.y, qv.y, sv.x * qv.x);
+            if (rho0) {
+                const float2 r = *(const float2*)(cgr + n0);
+                const float2 z = *(const float2*)(cgz + n0);
+                rz = fmaf(r.x, z.x, r.y * z.y);
+            }
         }
         const float bs = wave_sum(part);
         wacc = maxdot ? fmaxf(wacc, bs) : wacc + bs;
+        if (rho0) {
+            const float br = wave_sum(rz);
+            racc = maxdot ? fmaxf(racc, br) : racc + br;
+        }
     }
     const float r = combine_dot(dp, wacc, lds);
     if (threadIdx.x == 0) part_sq[blockIdx.x] = r;
+    if (rho0) {
+        const float rr = combine_dot(dp, racc, lds);
+        if (threadIdx.x == 0) part_rho[blockIdx.x] = rr;
+    }
 }
 
 // alpha = rho_old / sq; x += alpha s; r -= alpha q; q = 0; z = r / D where
@@ -531,7 +549,9 @@ static hipError_t pcg_iteration(mpmx_ctx* c, int k) {
     const uint64_t* amask = blocks ? c->cg_amask : nullptr;
     MPMX_TIMED(c, "cg_vector", ({
                    k_cg_dot<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_s, c->cg_q, c->cg_part,
-                                                       blocks ? 0 : 1, c->cap_tp, c->scg, c->wg_scene, c->dc);
+                                                       blocks ? 0 : 1, c->cg_r, c->cg_z, c->cg_part + nb,
+                                                       (blocks && k == 0) ? 1 : 0, c->cap_tp, c->scg,
+                                                       c->wg_scene, c->dc);
                    k_cg_update<<<nb, 256, 0, c->stream>>>(c->dp, c->grid, c->cg_x, c->cg_r, c->cg_z, c->cg_s,
                                                           c->cg_q, c->cg_part, nb, k, c->cap_tp, c->scg,
                                                           c->wg_scene, amask, blocks ? 0 : 1, c->dc);
