@@ -56,9 +56,9 @@ __device__ __forceinline__ uint32_t scene_page_key(const BlockTables& t, int px,
 // which node offsets can fall outside the block.  Each source page is listed
 // as runs of its passing cells (cell = i<<4 | j<<2 | k, i, j, k in 0..3), with
 // the particle ranges from the page's cell starts, so every listed particle
-// reaches the block; a workgroup's threads are split among the classes in
-// proportion to their entries, so that most waves hold particles of one class
-// and take the same branches.
+// reaches the block; a workgroup's threads are split among the classes, an
+// equal number of entries each, so that most waves hold particles of one
+// class and take the same branches.
 // class:             0  1(z) 2(y) 3(yz) 4(x) 5(xz) 6(xy) 7(xyz)
 // pages:             8  4    4    2     4    2     2     1
 // runs per page:     1  16   4    8     1    8     2     4
@@ -237,9 +237,11 @@ __device__ __forceinline__ void block_lists_wave(const BlockTables& t, uint32_t 
     }
     const int all = carry;
     if (lane == 0) dst->pre[kMaxRuns] = all;
-    // threads per class in proportion to its entries (one at least), so that
-    // every thread takes about all / 256 entries of a single class and waves
-    // stay within one class except where two classes meet
+    // every thread takes L = ceil(all / 248) entries of a single class (the
+    // last thread of a class fewer) and a class gets ceil(n_c / L) threads
+    // (at most 255 in all): waves stay within one class except where two
+    // classes meet, their lanes run L entries each, and the threads left
+    // over form idle trailing waves rather than short runs in busy ones
     const int c = lane < 8 ? lane : 0;
     int pre_c[2];
 #pragma unroll
@@ -250,8 +252,9 @@ __device__ __forceinline__ void block_lists_wave(const BlockTables& t, uint32_t 
         pre_c[h] = b >= kMaxRuns ? all : ((b >> 6) == 0 ? v0 : ((b >> 6) == 1 ? v1 : v2));
     }
     const int n_c = pre_c[1] - pre_c[0];
+    const int L = all > 0 ? (all + 247) / 248 : 1;
     int thr = 0;
-    if (lane < 8 && n_c > 0) thr = (int)((248ll * n_c) / all) + 1;
+    if (lane < 8 && n_c > 0) thr = (n_c + L - 1) / L;
     int tin = thr;
 #pragma unroll
     for (int o = 1; o < 8; o <<= 1) {
@@ -260,7 +263,7 @@ __device__ __forceinline__ void block_lists_wave(const BlockTables& t, uint32_t 
     }
     if (lane < 8) {
         dst->thr0[lane] = tin - thr;
-        dst->len[lane] = thr > 0 ? (n_c + thr - 1) / thr : 0;
+        dst->len[lane] = thr > 0 ? L : 0;
     }
     if (lane == 7) dst->thr0[8] = tin;
 }
