@@ -372,9 +372,10 @@ __global__ __launch_bounds__(256) void k_block_lists(DevParams dp, BlockTables t
     nt = nt < t.cap_tp ? nt : (int)t.cap_tp;
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
-    const int per = (int)(blockDim.x >> 6);
-    const int nwaves = (int)gridDim.x * per;
-    for (int blk = xcd_logical_block() * per + (int)(threadIdx.x >> 6); blk < nb; blk += nwaves)
+    // one block per wave
+    const int per = (int)(blockDim.x >> 6), wave = (int)(threadIdx.x >> 6);
+    const int end = xcd_item_end(nb), step = xcd_item_step(per);
+    for (int blk = xcd_item_first(nb, per) + wave; blk < end; blk += step)
         block_lists_wave(t, t.block_key[blk], n, np_raw, np, nt, lists + blk, g2p + blk);
 }
 

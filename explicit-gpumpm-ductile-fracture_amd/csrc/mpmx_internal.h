@@ -399,25 +399,52 @@ __device__ __forceinline__ float combine_waves(float v, float* lds) {
     return r;
 }
 
-// Blocks are dispatched round-robin over the 8 XCDs.  Remap the hardware
-// block index so that consecutive logical blocks (neighbouring pages in Morton
-// order, which share grid pages and thus atomics) run on the same XCD/L2.
-__device__ __forceinline__ int xcd_logical_block() {
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Items of `count` for the calling workgroup, `per_wg` consecutive items per
+// workgroup step (1, or one per wave): XCD x = blockIdx % 8 owns the
+// contiguous share [x s, (x + 1) s) of the items, s = ceil(count / 8), so that
+// neighbouring items share an L2 and every XCD gets an equal share of the
+// items that exist (the grids are sized with slack for growth, and the
+// workgroups beyond a share exit at once); its workgroups j = blockIdx / 8
+// take the share's items j per_wg, + per_wg G / 8, ...  A grid that is not a
+// multiple of 8 strides over all items instead.
+__device__ __forceinline__ int xcd_item_first(int count, int per_wg) {
     const int G = gridDim.x, b = blockIdx.x;
-    if (G % 8) return b;
-    return (b % 8) * (G / 8) + b / 8;
+    if (G & 7) return b * per_wg;
+    return (b & 7) * ((count + 7) >> 3) + (b >> 3) * per_wg;
 }
-// items lb, lb + G, lb + 2G, ... of `count` for the calling logical block lb:
-// neighbouring items run on the same XCD and the load evens out
-#define FOR_LOGICAL_ITEMS(var, count) for (int var = xcd_logical_block(); var < (count); var += (int)gridDim.x)
-
-// contiguous [begin, end) chunk of `count` items for the calling logical block
-__device__ __forceinline__ void block_range(int count, int& begin, int& end) {
-    const int lb = xcd_logical_block();
+__device__ __forceinline__ int xcd_item_end(int count) {
+    const int G = gridDim.x, b = blockIdx.x;
+    if (G & 7) return count;
+    const int e = ((b & 7) + 1) * ((count + 7) >> 3);
+    return e < count ? e : count;
+}
+__device__ __forceinline__ int xcd_item_step(int per_wg) {
     const int G = gridDim.x;
-    const int per = (count + G - 1) / G;
-    begin = lb * per;
-    end = begin + per < count ? begin + per : count;
+    return ((G & 7) ? G : (G >> 3)) * per_wg;
+}
+#define FOR_LOGICAL_ITEMS(var, count)                                                             \
+    for (int var = xcd_item_first((count), 1), var##_end = xcd_item_end(count); var < var##_end; \
+         var += xcd_item_step(1))
+
+// contiguous [begin, end) chunk of `count` items for the calling workgroup:
+// XCD x = blockIdx % 8 owns the share [x s, (x + 1) s), s = ceil(count / 8),
+// and its workgroups j = blockIdx / 8 split it into equal chunks (a grid that
+// is not a multiple of 8 splits all items by blockIdx)
+__device__ __forceinline__ void block_range(int count, int& begin, int& end) {
+    const int G = gridDim.x, b = blockIdx.x;
+    int lo = 0, hi = count, g = G, j = b;
+    if (!(G & 7)) {
+        const int s = (count + 7) >> 3;
+        lo = (b & 7) * s;
+        hi = lo + s < count ? lo + s : count;
+        hi = hi > lo ? hi : lo;
+        g = G >> 3;
+        j = b >> 3;
+    }
+    const int per = (hi - lo + g - 1) / g;
+    begin = lo + j * per;
+    end = begin + per < hi ? begin + per : hi;
 }
 
 // scene of page key pk
