@@ -28,11 +28,13 @@ struct BlockTables {
     const int* cell_start;     // [cap_p][65] first particle of each cell of each particle page
     int64_t cap_p, cap_tp, cap_blocks;
     int slot_shift;            // scene of a page key = key >> slot_shift
+    const int* order;          // block of each item of the XCD shares, or null (item = block)
 };
 
 inline BlockTables block_tables_of(const mpmx_ctx* c) {
     return BlockTables{c->pmask, c->nmask, c->wcount, c->scount, c->page2particle, c->keys, c->block_key,
-                       c->cell_start, c->cap_p, c->cap_tp, c->tp_bound, c->dp.slot_shift};
+                       c->cell_start, c->cap_p, c->cap_tp, c->tp_bound, c->dp.slot_shift,
+                       c->order_ok ? c->block_order : nullptr};
 }
 
 // page key of page (px, py, pz) (non-negative) when that page lies in the
@@ -282,6 +284,37 @@ __device__ __forceinline__ int class_entry(const BlockClasses& S, int cls, int j
         else hi = mid - 1;
     }
     return S.start[lo] + (e - S.pre[lo]);
+}
+
+// The order in which the block kernels take the blocks of XCD share x (items
+// [x s, (x + 1) s), s = ceil(nb / 8), as xcd_item_first splits them): the
+// blocks whose gather threads take two or more entries each (more than 248
+// entries) first, then the others, each group in block order, so that the
+// long blocks do not start last and leave the XCD's CUs idle at the end.
+// One wave; deterministic.  kOrderGroups workgroups (one share each) build it.
+constexpr int kOrderGroups = 8;
+__device__ __forceinline__ void order_share_wave(const BlockClasses* __restrict__ lists, int* __restrict__ order,
+                                                 int x, int nb) {
+    const int lane = threadIdx.x & 63;
+    const int s = (nb + 7) >> 3;
+    const int lo = x * s, hi = lo + s < nb ? lo + s : nb;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int heavy = 0;
+    for (int b0 = lo; b0 < hi; b0 += 64) {
+        const int b = b0 + lane;
+        heavy += __popcll(__ballot(b < hi && lists[b].pre[kMaxRuns] > 248));
+    }
+    int ph = lo, pl = lo + heavy;
+    for (int b0 = lo; b0 < hi; b0 += 64) {
+        const int b = b0 + lane;
+        const bool in = b < hi;
+        const bool h = in && lists[b].pre[kMaxRuns] > 248;
+        const uint64_t mh = __ballot(h), ml = __ballot(in && !h);
+        if (h) order[ph + __popcll(mh & below)] = b;
+        else if (in) order[pl + __popcll(ml & below)] = b;
+        ph += __popcll(mh);
+        pl += __popcll(ml);
+    }
 }
 
 // the class of the calling thread and its entries [jb, je) of that class;

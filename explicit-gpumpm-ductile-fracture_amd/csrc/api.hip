@@ -358,6 +358,8 @@ static void free_particles(mpmx_ctx* c) {
     c->block_lists = nullptr;
     if (c->block_g2p) (void)hipFree(c->block_g2p);
     c->block_g2p = nullptr;
+    dfree(c->block_order);
+    c->order_ok = false;
     dfree(c->grid); dfree(c->grid_p2g);
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q); dfree(c->cg_part); dfree(c->cg_amask);
     dfree(c->scan_tmp);
@@ -693,6 +695,7 @@ mpmx_status mpmx_set_particles_batch(mpmx_ctx* c, const int64_t* counts, const d
     HIPCHK(c, dalloc(&c->cell_start, 65 * c->cap_p));
     HIPCHK(c, hipMalloc(&c->block_lists, sizeof(BlockClasses) * (size_t)tp_bound));
     HIPCHK(c, hipMalloc(&c->block_g2p, sizeof(BlockG2P) * (size_t)tp_bound));
+    HIPCHK(c, dalloc(&c->block_order, tp_bound));
     HIPCHK(c, dalloc(&c->adj, 7 * c->cap_p));
     HIPCHK(c, dalloc(&c->nvp, c->cap_p + 1));
     HIPCHK(c, dalloc(&c->vpo, c->cap_p + 1));

@@ -225,7 +225,8 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
-    FOR_LOGICAL_ITEMS(blk, nb) {
+    FOR_LOGICAL_ITEMS(item, nb) {
+        const int blk = t.order ? t.order[item] : item;
         const uint32_t bkey = t.block_key[blk];
         int bc[3];
         unmorton(bkey, bc[0], bc[1], bc[2]);
@@ -414,6 +415,7 @@ static hipError_t gather_blocks_t(mpmx_ctx* c, float dt) {
 
 template <int TRANSFER, typename ST>
 static hipError_t gather_dispatch_t(mpmx_ctx* c, ST st, float dt) {
+    c->order_ok = false;   // the stage path leaves the blocks in their own order
     const BlockTables t = block_tables_of(c);
     hipError_t e = fill_zero_dev_count(c, reinterpret_cast<float*>(c->pbound), &c->dc->num_page, kP2GCh, c->cap_p);
     if (e != hipSuccess) return e;
@@ -448,6 +450,7 @@ hipError_t launch_stress_p2g(mpmx_ctx* c, float dt) {
     // the PCG reset and the zero page bounds come from k_block_lists, which
     // the substep's table build has just run
     hipError_t e = hipSuccess;
+    c->order_ok = true;   // the stress pass orders the blocks
     MPMX_TIMED(c, "stress_prep", e = launch_stress_prep(c, dt));
     if (e != hipSuccess) return e;
     const bool mixed = c->cfg.precision == MPMX_MIXED;

@@ -147,7 +147,8 @@ __global__ __launch_bounds__(256) void k_cg_col(DevParams dp, ST s, const float*
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
     const int tid = threadIdx.x;
-    FOR_LOGICAL_ITEMS(blk, nb) {
+    FOR_LOGICAL_ITEMS(item, nb) {
+        const int blk = t.order ? t.order[item] : item;
         const uint32_t bkey = t.block_key[blk];
         const int scene = __builtin_amdgcn_readfirstlane(scene_of_page(dp, bkey << 3));
         if (scg[scene].done) continue;   // uniform over the workgroup
@@ -297,7 +298,8 @@ __global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t,
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
     (void)n;
-    FOR_LOGICAL_ITEMS(blk, nb) {
+    FOR_LOGICAL_ITEMS(item, nb) {
+        const int blk = t.order ? t.order[item] : item;
         const uint32_t bkey = t.block_key[blk];
         const int scene = __builtin_amdgcn_readfirstlane(scene_of_page(dp, bkey << 3));
         if (scg[scene].done) continue;   // uniform over the workgroup

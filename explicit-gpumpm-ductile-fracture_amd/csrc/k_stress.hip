@@ -379,7 +379,18 @@ template <bool ALL64, int TRANSFER, typename ST>
 __global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, WaveMap wm, float* __restrict__ sigma,
                                                      float* __restrict__ FP, float* __restrict__ vol, BlockTables t,
                                                      float* __restrict__ pay, unsigned* __restrict__ pbound, float dt,
-                                                     int n, DevCounts* dc) {
+                                                     int n, const BlockClasses* __restrict__ lists,
+                                                     int* __restrict__ order, DevCounts* dc) {
+    // the last 8 workgroups order the blocks of the 8 XCD shares for the
+    // block kernels that follow (k_block_lists has just listed them)
+    if ((int)blockIdx.x >= (int)gridDim.x - kOrderGroups) {
+        if (threadIdx.x < 64) {
+            int nb = dc->num_blocks;
+            nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
+            order_share_wave(lists, order, (int)blockIdx.x - ((int)gridDim.x - kOrderGroups), nb);
+        }
+        return;
+    }
     int i, scene;
     const bool valid = wave_particle(wm, n, i, scene);
     float b[kP2GCh];
@@ -420,8 +431,9 @@ hipError_t launch_stress(mpmx_ctx* c) {
 
 template <bool ALL64, int TRANSFER, typename ST>
 static void stress_prep_dispatch(mpmx_ctx* c, ST st, float dt) {
-    k_stress_prep<ALL64, TRANSFER, ST><<<particle_blocks(c), c->particle_wg, 0, c->stream>>>(
-        c->dp, st, c->wmap, c->sigma, c->FP, c->vol, block_tables_of(c), c->payload, c->pbound, dt, (int)c->n, c->dc);
+    k_stress_prep<ALL64, TRANSFER, ST><<<particle_blocks(c) + kOrderGroups, c->particle_wg, 0, c->stream>>>(
+        c->dp, st, c->wmap, c->sigma, c->FP, c->vol, block_tables_of(c), c->payload, c->pbound, dt, (int)c->n,
+        static_cast<const BlockClasses*>(c->block_lists), c->block_order, c->dc);
 }
 
 template <bool ALL64, typename ST>

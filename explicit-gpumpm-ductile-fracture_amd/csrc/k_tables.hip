@@ -284,6 +284,7 @@ hipError_t build_tables(mpmx_ctx* c) {
     const int n = (int)c->n;
     const int64_t words = c->mask_words;
     hipError_t e;
+    c->order_ok = false;   // new blocks: identity order until the fused stress pass orders them
     k_mark_pages<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, reinterpret_cast<uint8_t*>(c->pflag));
     k_pack_pages<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->pflag, c->pmask, words);
     k_neighbour_mask<<<(int)((words + 255) / 256), 256, 0, c->stream>>>(c->dp, c->pmask, c->nmask, c->wcount,

@@ -308,7 +308,8 @@ __global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTabl
     __shared__ float tv[4 * kBR3];
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
-    FOR_LOGICAL_ITEMS(blk, nb) {
+    FOR_LOGICAL_ITEMS(item, nb) {
+        const int blk = t.order ? t.order[item] : item;
         const uint32_t bkey = t.block_key[blk];
         const int scene = __builtin_amdgcn_readfirstlane(scene_of_page(dp, bkey << 3));
         const DevParams sp = dp.scenes[scene];   // loaded once per block, before the particle loop
@@ -425,7 +426,10 @@ static void g2p_block(mpmx_ctx* c, ST st, float dt) {
 
 template <typename ST>
 static void g2p_block_kernel(mpmx_ctx* c, ST st, float dt) {
-    const BlockTables t = block_tables_of(c);
+    // the blocks in their own order: the G2P's work per block is its own
+    // particles, not the gather entries the order ranks by
+    BlockTables t = block_tables_of(c);
+    t.order = nullptr;
     const int g = block_grid(c, c->g2p_pct);
     MPMX_TIMED(c, "g2p_block", ({
                    if (c->P.transfer == 0)

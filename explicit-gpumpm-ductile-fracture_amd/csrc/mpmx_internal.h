@@ -214,6 +214,7 @@ struct mpmx_ctx {
     int* cell_start = nullptr;       // [cap_p][65] first particle of each cell of each particle page
     void* block_lists = nullptr;     // [tp_bound] per-block source lists of the block P2G (BlockClasses)
     void* block_g2p = nullptr;       // [tp_bound] per-block G2P pages and particle range (BlockG2P)
+    int* block_order = nullptr;      // [tp_bound] blocks of each XCD share, heavy ones first
     bool vpages_valid = false;       // adjacency and virtual pages match the current tables
     // launch shapes (mpmx_tuning; mpmx_autotune picks them): none changes results
     int particle_wg = 256;           // threads per workgroup of the per-particle kernels
@@ -247,6 +248,7 @@ struct mpmx_ctx {
     int cg_nb = 0;               // workgroups of every node-space kernel
     int cg_spec = 1;             // PCG iterations queued before the first host check
     bool grid_fused = false;     // last P2G also ran the grid update (gather variant)
+    bool order_ok = false;       // block_order holds this substep's order (fused stress pass)
     // counters
     mpmx::DevCounts* dc = nullptr;
     mpmx::DevCounts* hc = nullptr;   // pinned host mirror
