@@ -139,8 +139,9 @@ __device__ __forceinline__ void class_run(int cls, int r, int& c0, int& len) {
 }
 
 // The lists of block bkey, built by one wave (lane = threadIdx.x & 63) and
-// written straight to dst: the source pages from the page tables (lanes 0..26
-// one source slot each, lanes 32..39 the block's 8 grid pages), the 132 runs
+// written straight to dst: the page ids from the page tables (one rank per
+// lane, all at once: the 27 source slots, the G2P region's 27 grid pages and
+// the block's own 8 grid pages), the 132 runs
 // of passing cells (three per lane) with their particle ranges read from the
 // cell starts, the exclusive prefix over the runs, and the threads of each
 // class of the 256-thread gathers.
@@ -150,31 +151,41 @@ __device__ __forceinline__ void block_lists_wave(const BlockTables& t, uint32_t 
     int bc[3];
     unmorton(bkey, bc[0], bc[1], bc[2]);
     const int scene = (int)((bkey << 3) >> t.slot_shift);
-    int pg = -1;
+    // one page rank per lane, all issued at once: lanes 0..26 the source
+    // slots, 27..53 the G2P region's grid pages 2b + (0, 1, 2) per axis,
+    // 54..61 the block's own 8 pages
+    uint32_t key = ~0u;
     if (lane < 27) {
         int cls, d[3];
         source_slot(lane, cls, d);
         const int px = 2 * bc[0] + d[0], py = 2 * bc[1] + d[1], pz = 2 * bc[2] + d[2];
-        const uint32_t key = (px >= 0 && py >= 0 && pz >= 0) ? scene_page_key(t, px, py, pz, scene) : ~0u;
-        if (key != ~0u) {
-            const int id = page_rank(key, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
-            if (id >= 0 && id < np) {
-                int a = t.page2particle[id], b = t.page2particle[id + 1];
-                a = a < 0 ? 0 : (a > n ? n : a);
-                b = b < a ? a : (b > n ? n : b);
-                if (b > a) pg = id;
-            }
+        key = (px >= 0 && py >= 0 && pz >= 0) ? scene_page_key(t, px, py, pz, scene) : ~0u;
+    } else if (lane < 54) {
+        const int q = lane - 27;
+        key = scene_page_key(t, 2 * bc[0] + q / 9, 2 * bc[1] + (q / 3) % 3, 2 * bc[2] + q % 3, scene);
+    } else if (lane < 62) {
+        key = (bkey << 3) | (uint32_t)(lane - 54);
+    }
+    const int id = key != ~0u ? page_rank(key, t.pmask, t.nmask, t.wpre, t.spre, np_raw) : -1;
+    // source pages that hold particles
+    int pg = -1;
+    if (lane < 27) {
+        if (id >= 0 && id < np) {
+            int a = t.page2particle[id], b = t.page2particle[id + 1];
+            a = a < 0 ? 0 : (a > n ? n : a);
+            b = b < a ? a : (b > n ? n : b);
+            if (b > a) pg = id;
         }
         dst->page[lane] = pg;
+    } else if (lane < 54) {
+        g2p->page[lane - 27] = (id >= 0 && id < nt) ? id : -1;
     }
     // the block's own 8 pages: grid pages, and the particle range of those
     // that hold particles (consecutive page ids, so one range)
     {
-        const int o = lane - 32;
         int lo = 0x7fffffff, hi = -1;
-        if (o >= 0 && o < 8) {
-            const int id = page_rank((bkey << 3) | (uint32_t)o, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
-            dst->out_page[o] = (id >= 0 && id < nt) ? id : -1;
+        if (lane >= 54 && lane < 62) {
+            dst->out_page[lane - 54] = (id >= 0 && id < nt) ? id : -1;
             if (id >= 0 && id < np) lo = hi = id;
         }
 #pragma unroll
@@ -193,14 +204,6 @@ __device__ __forceinline__ void block_lists_wave(const BlockTables& t, uint32_t 
             g2p->pa = a;
             g2p->pb = b;
         }
-    }
-    // the G2P region's grid pages: 2b + (0, 1, 2) per axis
-    if (lane < 27) {
-        const int px = 2 * bc[0] + lane / 9, py = 2 * bc[1] + (lane / 3) % 3, pz = 2 * bc[2] + lane % 3;
-        int id = -1;
-        const uint32_t key = scene_page_key(t, px, py, pz, scene);
-        if (key != ~0u) id = page_rank(key, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
-        g2p->page[lane] = (id >= 0 && id < nt) ? id : -1;
     }
     // runs q = lane + 64 k in class order; an absent page's runs are empty
     int excl[3], carry = 0;
