@@ -76,6 +76,8 @@ def lib():
         L.mo_svd3.argtypes = [f64p, f64p, f64p, f64p]
         L.mo_svd3_last_sweeps.restype = C.c_int
         L.mo_inverse3.argtypes = [f64p, f64p]
+        L.mo_vdot.argtypes = [C.c_int, f64p, f64p, C.c_int]
+        L.mo_vdot.restype = C.c_double
         L.mo_neo_hookean.argtypes = [C.POINTER(Params), f64p, C.c_double, f64p, f64p, f64p, f64p]
         L.mo_neo_hookean.restype = None
         L.mo_fixed_corotated.argtypes = [C.POINTER(Params), f64p, f64p]
@@ -207,6 +209,14 @@ def inverse3(A):
     out = np.zeros(9)
     lib().mo_inverse3(A, out)
     return out.reshape(3, 3)
+
+
+def vdot(a, b, mode):
+    """The PCG's "dot" of the oracle: mode 0 the reference's largest
+    128-entry block sum, mode 1 the true dot product."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return float(lib().mo_vdot(len(a), a, b, mode))
 
 
 def von_mises(params, F, c, g, alpha, psi):

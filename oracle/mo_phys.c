@@ -557,10 +557,12 @@ static void boundary(mo_sim* s) {
     }
 }
 
-/* getDotProduct / getNormSquare_b (ExplicitTimeIntegrator.cu:712-747) */
-static double vdot(const mo_sim* s, const double* a, const double* b) {
-    const int n = s->counts[1] * 64;
-    if (s->p.dot_mode == 1) {
+/* getDotProduct (ExplicitTimeIntegrator.cu:731-747): myDotSum sums each
+   128-entry block (:574-610) and mysumFm reduces the block sums with a
+   maximum, not a sum (:528-571), so mode 0 is the largest 128-entry block
+   sum; mode 1 is the true dot product. */
+double mo_vdot(int n, const double* a, const double* b, int mode) {
+    if (mode == 1) {
         double acc = 0;
         for (int i = 0; i < n; ++i) acc += a[i] * b[i];
         return acc;
@@ -573,6 +575,9 @@ static double vdot(const mo_sim* s, const double* a, const double* b) {
         if (blk == 0 || acc > best) best = acc;
     }
     return best;
+}
+static double vdot(const mo_sim* s, const double* a, const double* b) {
+    return mo_vdot(s->counts[1] * 64, a, b, s->p.dot_mode);
 }
 static double vnorm(const mo_sim* s, const double* a) {
     const int n = s->counts[1] * 64;

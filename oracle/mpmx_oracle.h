@@ -19,6 +19,12 @@
  *    neighbour-only pages follow, also in ascending page-key order
  *    (the reference numbers them by atomicAdd order, Transformer_kernal.cu:85).
  * Matrices are 3x3 column-major: M(r,c) = m[r + 3c] (reference d_F layout).
+ *
+ * Parity unpinned (checked only as a faithful transcription, no golden vector
+ * or closed form fixes them): the branch quirks of applyVonMises (trace-based
+ * dlambda, clamps, alpha storage), the QR-SVD's choice among equivalent sign
+ * conventions beyond what the SVD tests fix, and the reference's float-literal
+ * constants.  The PCG's block-max "dot" (mo_vdot) is pinned by closed forms.
  */
 #ifndef MPMX_ORACLE_H
 #define MPMX_ORACLE_H
@@ -89,6 +95,9 @@ int  mo_svd3_last_sweeps(void);
 
 /* Gauss-Jordan inverse (ExplicitTimeIntegrator.cu:71-129), row-major 3x3 */
 void mo_inverse3(const double in[9], double out[9]);
+/* the PCG "dot" of n entries: mode 0 the largest 128-entry block sum (the
+   reference's getDotProduct), mode 1 the true dot product */
+double mo_vdot(int n, const double* a, const double* b, int mode);
 
 /* von Mises return mapping + psi+ history + phase-field coefficients for one
  * particle (ExplicitTimeIntegrator.cu:132-370).  F is updated in place.

@@ -422,6 +422,24 @@ def test_zero_weight_node_stays_inactive(oracle):
     assert sim.get("c")[0] == pytest.approx(1.0, abs=1e-12)
 
 
+def test_block_max_dot_closed_forms(oracle):
+    """The reference's PCG "dot" (getDotProduct, ExplicitTimeIntegrator.cu:731-747)
+    sums 128-entry blocks and reduces the block sums with a maximum.  Closed
+    forms that a plain dot, another block size, a sum of block maxima or a
+    zero start of the maximum would each fail."""
+    ones = np.ones(192)
+    assert oracle.vdot(ones, ones, 0) == 128.0     # blocks of 128 and 64
+    assert oracle.vdot(ones, ones, 1) == 192.0
+    a = np.concatenate([-np.ones(128), 0.5 * np.ones(64)])
+    assert oracle.vdot(a, np.ones(192), 0) == 32.0          # max(-128, 32)
+    assert oracle.vdot(a, np.ones(192), 1) == -96.0
+    b = np.concatenate([-np.ones(128), -4.0 * np.ones(64)])
+    assert oracle.vdot(b, np.ones(192), 0) == -128.0        # max(-128, -256): no 0 start
+    c = np.concatenate([np.ones(64), 3.0 * np.ones(64), 2.0 * np.ones(128), np.ones(64)])
+    assert oracle.vdot(c, np.ones(320), 0) == 256.0         # blocks 256, 256, 64
+    assert oracle.vdot(np.arange(64.0), np.ones(64), 0) == 2016.0   # one page: its sum
+
+
 def test_pcg_matches_dense_solve(oracle):
     """Phase-field system (D + M K) x = b assembled densely from the particles
     with textbook B-spline weights; the oracle's PCG (true dot products, tight
