@@ -1,6 +1,6 @@
 // k_tables.hip — sparse page tables from the sorted particle keys.
 //
-// Replaces DomainTransformer::rebuild (Transformer.cpp:125-136,
+// Replaces DomainTransformer::rebuild (Transformer.cpp:71-82,
 // Transformer_kernal.cu:15-209).  The reference numbers pages through a hash
 // table with atomicAdd, so ids depend on scheduling.  Here the occupied pages
 // and their +x/+y/+z neighbours are set in two dense bitmasks over the Morton
@@ -14,7 +14,7 @@
 namespace mpmx {
 
 // neighbour q (0..6) of a page = offset (i,j,k) with q+1 = i*4 + j*2 + k
-// (Transformer.cpp:81-88)
+// (Transformer.cpp:27-34)
 __device__ __forceinline__ uint32_t neighbor_key(uint32_t pk, int q) {
     int x, y, z;
     unmorton(pk, x, y, z);

@@ -1,7 +1,7 @@
 // scene.cpp — seeded synthetic scenes (host only).
 //
 // The reference builds its scenes from an SDF of two dragons or an OBJ point
-// list with rand() rejection sampling (Simulator.cpp:145-267,
+// list with rand() rejection sampling (MPMSimulator::build, Simulator.cpp:76-198,
 // LoadSample.cpp:109-214); those assets are not part of the checkout.  These
 // generators produce the benchmark configurations from a SplitMix64 stream:
 //   0  cube: 20^3 cells at [54,74)^3 of a 128^3 grid, 8 particles per cell
@@ -303,7 +303,7 @@ extern "C" int64_t mpmx_scene(int scene, uint64_t seed, double scale, mpmx_param
 }
 
 // ---- scene files ----------------------------------------------------------
-// Point-list OBJ (Sample::readObj, LoadSample.cpp:180-194): every line
+// Point-list OBJ (Sample::readObj, LoadSample.cpp:163-194): every line
 // "v x y z" is one particle; the reference's loader moves it down by 0.2 in
 // y.  Other lines (comments, faces) are skipped.
 extern "C" int64_t mpmx_read_obj_points(const char* path, double y_shift, double* x, int64_t cap) {

@@ -135,7 +135,7 @@ class DomainTransformer {
 public:
     DomainTransformer() {}
     void initialize(mpmx_ctx* ctx, const int numParticle, const int gridVolume);
-    void rebuild();                // Transformer.cpp:125-136
+    void rebuild();                // Transformer.cpp:71-82
     void refresh();
     int _numParticle = 0, _gridVolume = 0;
     int _numPage = 0, _numCell = 0, _numTotalPage = 0, _numVirtualPage = 0;
@@ -205,8 +205,8 @@ public:
     auto& getModelParticlePtr(const unsigned int modelId = 0) { return getModel(modelId).getParticlesPtr(); }
     auto& getModelMaterialPtr(const unsigned int modelId = 0) { return getModel(modelId).getMaterialPtr(); }
 
-    T computeDt(const T cur, const T next);                                       // Simulator.cpp:269-294
-    void simulateStick(float* cflTime, float* preTime, float* simuTime, int type);  // Simulator.cpp:296-328
+    T computeDt(const T cur, const T next);                                       // Simulator.cpp:200-225
+    void simulateStick(float* cflTime, float* preTime, float* simuTime, int type);  // Simulator.cpp:227-259
     void writePartio(const std::string& filename);   // declared in the reference (Simulator.h:35): binary dump
 
     std::vector<vector3T> h_pos;

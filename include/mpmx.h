@@ -153,13 +153,14 @@ int         mpmx_device_count(void);
 
 /* MPMSimulator::MallocDeviceMEM + Particles::Particles_Malloc_MEM +
  * SPGrid::SPGrid_MEM_Malloc + DomainTransformer::Transformer_Malloc_MEM
- * (Simulator.cpp:85-100, Particles.cpp:54-94, SPGrid.cpp:10-55, Transformer.cpp:59-89) */
+ * (Simulator.cpp:16-31, Particles.cpp:54-94, SPGrid.cpp:10-55, Transformer.cpp:5-35) */
 mpmx_status mpmx_create(const mpmx_params* p, const mpmx_config* cfg, mpmx_ctx** out);
-/* the destructors (Simulator.cpp:102-111, Particles.cpp:96-125, SPGrid.cpp:63-79) */
+/* MPMSimulator::FreeDeviceMEM and the destructors (Simulator.cpp:33-43, Particles.cpp:96-125,
+ * SPGrid.cpp:63-79, Transformer.cpp:50-69) */
 mpmx_status mpmx_destroy(mpmx_ctx* ctx);
 
 /* Particles::initialize / initialize_kernal (Particles.cpp:8-31, Particle_kernal.cu:441-481)
- * and MPMSimulator::buildIntegrator's F = I, B = 0 (Simulator.cpp:136-142).
+ * and MPMSimulator::buildIntegrator's F = I, B = 0 (Simulator.cpp:67-73, TimeIntegrator.cu:46-51).
  * x, v: n*3 doubles (xyz per particle); mass: n doubles.  Keys and the first
  * canonical sort are computed here. */
 mpmx_status mpmx_set_particles(mpmx_ctx* ctx, int64_t n, const double* x, const double* v,
@@ -168,7 +169,7 @@ mpmx_status mpmx_set_particles(mpmx_ctx* ctx, int64_t n, const double* x, const 
 /* Batched scenes: num_scenes independent scenes, each with its own parameters
  * (grid N, material, yield, grips, walls, gravity, energy model), advanced
  * together by every stage and substep call with one dt.  No counterpart in
- * the reference, which runs one scene per process (Simulator.cpp:113-143).
+ * the reference, which runs one scene per process (MPMSimulator::build, Simulator.cpp:76-198).
  * transfer, vpage, dot_mode and cg_max_iter must agree across scenes;
  * p2g_variant 1 is single-scene only.  Each scene's PCG keeps its own
  * scalars, iteration count and stopping test.  Positions stay in each
@@ -189,7 +190,7 @@ mpmx_status mpmx_get_scene_stats(mpmx_ctx* ctx, int scene, mpmx_scene_stats* out
 /* Particles::reorder / sort_by_offsets (Particle_kernal.cu:484-573): canonical
  * stable sort by cell key, then every particle field is gathered to the new order */
 mpmx_status mpmx_bin_sort(mpmx_ctx* ctx);
-/* DomainTransformer::rebuild (Transformer.cpp:125-136) */
+/* DomainTransformer::rebuild (Transformer.cpp:71-82) */
 mpmx_status mpmx_rebuild(mpmx_ctx* ctx);
 /* ExplicitTimeIntegrator::computeForceCoefficient -> applyVonMises
  * (ExplicitTimeIntegrator.cu:846-865, 132-370) */
@@ -199,7 +200,7 @@ mpmx_status mpmx_p2g(mpmx_ctx* ctx, double dt);
 /* MPMTimeIntegrator::undateGrid + resolveCollision (TimeIntegrator.cu:378-403),
  * also forms the phase-field right-hand side and Jacobi diagonal */
 mpmx_status mpmx_grid_update(mpmx_ctx* ctx, double dt);
-/* the damage solve of ExplicitTimeIntegrator::integrate (ExplicitTimeIntegrator.cu:909-968) */
+/* the damage solve inside ExplicitTimeIntegrator::integrate, ExplicitTimeIntegrator.cu:909-968 */
 mpmx_status mpmx_phase_field(mpmx_ctx* ctx, double dt);
 /* MPMTimeIntegrator::transferG2P (TimeIntegrator.cu:411-474) */
 mpmx_status mpmx_g2p(mpmx_ctx* ctx, double dt);
@@ -215,7 +216,7 @@ mpmx_status mpmx_substep(mpmx_ctx* ctx, double dt);
 mpmx_status mpmx_run(mpmx_ctx* ctx, int nsteps, double dt);
 /* Particles::getMaxParticleVel_c (Particle_kernal.cu:421-440): max |v|^2 over particles */
 mpmx_status mpmx_max_velocity_sq(mpmx_ctx* ctx, double* out);
-/* MPMSimulator::computeDt (Simulator.cpp:269-294): min(dt_default, cfl*dx/max|v|) */
+/* MPMSimulator::computeDt (Simulator.cpp:200-225): min(dt_default, cfl*dx/max|v|) */
 mpmx_status mpmx_compute_dt(mpmx_ctx* ctx, double dt_default, double cfl, double* dt_out);
 
 mpmx_status mpmx_sync(mpmx_ctx* ctx);
@@ -350,7 +351,7 @@ mpmx_status mpmx_set_tuning(mpmx_ctx* ctx, const mpmx_tuning* t);
 mpmx_status mpmx_autotune(mpmx_ctx* ctx, double dt, int trial_steps, mpmx_tuning* best);
 
 /* ---- scene files and particle files (host; checkpoints also touch the GPU) ---- */
-/* Sample::readObj (LoadSample.cpp:180-194): a point-list OBJ, one particle per
+/* Sample::readObj (LoadSample.cpp:163-194): a point-list OBJ, one particle per
  * "v x y z" line, y moved by y_shift (the reference's loader uses -0.2).
  * Returns the count (x may be NULL to size it) or -1. */
 int64_t mpmx_read_obj_points(const char* path, double y_shift, double* x, int64_t cap);

@@ -104,6 +104,14 @@ def lib():
         L.mo_sim_pages.argtypes = [C.c_void_p, u64p]
         L.mo_sim_pages.restype = C.c_int
         L.mo_sim_conservation.argtypes = [C.c_void_p, f64p]
+        L.mo_sim_cfl_dt.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)]
+        L.mo_sim_cfl_dt.restype = C.c_double
+        L.mo_node_boundary.argtypes = [C.POINTER(Params), C.c_int, C.c_int, C.c_int, f64p]
+        L.mo_node_boundary.restype = None
+        L.mo_speed_clamp.argtypes = [C.c_double, f64p]
+        L.mo_speed_clamp.restype = None
+        L.mo_damage_update.argtypes = [C.c_double, C.c_double]
+        L.mo_damage_update.restype = C.c_double
         L.mo_p2g_mls.argtypes = [C.POINTER(Params), C.c_int, u64p, f64p, f64p, f64p, f64p,
                                  f64p, f64p, f64p, f64p, C.c_double, C.c_int, u64p, f64p]
         L.mo_p2g_mls.restype = C.c_int
@@ -325,6 +333,29 @@ class Sim:
         out = np.zeros(4)
         lib().mo_sim_conservation(self.h, out)
         return out
+
+    def cfl_dt(self, dt_default, cfl=0.3, wave=False):
+        """(dt, largest speed) of MPMSimulator::computeDt on the current state."""
+        sp = C.c_double(0.0)
+        dt = lib().mo_sim_cfl_dt(self.h, float(dt_default), float(cfl), 1 if wave else 0, C.byref(sp))
+        return float(dt), float(sp.value)
+
+
+def node_boundary(params, i, j, k, v):
+    """Grid node (i, j, k)'s velocity after the grips and the sticky walls."""
+    out = np.ascontiguousarray(v, np.float64).copy()
+    lib().mo_node_boundary(C.byref(params), int(i), int(j), int(k), out)
+    return out
+
+
+def speed_clamp(vmax, v):
+    out = np.ascontiguousarray(v, np.float64).copy()
+    lib().mo_speed_clamp(float(vmax), out)
+    return out
+
+
+def damage_update(c, dc):
+    return float(lib().mo_damage_update(float(c), float(dc)))
 
 
 def g2p_affine(N, x, A, b):

@@ -10,7 +10,7 @@
  *   Particle_kernal.cu:484-542           sort_by_offsets (bucket sort by page/cell)
  *   Transformer_kernal.cu:15-126,129-209 page/cell boundaries, hash of pages,
  *                                        7 neighbour pages, topology, virtual pages
- *   Transformer.cpp:81-88                neighbour offsets (i*4, j*4, k*4) order
+ *   Transformer.cpp:27-34                neighbour offsets (i*4, j*4, k*4) order
  */
 #include "mpmx_oracle.h"
 #include <math.h>
@@ -191,7 +191,7 @@ static int find_u64(const uint64_t* a, int n, uint64_t key) {
     return -1;
 }
 
-/* Transformer.cpp:81-88: offsets of pages (i,j,k) in {0,1}^3 \ {0}, order i*4+j*2+k */
+/* Transformer.cpp:27-34: offsets of pages (i,j,k) in {0,1}^3 \ {0}, order i*4+j*2+k */
 static void neighbor_offsets(uint64_t nb[7]) {
     int q = 0;
     for (int i = 0; i < 2; ++i)
@@ -216,7 +216,7 @@ int mo_page_tables(int n, const uint64_t* sorted, int vpage, int cap_p, int cap_
     }
     if (cell2particle) cell2particle[numCell] = n;
 
-    /* markPageBoundary + scan + markBlockOffset (Transformer_kernal.cu:38-47, 148-161) */
+    /* markPageBoundary + scan + markBlockOffset (Transformer_kernal.cu:26-47, 148-161) */
     int numPage = 0;
     for (int p = 0; p < n; ++p) {
         if (p == 0 || (sorted[p] >> 12) != (sorted[p - 1] >> 12)) {

@@ -356,7 +356,7 @@ mo_sim* mo_sim_create(const mo_params* p, int n, const double* x, const double* 
     memcpy(s->x, x, sizeof(double) * 3 * (size_t)n);
     memcpy(s->v, v, sizeof(double) * 3 * (size_t)n);
     memcpy(s->m, mass, sizeof(double) * (size_t)n);
-    /* Particles::initialize_kernal (Particle_kernal.cu:451-454), initMatrix (TimeIntegrator.cu:46-51) */
+    /* Particles::initialize_kernal (Particle_kernal.cu:441-481), initMatrix (TimeIntegrator.cu:46-51) */
     for (int k = 0; k < n; ++k) {
         for (int a = 0; a < 9; ++a) s->F[9 * k + a] = (a % 3 == a / 3) ? 1.0 : 0.0;
         s->c[k] = 1.0;
@@ -530,31 +530,72 @@ static void grid_update(mo_sim* s, double dt) {
     }
 }
 
-/* collideWithGround (TimeIntegrator.cu:117-165), with the grip slabs applied
- * first as in PullGrid_kernal (TimeIntegrator.cu:169-261) */
+/* one grid node's velocity after the boundaries: the grip slabs first, as in
+ * PullGrid_kernal (TimeIntegrator.cu:169-261, here two x-slabs moving at
+ * -+grip_v), then the sticky walls of collideWithGround
+ * (TimeIntegrator.cu:117-165): a node within wall_cells of a face whose
+ * velocity points out through that face stops entirely */
+void mo_node_boundary(const mo_params* p, int i, int j, int k, double v[3]) {
+    const int N = p->N, W = p->wall_cells;
+    if (p->grip_lo >= 0) {
+        if (i <= p->grip_lo) { v[0] = -p->grip_v; v[1] = 0; v[2] = 0; }
+        if (i >= p->grip_hi) { v[0] = p->grip_v; v[1] = 0; v[2] = 0; }
+    }
+    if (i <= W && v[0] < 0.0) { v[0] = 0; v[1] = 0; v[2] = 0; }
+    if (i > N - W && v[0] > 0.0) { v[0] = 0; v[1] = 0; v[2] = 0; }
+    if (j <= W && v[1] < 0.0) { v[1] = 0; v[0] = 0; v[2] = 0; }
+    if (j > N - W && v[1] > 0.0) { v[1] = 0; v[0] = 0; v[2] = 0; }
+    if (k <= W && v[2] < 0.0) { v[2] = 0; v[0] = 0; v[1] = 0; }
+    if (k > N - W && v[2] > 0.0) { v[2] = 0; v[0] = 0; v[1] = 0; }
+}
+
 static void boundary(mo_sim* s) {
-    const int N = s->p.N, W = s->p.wall_cells;
     for (int id = 0; id < s->counts[1]; ++id) {
         const int pi = page_coord(s->page_key[id], 0) * 4, pj = page_coord(s->page_key[id], 1) * 4,
                   pk = page_coord(s->page_key[id], 2) * 4;
         for (int cell = 0; cell < 64; ++cell) {
             const int i = pi + (cell >> 4), j = pj + ((cell >> 2) & 3), k = pk + (cell & 3);
             const int ni = id * 64 + cell;
-            double* v0 = CH(s, 1) + ni;
-            double* v1 = CH(s, 2) + ni;
-            double* v2 = CH(s, 3) + ni;
-            if (s->p.grip_lo >= 0) {
-                if (i <= s->p.grip_lo) { *v0 = -s->p.grip_v; *v1 = 0; *v2 = 0; }
-                if (i >= s->p.grip_hi) { *v0 = s->p.grip_v; *v1 = 0; *v2 = 0; }
-            }
-            if (i <= W && *v0 < 0.0) { *v0 = 0; *v1 = 0; *v2 = 0; }
-            if (i > N - W && *v0 > 0.0) { *v0 = 0; *v1 = 0; *v2 = 0; }
-            if (j <= W && *v1 < 0.0) { *v1 = 0; *v0 = 0; *v2 = 0; }
-            if (j > N - W && *v1 > 0.0) { *v1 = 0; *v0 = 0; *v2 = 0; }
-            if (k <= W && *v2 < 0.0) { *v2 = 0; *v0 = 0; *v1 = 0; }
-            if (k > N - W && *v2 > 0.0) { *v2 = 0; *v0 = 0; *v1 = 0; }
+            double v[3] = {CH(s, 1)[ni], CH(s, 2)[ni], CH(s, 3)[ni]};
+            mo_node_boundary(&s->p, i, j, k, v);
+            CH(s, 1)[ni] = v[0];
+            CH(s, 2)[ni] = v[1];
+            CH(s, 3)[ni] = v[2];
         }
     }
+}
+
+/* the MLS grid-to-particle speed clamp (G2PKernal.cu:134-142): a velocity
+ * faster than vmax keeps its direction at speed vmax */
+void mo_speed_clamp(double vmax, double v[3]) {
+    const double norms = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (norms > vmax)
+        for (int a = 0; a < 3; ++a) v[a] = v[a] / norms * vmax;
+}
+
+/* the monotone damage update of G2P (G2PKernal.cu:155): c only decreases
+ * and stays >= 0 */
+double mo_damage_update(double c, double dc) { return fmax(fmin(c, c + dc), 0.0); }
+
+/* MPMSimulator::computeDt (Simulator.cpp:200-225) over
+ * Particles::getMaxParticleVel_c (Particle_kernal.cu:421-440): dt =
+ * min(dt_default, cfl dx / max|v|).  wave = 1 adds each particle's elastic
+ * wave speed sqrt((lambda + 2 mu) V0 / m) to its speed (an extension). */
+double mo_sim_cfl_dt(const mo_sim* s, double dt_default, double cfl, int wave, double* max_speed) {
+    double lambda, mu, kappa;
+    lame(&s->p, &lambda, &mu, &kappa);
+    double best = 0.0;
+    for (int id = 0; id < s->n; ++id) {
+        const double* v = s->v + 3 * id;
+        double sp = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (wave) sp += sqrt((lambda + 2.0 * mu) * s->p.V0 / s->m[id]);
+        if (sp > best) best = sp;
+    }
+    if (max_speed) *max_speed = best;
+    const double dx = 1.0 / s->p.N;
+    double dt = dt_default;
+    if (best > 0 && cfl * dx / best < dt_default) dt = cfl * dx / best;
+    return dt;
 }
 
 /* getDotProduct (ExplicitTimeIntegrator.cu:731-747): myDotSum sums each
@@ -757,15 +798,9 @@ static void g2p(mo_sim* s, double dt) {
                             G[r + 3 * c] += vi[r] * g3[c];
                         }
                 }
-        if (s->p.transfer == 0) {
-            const double norms = sqrt(val[0] * val[0] + val[1] * val[1] + val[2] * val[2]);
-            if (norms > vmax)
-                for (int a = 0; a < 3; ++a) val[a] = val[a] / norms * vmax;
-        }
+        if (s->p.transfer == 0) mo_speed_clamp(vmax, val);
         for (int a = 0; a < 3; ++a) s->v[3 * id + a] = val[a];
-        const double pc = s->c[id];
-        const double nc = pc + val[3];
-        s->c[id] = fmax(fmin(pc, nc), 0.0);
+        s->c[id] = mo_damage_update(s->c[id], val[3]);
         for (int a = 0; a < 3; ++a) s->x[3 * id + a] += val[a] * dt;
         double A[9];
         for (int a = 0; a < 9; ++a)

@@ -144,6 +144,20 @@ int     mo_sim_grid(const mo_sim* s, int stage, int ch, double* out);
 int     mo_sim_pages(const mo_sim* s, uint64_t* page_key);
 /* FP64 conservation sums over particles: [mass, px, py, pz] */
 void    mo_sim_conservation(const mo_sim* s, double out[4]);
+/* MPMSimulator::computeDt (Simulator.cpp:200-225) with getMaxParticleVel_c
+ * (Particle_kernal.cu:421-440): min(dt_default, cfl dx / max speed); wave = 1
+ * adds each particle's elastic wave speed sqrt((lambda + 2 mu) V0 / m).
+ * *max_speed (may be NULL) receives the largest speed. */
+double  mo_sim_cfl_dt(const mo_sim* s, double dt_default, double cfl, int wave, double* max_speed);
+
+/* single-node and single-particle pieces the substep uses */
+/* grips (PullGrid_kernal, TimeIntegrator.cu:169-261) then sticky walls
+ * (collideWithGround, TimeIntegrator.cu:117-165) on node (i, j, k)'s velocity */
+void    mo_node_boundary(const mo_params* p, int i, int j, int k, double v[3]);
+/* MLS speed clamp (G2PKernal.cu:134-142) */
+void    mo_speed_clamp(double vmax, double v[3]);
+/* monotone damage update max(min(c, c + dc), 0) (G2PKernal.cu:155) */
+double  mo_damage_update(double c, double dc);
 
 /* Stage-wise entry points for single-kernel parity (used on the same inputs as
  * the product kernels).  All particle arrays are in the given (sorted) order. */
