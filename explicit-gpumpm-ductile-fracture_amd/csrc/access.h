@@ -402,11 +402,13 @@ __device__ __forceinline__ void st_B(const MState& s, int i, int k, int n, float
 
 __device__ __forceinline__ float ld_F(const PState& s, int i, int k, int n) { return s.F[(int64_t)k * n + i]; }
 __device__ __forceinline__ float ld_F(const MState& s, int i, int k, int n) {
+    if (s.F32) return s.F32[(int64_t)k * n + i];
     return h2f(s.F[(int64_t)k * n + i]) + ((k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f);
 }
 __device__ __forceinline__ void st_F(const PState& s, int i, int k, int n, float f) { s.F[(int64_t)k * n + i] = f; }
 __device__ __forceinline__ void st_F(const MState& s, int i, int k, int n, float f) {
-    s.F[(int64_t)k * n + i] = f2h(f - ((k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f));
+    if (s.F32) s.F32[(int64_t)k * n + i] = f;
+    else s.F[(int64_t)k * n + i] = f2h(f - ((k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f));
 }
 
 __device__ __forceinline__ float ld_c(const PState& s, int i) { return s.c[i]; }

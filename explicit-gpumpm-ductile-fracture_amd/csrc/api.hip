@@ -48,7 +48,8 @@ __global__ void k_upload_mixed(MState s, const double* __restrict__ x, const dou
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         s.B[(int64_t)k * n + i] = f2bf(0.f);
-        s.F[(int64_t)k * n + i] = f2h(0.f);
+        if (s.F32) s.F32[(int64_t)k * n + i] = (k % 3 == k / 3) ? 1.f : 0.f;
+        else s.F[(int64_t)k * n + i] = f2h(0.f);
     }
     s.c[i] = f2bf(1.f);
     s.g[i] = f2bf(1.f);
@@ -58,9 +59,25 @@ __global__ void k_upload_mixed(MState s, const double* __restrict__ x, const dou
     s.id[i] = i;
 }
 
-// FP64 sums of mass, momentum and kinetic energy
-__global__ void k_conservation(const float* __restrict__ mass, const float* __restrict__ v,
-                               const uint16_t* __restrict__ vb, int n, double* out) {
+// ---- particle reductions: one partial per workgroup, then one workgroup
+// combines the partials in a fixed order (no atomics, bit-reproducible) ----
+// mpmx_ctx::red_part holds kRedParts * 5 doubles of partials, then results
+// from double kRedResult on
+constexpr int kRedParts = 1024;
+constexpr int kRedResult = 5 * kRedParts;
+constexpr int kRedDoubles = kRedResult + 8;
+
+__global__ void k_clear_time(DevCounts* dc) {
+    if (threadIdx.x == 0) dc->time = 0.0;
+}
+
+// FP64 sums of mass, momentum and kinetic energy of particles [0, n) of the
+// arrays given (a range of the current order): part[5 * blockIdx.x + q]
+__global__ __launch_bounds__(256) void k_conservation_part(const float* __restrict__ mass,
+                                                           const float* __restrict__ v,
+                                                           const uint16_t* __restrict__ vb, int n,
+                                                           double* __restrict__ part) {
+    __shared__ double lds[5][4];
     double acc[5] = {0, 0, 0, 0, 0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double m = mass[i];
@@ -78,24 +95,85 @@ __global__ void k_conservation(const float* __restrict__ mass, const float* __re
         double a = acc[q];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&out[q], a);
+        if ((threadIdx.x & 63) == 0) lds[q][threadIdx.x >> 6] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += lds[threadIdx.x][w];
+        part[5 * blockIdx.x + threadIdx.x] = s;
+    }
+}
+// out[q] = sum of the G partials of quantity q, in workgroup order
+__global__ void k_conservation_final(const double* __restrict__ part, int G, double* __restrict__ out) {
+    if (threadIdx.x < 5) {
+        double s = 0;
+        for (int g = 0; g < G; ++g) s += part[5 * g + threadIdx.x];
+        out[threadIdx.x] = s;
     }
 }
 
-__global__ void k_max_speed2(const float* __restrict__ v, const uint16_t* __restrict__ vb, int n,
-                             unsigned* out) {
+// Largest CFL speed of the particles: q_p = ((|v_p| + wave c_p) N_s)^2 with
+// c_p = sqrt((lambda + 2 mu) V0 / m_p) the elastic wave speed (mode bit 0)
+// and N_s the cells per axis of the particle's scene (mode bit 1); mode 0 is
+// |v_p|^2 (getMaxParticleVel_c).  Waves hold particles of one scene (the
+// per-particle wave map), so a wave reads its scene's constants once.
+// part[blockIdx.x] = the workgroup's largest q (floats, exact under max).
+__global__ __launch_bounds__(256) void k_speed_part(DevParams dp, const float* __restrict__ v,
+                                                    const uint16_t* __restrict__ vb, const float* __restrict__ mass,
+                                                    WaveMap wm, int n, int mode, float* __restrict__ part) {
+    __shared__ float lds[4];
+    const int64_t W = wm.first ? wm.waves : ((int64_t)n + 63) / 64;
+    const int lane = threadIdx.x & 63;
     float best = 0.f;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float s2 = 0.f;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const float va = vb ? bf2f(vb[3 * i + a]) : v[3 * i + a];
-            s2 += va * va;
+    for (int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < W;
+         w += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+        int i, scene;
+        bool valid;
+        if (wm.first) {
+            i = wm.first[w] + lane;
+            scene = __builtin_amdgcn_readfirstlane(wm.scene[w]);
+            valid = lane < wm.count[w];
+        } else {
+            i = (int)(w * 64 + lane);
+            scene = 0;
+            valid = i < n;
         }
-        best = fmaxf(best, s2);
+        if (!valid) continue;
+        float va[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) va[a] = vb ? bf2f(vb[3 * i + a]) : v[3 * i + a];
+        float q = va[0] * va[0] + va[1] * va[1] + va[2] * va[2];
+        if (mode) {
+            const DevParams& sp = dp.scenes[scene];
+            float s = sqrtf(q);
+            if (mode & 1) s += sqrtf((sp.lambda + 2.0f * sp.mu) * sp.V0 / mass[i]);
+            if (mode & 2) s *= (float)sp.N;
+            q = s * s;
+        }
+        best = fmaxf(best, q);
     }
-    best = wave_max(best);
-    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(best));
+    best = combine_waves<true>(wave_max(best), lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = best;
+}
+// out[0] = the largest q, out[1] = dt = min(dt_default, cfl / sqrt(q)) (FP64,
+// Simulator.cpp:213); with set_dt the device keeps dt for the next substep
+// launched with dt <= 0 and adds it to the simulated time
+__global__ __launch_bounds__(256) void k_speed_final(const float* __restrict__ part, int G, double dt_default,
+                                                     double cfl, int set_dt, DevCounts* dc, double* __restrict__ out) {
+    __shared__ float lds[4];
+    const float q = reduce_parts<true>(part, G, lds);
+    if (threadIdx.x == 0) {
+        const double vmax = sqrt((double)q);
+        double dt = dt_default;
+        if (vmax > 0 && cfl / vmax < dt_default) dt = cfl / vmax;
+        out[0] = (double)q;
+        out[1] = dt;
+        if (set_dt) {
+            dc->dt = (float)dt;
+            dc->time += (double)(float)dt;
+        }
+    }
 }
 
 // cell tables (Transformer_kernal.cu:15-36): head flags, then ids
@@ -347,7 +425,7 @@ static void free_particles(mpmx_ctx* c) {
         dfree(s.x); dfree(s.v); dfree(s.B); dfree(s.F); dfree(s.c); dfree(s.alpha); dfree(s.psi); dfree(s.g);
         dfree(s.mass); dfree(s.id);
         MState& m = c->ms[b];
-        dfree(m.x); dfree(m.v); dfree(m.B); dfree(m.F); dfree(m.c); dfree(m.alpha); dfree(m.psi); dfree(m.g);
+        dfree(m.x); dfree(m.v); dfree(m.B); dfree(m.F); dfree(m.F32); dfree(m.c); dfree(m.alpha); dfree(m.psi); dfree(m.g);
         dfree(m.mass); dfree(m.id);
     }
     dfree(c->sigma); dfree(c->FP); dfree(c->vol); dfree(c->cg_col); dfree(c->payload); dfree(c->pbound);
@@ -425,6 +503,9 @@ static mpmx_status alloc_grid(mpmx_ctx* c, int64_t cap_tp) {
     HIPCHK(c, fill_zero(c, c->cg_x, cap_tp * 64 * 4));
     return MPMX_OK;
 }
+
+static mpmx_status upload_particles(mpmx_ctx* c, const int64_t* counts, const std::vector<int>& start, int64_t n,
+                                    const double* x, const double* v, const double* mass);
 
 // ---- public API --------------------------------------------------------------
 extern "C" {
@@ -549,7 +630,8 @@ mpmx_status mpmx_create_batch(int num_scenes, const mpmx_params* ps, const mpmx_
     (void)hipMemcpy(c->dc, c->hc, sizeof(DevCounts), hipMemcpyHostToDevice);
     if (hipMalloc((void**)&c->scene_dp, sizeof(DevParams) * num_scenes) != hipSuccess ||
         hipMalloc((void**)&c->scg, sizeof(SceneCG) * num_scenes) != hipSuccess ||
-        hipMalloc((void**)&c->scene_start, sizeof(int) * (num_scenes + 1)) != hipSuccess) {
+        hipMalloc((void**)&c->scene_start, sizeof(int) * (num_scenes + 1)) != hipSuccess ||
+        hipMalloc((void**)&c->red_part, sizeof(double) * kRedDoubles) != hipSuccess) {
         mpmx_destroy(c);
         return MPMX_ERR_HIP;
     }
@@ -571,6 +653,7 @@ mpmx_status mpmx_destroy(mpmx_ctx* c) {
     dfree(c->scene_dp);
     dfree(c->scg);
     dfree(c->scene_start);
+    dfree(c->red_part);
     dfree(c->wg_scene);
     dfree(c->wave_buf);
     if (c->dc) (void)hipFree(c->dc);
@@ -599,12 +682,37 @@ mpmx_status mpmx_set_particles_batch(mpmx_ctx* c, const int64_t* counts, const d
     }
     HIPCHK(c, hipSetDevice(c->device));
     for (int k = 0; k < c->num_scenes; ++k) {
-        const int Nk = c->scene_P[(size_t)k].N;
-        const double lo = 2.0 / Nk, hi = 1.0 - 2.0 / Nk;
-        for (int64_t i = 3 * (int64_t)start[(size_t)k]; i < 3 * (int64_t)start[(size_t)k + 1]; ++i)
-            if (!(x[i] >= lo && x[i] <= hi)) return fail(c, MPMX_ERR_OUT_OF_DOMAIN, "particle outside [2dx, 1-2dx]");
+        // the float test the kernels apply to the stored (float) positions
+        const float dxf = (float)(1.0 / c->scene_P[(size_t)k].N);
+        const float lo = 2.0f * dxf, hi = 1.0f - 2.0f * dxf;
+        for (int64_t i = 3 * (int64_t)start[(size_t)k]; i < 3 * (int64_t)start[(size_t)k + 1]; ++i) {
+            const float xf = (float)x[i];
+            if (!(xf >= lo && xf <= hi)) return fail(c, MPMX_ERR_OUT_OF_DOMAIN, "particle outside [2dx, 1-2dx]");
+        }
     }
+    // from here on the old state is gone: a failure leaves an empty context
+    // (n = 0, every buffer freed), which the stage calls refuse
     free_particles(c);
+    mpmx_status s = upload_particles(c, counts, start, n, x, v, mass);
+    if (s != MPMX_OK) {
+        const std::string msg = c->last_error;
+        (void)hipStreamSynchronize(c->stream);
+        free_particles(c);
+        dfree(c->wave_buf);
+        c->wmap = WaveMap{};
+        c->n = 0;
+        c->scene_n.assign((size_t)c->num_scenes, 0);
+        c->sorted = false;
+        c->last_error = msg;
+    }
+    return s;
+}
+
+}  // extern "C"
+
+// buffers, upload, first sort and tables of mpmx_set_particles_batch
+static mpmx_status upload_particles(mpmx_ctx* c, const int64_t* counts, const std::vector<int>& start, int64_t n,
+                                    const double* x, const double* v, const double* mass) {
     c->scene_n.assign(counts, counts + c->num_scenes);
     HIPCHK(c, hipMemcpy(c->scene_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
     // waves of the per-particle kernels: each scene's range (contiguous in
@@ -628,14 +736,15 @@ mpmx_status mpmx_set_particles_batch(mpmx_ctx* c, const int64_t* counts, const d
     }
     c->n = n;
     c->cap_n = std::max<int64_t>(n, c->cfg.max_particles);
-    const bool mixed = c->cfg.precision == MPMX_MIXED;
+    const bool mixed = c->cfg.precision != MPMX_FP32;
     for (int b = 0; b < 2; ++b) {
         if (mixed) {
             MState& s = c->ms[b];
             HIPCHK(c, dalloc(&s.x, 3 * n));
             HIPCHK(c, dalloc(&s.v, 3 * n));
             HIPCHK(c, dalloc(&s.B, 9 * n));
-            HIPCHK(c, dalloc(&s.F, 9 * n));
+            if (c->cfg.precision == MPMX_MIXED_F16) HIPCHK(c, dalloc(&s.F, 9 * n));
+            else HIPCHK(c, dalloc(&s.F32, 9 * n));
             HIPCHK(c, dalloc(&s.c, n));
             HIPCHK(c, dalloc(&s.alpha, n));
             HIPCHK(c, dalloc(&s.psi, n));
@@ -736,8 +845,11 @@ mpmx_status mpmx_set_particles_batch(mpmx_ctx* c, const int64_t* counts, const d
     if (s != MPMX_OK) return s;
     c->sorted = true;
     c->substeps = 0;
+    c->stress_fields = false;
     return MPMX_OK;
 }
+
+extern "C" {
 
 mpmx_status mpmx_bin_sort(mpmx_ctx* c) {
     if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
@@ -761,27 +873,63 @@ mpmx_status mpmx_stress(mpmx_ctx* c) {
     return MPMX_OK;
 }
 
-mpmx_status mpmx_p2g(mpmx_ctx* c, double dt) {
-    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
+}  // extern "C"
+
+// The stages with their time step: dt > 0 as given, dt <= 0 the step the
+// device-side CFL left in the counters (mpmx_run_cfl).  The public entry
+// points accept dt > 0 only.
+static mpmx_status p2g_dt(mpmx_ctx* c, double dt) {
     MPMX_TIMED(c, "p2g", HIPCHK(c, launch_p2g(c, (float)dt)));
     return MPMX_OK;
 }
-
 // stress and P2G of a substep; with the gather P2G one pass computes the
 // stress and the P2G payload
 static mpmx_status stress_and_p2g(mpmx_ctx* c, double dt) {
     if (c->cfg.p2g_variant == 1) {
         mpmx_status s = mpmx_stress(c);
-        return s == MPMX_OK ? mpmx_p2g(c, dt) : s;
+        return s == MPMX_OK ? p2g_dt(c, dt) : s;
     }
     HIPCHK(c, launch_stress_p2g(c, (float)dt));
     return MPMX_OK;
 }
-
-mpmx_status mpmx_grid_update(mpmx_ctx* c, double dt) {
-    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
+static mpmx_status grid_update_dt(mpmx_ctx* c, double dt) {
     MPMX_TIMED(c, "grid_update", HIPCHK(c, launch_grid_update(c, (float)dt)));
     return MPMX_OK;
+}
+static mpmx_status g2p_dt(mpmx_ctx* c, double dt) {
+    MPMX_TIMED(c, "g2p", HIPCHK(c, launch_g2p(c, (float)dt)));
+    c->sorted = false;
+    return MPMX_OK;
+}
+static mpmx_status substep_dt(mpmx_ctx* c, double dt) {
+    HIPCHK(c, hipSetDevice(c->device));
+    mpmx_status s;
+    if ((s = mpmx_bin_sort(c)) != MPMX_OK) return s;
+    if ((s = mpmx_rebuild(c)) != MPMX_OK) return s;
+    if ((s = stress_and_p2g(c, dt)) != MPMX_OK) return s;
+    if ((s = grid_update_dt(c, dt)) != MPMX_OK) return s;
+    if ((s = mpmx_phase_field(c, dt)) != MPMX_OK) return s;
+    if (c->dp.phase_field) note_cg_iters(c, c->cg_spec);
+    if ((s = g2p_dt(c, dt)) != MPMX_OK) return s;
+    ++c->substeps;
+    return MPMX_OK;
+}
+static mpmx_status check_ready(mpmx_ctx* c, double dt) {
+    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(c, MPMX_ERR_ARG, "dt must be positive and finite");
+    return MPMX_OK;
+}
+
+extern "C" {
+
+mpmx_status mpmx_p2g(mpmx_ctx* c, double dt) {
+    mpmx_status s = check_ready(c, dt);
+    return s == MPMX_OK ? p2g_dt(c, dt) : s;
+}
+
+mpmx_status mpmx_grid_update(mpmx_ctx* c, double dt) {
+    mpmx_status s = check_ready(c, dt);
+    return s == MPMX_OK ? grid_update_dt(c, dt) : s;
 }
 
 mpmx_status mpmx_phase_field(mpmx_ctx* c, double dt) {
@@ -791,25 +939,13 @@ mpmx_status mpmx_phase_field(mpmx_ctx* c, double dt) {
 }
 
 mpmx_status mpmx_g2p(mpmx_ctx* c, double dt) {
-    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
-    MPMX_TIMED(c, "g2p", HIPCHK(c, launch_g2p(c, (float)dt)));
-    c->sorted = false;
-    return MPMX_OK;
+    mpmx_status s = check_ready(c, dt);
+    return s == MPMX_OK ? g2p_dt(c, dt) : s;
 }
 
 mpmx_status mpmx_substep(mpmx_ctx* c, double dt) {
-    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
-    HIPCHK(c, hipSetDevice(c->device));
-    mpmx_status s;
-    if ((s = mpmx_bin_sort(c)) != MPMX_OK) return s;
-    if ((s = mpmx_rebuild(c)) != MPMX_OK) return s;
-    if ((s = stress_and_p2g(c, dt)) != MPMX_OK) return s;
-    if ((s = mpmx_grid_update(c, dt)) != MPMX_OK) return s;
-    if ((s = mpmx_phase_field(c, dt)) != MPMX_OK) return s;
-    if (c->dp.phase_field) note_cg_iters(c, c->cg_spec);
-    if ((s = mpmx_g2p(c, dt)) != MPMX_OK) return s;
-    ++c->substeps;
-    return MPMX_OK;
+    mpmx_status s = check_ready(c, dt);
+    return s == MPMX_OK ? substep_dt(c, dt) : s;
 }
 
 // ---- captured substeps ---------------------------------------------------
@@ -830,7 +966,7 @@ static mpmx_status capture_substep(mpmx_ctx* c, double dt, int iters) {
     mpmx_status s = MPMX_OK;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     if ((s = mpmx_bin_sort(c)) == MPMX_OK && (s = mpmx_rebuild(c)) == MPMX_OK &&
-        (s = stress_and_p2g(c, dt)) == MPMX_OK && (s = mpmx_grid_update(c, dt)) == MPMX_OK) {
+        (s = stress_and_p2g(c, dt)) == MPMX_OK && (s = grid_update_dt(c, dt)) == MPMX_OK) {
         hipError_t e = c->dp.phase_field ? queue_pcg_iterations(c, 0, iters) : hipSuccess;
         if (e == hipSuccess)
             e = hipMemcpyAsync(c->hc, c->dc, sizeof(DevCounts), hipMemcpyDeviceToHost, c->stream);
@@ -840,7 +976,7 @@ static mpmx_status capture_substep(mpmx_ctx* c, double dt, int iters) {
     if (s == MPMX_OK && ee != hipSuccess) s = hipfail(c, ee, "hipStreamEndCapture");
     if (s == MPMX_OK) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        s = mpmx_g2p(c, dt);
+        s = g2p_dt(c, dt);
         ee = hipStreamEndCapture(c->stream, &gb);
         if (s == MPMX_OK && ee != hipSuccess) s = hipfail(c, ee, "hipStreamEndCapture");
     }
@@ -897,15 +1033,17 @@ static mpmx_status substep_graph(mpmx_ctx* c, double dt) {
     return MPMX_OK;
 }
 
+static mpmx_status run_dt(mpmx_ctx* c, int k, double dt) {
+    // the first substep after an upload or a sort, and timed runs, go eagerly
+    (void)k;
+    const bool graph = c->cfg.use_graph && !c->timing && !c->sorted;
+    return graph ? substep_graph(c, dt) : substep_dt(c, dt);
+}
+
 mpmx_status mpmx_run(mpmx_ctx* c, int nsteps, double dt) {
-    if (!c || c->n == 0) return fail(c, MPMX_ERR_ARG, "no particles");
-    for (int k = 0; k < nsteps; ++k) {
-        // the first substep after an upload or a sort, and timed runs, go eagerly
-        const bool graph = c->cfg.use_graph && !c->timing && !c->sorted;
-        mpmx_status s = graph ? substep_graph(c, dt) : mpmx_substep(c, dt);
-        if (s != MPMX_OK) return s;
-    }
-    return MPMX_OK;
+    mpmx_status s = check_ready(c, dt);
+    for (int k = 0; s == MPMX_OK && k < nsteps; ++k) s = run_dt(c, k, dt);
+    return s;
 }
 
 mpmx_status mpmx_sync(mpmx_ctx* c) {
@@ -915,37 +1053,72 @@ mpmx_status mpmx_sync(mpmx_ctx* c) {
     return check_err(c);
 }
 
+}  // extern "C"
+
+// workgroups of the particle reductions (one partial each)
+static int reduction_groups(const mpmx_ctx* c) {
+    const int64_t threads = c->wmap.first ? c->wmap.waves * 64 : c->n;
+    return (int)std::min<int64_t>(kRedParts, std::max<int64_t>(1, (threads + 1023) / 1024));
+}
+
+// the CFL speed reduction (k_speed_part / k_speed_final) with its partials
+// in `part` (kRedParts floats) and its two results in `out`
+static hipError_t launch_speed(mpmx_ctx* c, int mode, double dt_default, double cfl, int set_dt, float* part,
+                               double* out) {
+    const bool mixed = c->cfg.precision != MPMX_FP32;
+    const int G = reduction_groups(c);
+    k_speed_part<<<G, 256, 0, c->stream>>>(c->dp, mixed ? nullptr : c->ps[c->cur].v,
+                                           mixed ? c->ms[c->cur].v : nullptr,
+                                           mixed ? c->ms[c->cur].mass : c->ps[c->cur].mass, c->wmap, (int)c->n,
+                                           mode, part);
+    k_speed_final<<<1, 256, 0, c->stream>>>(part, G, dt_default, cfl, set_dt, c->dc, out);
+    return hipGetLastError();
+}
+
+extern "C" {
+
 mpmx_status mpmx_max_velocity_sq(mpmx_ctx* c, double* out) {
     if (!c || !out || c->n == 0) return MPMX_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
-    unsigned* d = nullptr;
-    HIPCHK(c, dalloc(&d, 1));
-    HIPCHK(c, hipMemsetAsync(d, 0, sizeof(unsigned), c->stream));
-    const bool mixed = c->cfg.precision == MPMX_MIXED;
-    k_max_speed2<<<256, 256, 0, c->stream>>>(mixed ? nullptr : c->ps[c->cur].v, mixed ? c->ms[c->cur].v : nullptr,
-                                             (int)c->n, d);
-    unsigned h = 0;
-    HIPCHK(c, hipMemcpyAsync(&h, d, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    double* res = c->red_part + kRedResult;
+    HIPCHK(c, launch_speed(c, 0, 1.0, 1.0, 0, reinterpret_cast<float*>(c->red_part), res));
+    double h[2];
+    HIPCHK(c, hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    dfree(d);
-    float s2;
-    std::memcpy(&s2, &h, 4);
-    *out = s2;
+    *out = h[0];
     return MPMX_OK;
 }
 
 mpmx_status mpmx_compute_dt(mpmx_ctx* c, double dt_default, double cfl, double* dt_out) {
-    if (!c || !dt_out || c->n == 0) return MPMX_ERR_ARG;
-    double s2 = 0;
-    mpmx_status st = mpmx_max_velocity_sq(c, &s2);
-    if (st != MPMX_OK) return st;
-    double dt = dt_default;
-    const double vmax = std::sqrt(s2);
-    // Simulator.cpp:282, with the finest grid of the batch
-    int N = 0;
-    for (const auto& P : c->scene_P) N = std::max(N, P.N);
-    if (vmax > 0 && cfl / N / vmax < dt_default) dt = cfl / N / vmax;
-    *dt_out = dt;
+    if (!c || !dt_out || c->n == 0 || !(dt_default > 0) || !(cfl > 0)) return MPMX_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    // Simulator.cpp:213 per scene: max over particles of |v| N_s
+    double* res = c->red_part + kRedResult;
+    HIPCHK(c, launch_speed(c, 2, dt_default, cfl, 1, reinterpret_cast<float*>(c->red_part), res));
+    double h[2];
+    HIPCHK(c, hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *dt_out = h[1];
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_run_cfl(mpmx_ctx* c, int nsteps, double dt_default, double cfl, int wave, double* time_out) {
+    mpmx_status s = check_ready(c, dt_default);
+    if (s != MPMX_OK) return s;
+    if (!(cfl > 0)) return fail(c, MPMX_ERR_ARG, "cfl must be positive");
+    HIPCHK(c, hipSetDevice(c->device));
+    k_clear_time<<<1, 64, 0, c->stream>>>(c->dc);
+    for (int k = 0; s == MPMX_OK && k < nsteps; ++k) {
+        HIPCHK(c, launch_speed(c, 2 | (wave ? 1 : 0), dt_default, cfl, 2, reinterpret_cast<float*>(c->red_part),
+                               c->red_part + kRedResult));
+        s = run_dt(c, k, -1.0);   // the kernels read the step the CFL kernels left on the device
+    }
+    if (s != MPMX_OK) return s;
+    if (time_out) {
+        HIPCHK(c, hipMemcpyAsync(c->hc, c->dc, sizeof(DevCounts), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *time_out = c->hc->time;
+    }
     return MPMX_OK;
 }
 
@@ -962,7 +1135,7 @@ static mpmx_status download_sorted(mpmx_ctx* c, int f, std::vector<double>& out)
     const int64_t n = c->n;
     const int w = field_width(f);
     out.assign((size_t)(n * w), 0.0);
-    const bool mixed = c->cfg.precision == MPMX_MIXED;
+    const bool mixed = c->cfg.precision != MPMX_FP32;
     const PState& s = c->ps[c->cur];
     const MState& m = c->ms[c->cur];
     auto get_f32 = [&](const float* p, int64_t count, std::vector<float>& h) -> hipError_t {
@@ -993,8 +1166,8 @@ static mpmx_status download_sorted(mpmx_ctx* c, int f, std::vector<double>& out)
     }
     // component-major 9-wide fields -> per particle column-major
     if (f == MPMX_FIELD_F || f == MPMX_FIELD_B || f == MPMX_FIELD_SIGMA) {
-        if (f == MPMX_FIELD_SIGMA || !mixed) {
-            const float* p = f == MPMX_FIELD_SIGMA ? c->sigma : (f == MPMX_FIELD_F ? s.F : s.B);
+        if (f == MPMX_FIELD_SIGMA || !mixed || (f == MPMX_FIELD_F && m.F32)) {
+            const float* p = f == MPMX_FIELD_SIGMA ? c->sigma : (f == MPMX_FIELD_F ? (mixed ? m.F32 : s.F) : s.B);
             HIPCHK(c, get_f32(p, 9 * n, hf));
             for (int64_t i = 0; i < n; ++i)
                 for (int k = 0; k < 9; ++k) out[9 * i + k] = hf[k * n + i];
@@ -1031,7 +1204,7 @@ static mpmx_status download_sorted(mpmx_ctx* c, int f, std::vector<double>& out)
 
 static mpmx_status download_ids(mpmx_ctx* c, std::vector<int>& ids) {
     ids.resize((size_t)c->n);
-    const int* p = c->cfg.precision == MPMX_MIXED ? c->ms[c->cur].id : c->ps[c->cur].id;
+    const int* p = c->cfg.precision != MPMX_FP32 ? c->ms[c->cur].id : c->ps[c->cur].id;
     HIPCHK(c, hipMemcpy(ids.data(), p, sizeof(int) * c->n, hipMemcpyDeviceToHost));
     return MPMX_OK;
 }
@@ -1041,6 +1214,10 @@ mpmx_status mpmx_get_field(mpmx_ctx* c, int f, double* host, size_t count) {
     HIPCHK(c, hipSetDevice(c->device));
     const int w = field_width(f);
     if (count < (size_t)(c->n * w)) return fail(c, MPMX_ERR_ARG, "output buffer too small");
+    if ((f == MPMX_FIELD_SIGMA || f == MPMX_FIELD_FP) && !c->stress_fields)
+        return fail(c, MPMX_ERR_ARG,
+                    "sigma and FP of the last stress update are not kept: set mpmx_config.store_stress "
+                    "or run the stages (mpmx_stress)");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<double> sorted;
     std::vector<int> ids;
@@ -1067,7 +1244,7 @@ mpmx_status mpmx_set_field(mpmx_ctx* c, int f, const double* host, size_t count)
     mpmx_status s = download_ids(c, ids);
     if (s != MPMX_OK) return s;
     const int64_t n = c->n;
-    const bool mixed = c->cfg.precision == MPMX_MIXED;
+    const bool mixed = c->cfg.precision != MPMX_FP32;
     const PState& ps = c->ps[c->cur];
     const MState& ms = c->ms[c->cur];
     std::vector<float> hf;
@@ -1090,7 +1267,9 @@ mpmx_status mpmx_set_field(mpmx_ctx* c, int f, const double* host, size_t count)
         hf.resize(9 * n);
         for (int64_t i = 0; i < n; ++i)
             for (int k = 0; k < 9; ++k) hf[k * n + i] = (float)host[9 * ids[i] + k];
-        if (mixed) {
+        if (mixed && f == MPMX_FIELD_F && ms.F32) {
+            HIPCHK(c, put(ms.F32, hf.data(), 4 * hf.size()));
+        } else if (mixed) {
             hu.resize(9 * n);
             for (int64_t i = 0; i < n; ++i)
                 for (int k = 0; k < 9; ++k)
@@ -1232,16 +1411,17 @@ mpmx_status mpmx_get_grid_p2g(mpmx_ctx* c, int ch, float* host, size_t count) {
 // FP64 mass, momentum and kinetic energy of particles [first, first + count)
 // of the current order
 static mpmx_status conservation_sums(mpmx_ctx* c, int64_t first, int64_t count, double h[5]) {
-    double* d = nullptr;
-    HIPCHK(c, dalloc(&d, 5));
-    HIPCHK(c, hipMemsetAsync(d, 0, sizeof(double) * 5, c->stream));
-    const bool mixed = c->cfg.precision == MPMX_MIXED;
-    k_conservation<<<256, 256, 0, c->stream>>>((mixed ? c->ms[c->cur].mass : c->ps[c->cur].mass) + first,
-                                               mixed ? nullptr : c->ps[c->cur].v + 3 * first,
-                                               mixed ? c->ms[c->cur].v + 3 * first : nullptr, (int)count, d);
-    HIPCHK(c, hipMemcpyAsync(h, d, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+    const bool mixed = c->cfg.precision != MPMX_FP32;
+    const int G = (int)std::min<int64_t>(kRedParts, std::max<int64_t>(1, (count + 1023) / 1024));
+    double* out = c->red_part + kRedResult;
+    k_conservation_part<<<G, 256, 0, c->stream>>>((mixed ? c->ms[c->cur].mass : c->ps[c->cur].mass) + first,
+                                                  mixed ? nullptr : c->ps[c->cur].v + 3 * first,
+                                                  mixed ? c->ms[c->cur].v + 3 * first : nullptr, (int)count,
+                                                  c->red_part);
+    k_conservation_final<<<1, 64, 0, c->stream>>>(c->red_part, G, out);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(h, out, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    dfree(d);
     return MPMX_OK;
 }
 
@@ -1269,6 +1449,7 @@ mpmx_status mpmx_get_stats(mpmx_ctx* c, mpmx_stats* out) {
     out->num_virtual_page = c->hc->num_vp;
     out->cg_iters = c->hc->cg_iters;
     out->cg_breakdown = c->hc->cg_breakdown;
+    out->svd_fallbacks = (unsigned)c->hc->svd_fallbacks;
     std::vector<SceneCG> g((size_t)c->num_scenes);
     HIPCHK(c, hipMemcpy(g.data(), c->scg, sizeof(SceneCG) * g.size(), hipMemcpyDeviceToHost));
     double norm = 0;
@@ -1309,11 +1490,11 @@ int mpmx_num_scenes(const mpmx_ctx* c) { return c ? c->num_scenes : 0; }
 mpmx_status mpmx_get_device_pointers(mpmx_ctx* c, mpmx_device_ptrs* o) {
     if (!c || !o) return MPMX_ERR_ARG;
     std::memset(o, 0, sizeof(*o));
-    const bool mixed = c->cfg.precision == MPMX_MIXED;
+    const bool mixed = c->cfg.precision != MPMX_FP32;
     if (mixed) {
         const MState& s = c->ms[c->cur];
-        o->pos = s.x; o->vel = s.v; o->F = s.F; o->B = s.B; o->phase_c = s.c; o->alpha = s.alpha;
-        o->psi = s.psi; o->g = s.g; o->mass = s.mass; o->ids = s.id;
+        o->pos = s.x; o->vel = s.v; o->F = s.F32 ? (void*)s.F32 : (void*)s.F; o->B = s.B; o->phase_c = s.c;
+        o->alpha = s.alpha; o->psi = s.psi; o->g = s.g; o->mass = s.mass; o->ids = s.id;
     } else {
         const PState& s = c->ps[c->cur];
         o->pos = s.x; o->vel = s.v; o->F = s.F; o->B = s.B; o->phase_c = s.c; o->alpha = s.alpha;

@@ -5,12 +5,14 @@
 // the context's buffers.
 //
 // Checkpoint layout (little-endian):
-//   "MPMXCKP1", int32 version 1, int32 num_scenes, int32 precision,
+//   "MPMXCKP1", int32 version 2, int32 num_scenes, int32 precision,
 //   int32 sorted, int64 n, int64 substeps, int64 cap_total_pages, int64 cg_spec,
 //   mpmx_params[num_scenes], mpmx_config, int64 counts[num_scenes],
 //   then the particle buffers in the context's current order, as stored on
 //   the device: x, v, B, F, c, alpha, psi, g, mass, id, keys (FP32: float
-//   fields; mixed: v, B, F, c, alpha, psi, g as 16-bit codes).
+//   fields; mixed: v, B, c, alpha, psi, g as 16-bit codes, F float, or as
+//   16-bit codes for MPMX_MIXED_F16).  A loader checks the counts against n
+//   and the buffer bytes against the file size before it allocates.
 // Resuming restores every buffer the next substep reads, so the run continues
 // exactly as it would have without the interruption.
 #include <cstdio>
@@ -34,10 +36,13 @@ struct Header {
 std::vector<std::pair<void*, size_t>> particle_buffers(mpmx_ctx* c) {
     const size_t n = (size_t)c->n;
     std::vector<std::pair<void*, size_t>> b;
-    if (c->cfg.precision == MPMX_MIXED) {
+    if (c->cfg.precision != MPMX_FP32) {
         const MState& s = c->ms[c->cur];
-        b = {{s.x, 12 * n}, {s.v, 6 * n},    {s.B, 18 * n}, {s.F, 18 * n}, {s.c, 2 * n},   {s.alpha, 2 * n},
-             {s.psi, 2 * n}, {s.g, 2 * n}, {s.mass, 4 * n}, {s.id, 4 * n}};
+        b = {{s.x, 12 * n}, {s.v, 6 * n}, {s.B, 18 * n}};
+        if (s.F32) b.push_back({s.F32, 36 * n});
+        else b.push_back({s.F, 18 * n});
+        b.insert(b.end(), {{s.c, 2 * n}, {s.alpha, 2 * n}, {s.psi, 2 * n}, {s.g, 2 * n}, {s.mass, 4 * n},
+                           {s.id, 4 * n}});
     } else {
         const PState& s = c->ps[c->cur];
         b = {{s.x, 12 * n}, {s.v, 12 * n},   {s.B, 36 * n}, {s.F, 36 * n}, {s.c, 4 * n},   {s.alpha, 4 * n},
@@ -110,7 +115,7 @@ mpmx_status mpmx_save_checkpoint(mpmx_ctx* c, const char* path) {
         return io_fail(c, MPMX_ERR_HIP, "device sync failed");
     FILE* f = std::fopen(path, "wb");
     if (!f) return io_fail(c, MPMX_ERR_ARG, "cannot open checkpoint for writing");
-    Header h{1, c->num_scenes, c->cfg.precision, c->sorted ? 1 : 0, c->n, c->substeps, c->cap_tp, c->cg_spec};
+    Header h{2, c->num_scenes, c->cfg.precision, c->sorted ? 1 : 0, c->n, c->substeps, c->cap_tp, c->cg_spec};
     bool ok = std::fwrite(kMagic, 1, 8, f) == 8 && std::fwrite(&h, sizeof(h), 1, f) == 1 &&
               std::fwrite(c->scene_P.data(), sizeof(mpmx_params), c->scene_P.size(), f) == c->scene_P.size() &&
               std::fwrite(&c->cfg, sizeof(mpmx_config), 1, f) == 1 &&
@@ -126,15 +131,19 @@ mpmx_status mpmx_save_checkpoint(mpmx_ctx* c, const char* path) {
     return ok ? MPMX_OK : io_fail(c, MPMX_ERR_HIP, "checkpoint write failed");
 }
 
-mpmx_status mpmx_load_checkpoint(const char* path, int device, mpmx_ctx** out) {
-    if (!path || !out) return MPMX_ERR_ARG;
-    *out = nullptr;
+static mpmx_status load_checkpoint(const char* path, int device, mpmx_ctx** out) {
     FILE* f = std::fopen(path, "rb");
     if (!f) return MPMX_ERR_ARG;
+    // the file's size bounds everything read from it
+    long fsize = -1;
+    if (std::fseek(f, 0, SEEK_END) == 0) fsize = std::ftell(f);
+    std::rewind(f);
     char magic[8];
     Header h;
-    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kMagic, 8) == 0 &&
-              std::fread(&h, sizeof(h), 1, f) == 1 && h.version == 1 && h.num_scenes >= 1 && h.n > 0;
+    bool ok = fsize > 0 && std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kMagic, 8) == 0 &&
+              std::fread(&h, sizeof(h), 1, f) == 1 && h.version == 2 && h.num_scenes >= 1 &&
+              h.num_scenes <= 4096 && h.n > 0 && h.n <= (1ll << 30) && h.precision >= MPMX_FP32 &&
+              h.precision <= MPMX_MIXED_F16;
     std::vector<mpmx_params> ps;
     mpmx_config cfg;
     std::vector<int64_t> counts;
@@ -145,6 +154,23 @@ mpmx_status mpmx_load_checkpoint(const char* path, int device, mpmx_ctx** out) {
              std::fread(&cfg, sizeof(cfg), 1, f) == 1 &&
              std::fread(counts.data(), sizeof(int64_t), counts.size(), f) == counts.size();
     }
+    // every scene holds particles and the scenes hold all of them
+    int64_t total = 0;
+    for (size_t k = 0; ok && k < counts.size(); ++k) {
+        ok = counts[k] > 0 && counts[k] <= h.n;
+        total += ok ? counts[k] : 0;
+    }
+    ok = ok && total == h.n && cfg.precision == h.precision;
+    // the raw buffers, then a context sized by them
+    const size_t n = ok ? (size_t)h.n : 0;
+    const bool mixed = h.precision != MPMX_FP32;
+    const size_t fbytes = h.precision == MPMX_MIXED_F16 ? 18 : 36;
+    const size_t sizes[11] = {12 * n, (mixed ? 6 : 12) * n, (mixed ? 18 : 36) * n, fbytes * n,
+                              (mixed ? 2 : 4) * n, (mixed ? 2 : 4) * n, (mixed ? 2 : 4) * n, (mixed ? 2 : 4) * n,
+                              4 * n, 4 * n, 4 * n};
+    size_t need = 0;
+    for (size_t s : sizes) need += s;
+    if (ok) ok = (size_t)std::ftell(f) + need == (size_t)fsize;
     if (!ok) {
         std::fclose(f);
         return MPMX_ERR_ARG;
@@ -152,12 +178,6 @@ mpmx_status mpmx_load_checkpoint(const char* path, int device, mpmx_ctx** out) {
     cfg.device = device;
     cfg.max_pages = h.cap_tp;   // the capacity the run had
     cfg.max_particles = 0;
-    // the raw buffers, then a context sized by them
-    const size_t n = (size_t)h.n;
-    const bool mixed = h.precision == MPMX_MIXED;
-    const size_t sizes[11] = {12 * n, (mixed ? 6 : 12) * n, (mixed ? 18 : 36) * n, (mixed ? 18 : 36) * n,
-                              (mixed ? 2 : 4) * n, (mixed ? 2 : 4) * n, (mixed ? 2 : 4) * n, (mixed ? 2 : 4) * n,
-                              4 * n, 4 * n, 4 * n};
     std::vector<std::vector<char>> raw(11);
     for (int k = 0; k < 11 && ok; ++k) {
         raw[(size_t)k].resize(sizes[k]);
@@ -200,6 +220,16 @@ mpmx_status mpmx_load_checkpoint(const char* path, int device, mpmx_ctx** out) {
     c->cg_spec = (int)(h.cg_spec > 0 ? h.cg_spec : 1);
     *out = c;
     return MPMX_OK;
+}
+
+mpmx_status mpmx_load_checkpoint(const char* path, int device, mpmx_ctx** out) {
+    if (!path || !out) return MPMX_ERR_ARG;
+    *out = nullptr;
+    try {
+        return load_checkpoint(path, device, out);
+    } catch (...) {   // host allocation failures stay on this side of the ABI
+        return MPMX_ERR_ARG;
+    }
 }
 
 // PLY (binary, little-endian) of every particle in upload order: position

@@ -181,6 +181,7 @@ __global__ __launch_bounds__(256) void k_p2g_prep(DevParams dp, ST s, WaveMap wm
                                                   const float* __restrict__ FP, const float* __restrict__ vol,
                                                   BlockTables t, float* __restrict__ pay, unsigned* __restrict__ pbound,
                                                   float dt, int n, const DevCounts* __restrict__ dc) {
+    dt = substep_dt(dt, dc);
     int i, scene;
     const bool valid = wave_particle(wm, n, i, scene);
     float b[kP2GCh];
@@ -222,6 +223,7 @@ __global__ __launch_bounds__(256, 4) void k_p2g_block(DevParams dp, const float*
     __shared__ unsigned long long acc[kP2GCh * kAccStride];
     __shared__ BlockClasses S;
     __shared__ int sexp[kP2GCh];
+    dt = substep_dt(dt, dc);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
@@ -306,6 +308,7 @@ __global__ __launch_bounds__(256) void k_grid_update(DevParams dp, const uint32_
                                                      float* __restrict__ part_rho, float dt, int64_t cap_tp,
                                                      const DevCounts* __restrict__ dc) {
     __shared__ float lds[4];
+    dt = substep_dt(dt, dc);
     int nt = dc->num_total;
     nt = nt < cap_tp ? nt : (int)cap_tp;
     const int nblk = (nt + 1) / 2;
@@ -438,7 +441,7 @@ hipError_t launch_p2g(mpmx_ctx* c, float dt) {
         c->grid_fused = false;
         return launch_p2g_scatter(c, dt);
     }
-    e = (c->cfg.precision == MPMX_MIXED) ? gather_dispatch(c, c->ms[c->cur], dt) : gather_dispatch(c, c->ps[c->cur], dt);
+    e = (c->cfg.precision != MPMX_FP32) ? gather_dispatch(c, c->ms[c->cur], dt) : gather_dispatch(c, c->ps[c->cur], dt);
     c->grid_fused = true;
     return e;
 }
@@ -453,7 +456,7 @@ hipError_t launch_stress_p2g(mpmx_ctx* c, float dt) {
     c->order_ok = true;   // the stress pass orders the blocks
     MPMX_TIMED(c, "stress_prep", e = launch_stress_prep(c, dt));
     if (e != hipSuccess) return e;
-    const bool mixed = c->cfg.precision == MPMX_MIXED;
+    const bool mixed = c->cfg.precision != MPMX_FP32;
     if (c->P.transfer == 0) e = mixed ? gather_blocks_t<0, MState>(c, dt) : gather_blocks_t<0, PState>(c, dt);
     else e = mixed ? gather_blocks_t<1, MState>(c, dt) : gather_blocks_t<1, PState>(c, dt);
     c->grid_fused = true;

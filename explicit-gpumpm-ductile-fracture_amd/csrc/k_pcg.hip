@@ -540,10 +540,10 @@ static void apply_blocks(mpmx_ctx* c, ST st) {
 static hipError_t pcg_iteration(mpmx_ctx* c, int k) {
     const bool blocks = c->cfg.p2g_variant != 1;
     if (blocks)
-        MPMX_TIMED(c, "cg_apply", (c->cfg.precision == MPMX_MIXED) ? apply_blocks(c, c->ms[c->cur])
+        MPMX_TIMED(c, "cg_apply", (c->cfg.precision != MPMX_FP32) ? apply_blocks(c, c->ms[c->cur])
                                                                    : apply_blocks(c, c->ps[c->cur]));
     else
-        MPMX_TIMED(c, "cg_apply", (c->cfg.precision == MPMX_MIXED) ? apply_block(c, c->ms[c->cur])
+        MPMX_TIMED(c, "cg_apply", (c->cfg.precision != MPMX_FP32) ? apply_block(c, c->ms[c->cur])
                                                                    : apply_block(c, c->ps[c->cur]));
     const int nb = c->cg_nb;
     // the block P2G records which nodes are active; the page-tile variant's

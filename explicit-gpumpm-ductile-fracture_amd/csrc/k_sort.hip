@@ -193,7 +193,8 @@ __global__ void k_keys(DevParams dp, const float* __restrict__ x, uint32_t* __re
     const DevParams& sp = dp.scenes[lo];
     const int N = sp.N;
     const float x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
-    const float lo_x = 2.0f / N, hi_x = 1.0f - 2.0f / N;
+    // the same float bounds as the G2P's domain test and the upload's host check
+    const float lo_x = 2.0f * sp.dx, hi_x = 1.0f - 2.0f * sp.dx;
     if (!(x0 >= lo_x && x0 <= hi_x && x1 >= lo_x && x1 <= hi_x && x2 >= lo_x && x2 <= hi_x))
         atomicOr(&dc->err, kErrDomain);
     keys[i] = key_of(base_node(x0, N) + sp.origin[0], base_node(x1, N) + sp.origin[1],
@@ -202,7 +203,7 @@ __global__ void k_keys(DevParams dp, const float* __restrict__ x, uint32_t* __re
 
 hipError_t compute_keys(mpmx_ctx* c) {
     const int n = (int)c->n;
-    const float* x = c->cfg.precision == MPMX_MIXED ? c->ms[c->cur].x : c->ps[c->cur].x;
+    const float* x = c->cfg.precision != MPMX_FP32 ? c->ms[c->cur].x : c->ps[c->cur].x;
     k_keys<<<(n + 255) / 256, 256, 0, c->stream>>>(c->dp, x, c->keys, n, c->scene_start, c->dc);
     return hipGetLastError();
 }
@@ -306,7 +307,8 @@ __global__ void k_gather_mixed(const uint32_t* __restrict__ perm, MState s, MSta
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
         d.B[(int64_t)a * n + i] = s.B[(int64_t)a * n + o];
-        d.F[(int64_t)a * n + i] = s.F[(int64_t)a * n + o];
+        if (s.F32) d.F32[(int64_t)a * n + i] = s.F32[(int64_t)a * n + o];
+        else d.F[(int64_t)a * n + i] = s.F[(int64_t)a * n + o];
     }
     d.c[i] = s.c[o];
     d.alpha[i] = s.alpha[o];
@@ -344,7 +346,7 @@ hipError_t sort_and_gather(mpmx_ctx* c) {
     const uint32_t* perm = vi < 0 ? nullptr : vb[vi];
     if (!perm) return hipErrorInvalidValue;   // no pass ran: key_bits is at least 1
     const int nxt = 1 - c->cur;
-    if (c->cfg.precision == MPMX_MIXED)
+    if (c->cfg.precision != MPMX_FP32)
         k_gather_mixed<<<(n + 255) / 256, 256, 0, c->stream>>>(perm, c->ms[c->cur], c->ms[nxt], n);
     else
         k_gather_fp32<<<(n + 255) / 256, 256, 0, c->stream>>>(perm, c->ps[c->cur], c->ps[nxt], n);

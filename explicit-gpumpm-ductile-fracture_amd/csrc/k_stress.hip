@@ -308,7 +308,8 @@ __device__ __forceinline__ void stress_load(const MState& s, int i, int n, Stres
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc)
-            in.F[r][cc] = h2f(s.F[(int64_t)(r + 3 * cc) * n + i]) + (r == cc ? 1.0f : 0.0f);
+            in.F[r][cc] = s.F32 ? s.F32[(int64_t)(r + 3 * cc) * n + i]
+                                : h2f(s.F[(int64_t)(r + 3 * cc) * n + i]) + (r == cc ? 1.0f : 0.0f);
     in.c = bf2f(s.c[i]);
     in.g = bf2f(s.g[i]);
     in.alpha = bf2f(s.alpha[i]);
@@ -327,15 +328,19 @@ __device__ __forceinline__ void stress_store(const MState& s, int i, int n, cons
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc)
-            s.F[(int64_t)(r + 3 * cc) * n + i] = f2h(out.F[r][cc] - (r == cc ? 1.0f : 0.0f));
+        for (int cc = 0; cc < 3; ++cc) {
+            if (s.F32) s.F32[(int64_t)(r + 3 * cc) * n + i] = out.F[r][cc];
+            else s.F[(int64_t)(r + 3 * cc) * n + i] = f2h(out.F[r][cc] - (r == cc ? 1.0f : 0.0f));
+        }
     s.g[i] = f2bf(out.g);
     s.alpha[i] = f2bf(out.alpha);
     s.psi[i] = f2bf(out.psi);
 }
 
 // stress of particle i: reads its F and phase-field state, writes back F,
-// g, alpha, psi and stores sigma (component r + 3 c), FP and V
+// g, alpha, psi and stores V and, where sigma is not null, sigma (component
+// r + 3 c) and FP (the fused stress pass carries both in the P2G payload and
+// stores them only when the caller keeps them)
 template <bool ALL64, typename ST>
 __device__ __forceinline__ void stress_particle(const DevParams& sp, const ST& s, int i, int n,
                                                 float* __restrict__ sigma, float* __restrict__ FP,
@@ -348,11 +353,11 @@ __device__ __forceinline__ void stress_particle(const DevParams& sp, const ST& s
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) {
-            sigma[(int64_t)(r + 3 * cc) * n + i] = out.sigma[r][cc];
+            if (sigma) sigma[(int64_t)(r + 3 * cc) * n + i] = out.sigma[r][cc];
             finite = finite && isfinite(out.sigma[r][cc]);
         }
     stress_store(s, i, n, out);
-    FP[i] = out.FP;
+    if (sigma) FP[i] = out.FP;
     vol[i] = out.vol;
     if (!finite) atomicOr(&dc->err, kErrNonFinite);
 }
@@ -381,6 +386,7 @@ __global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, Wave
                                                      float* __restrict__ pay, unsigned* __restrict__ pbound, float dt,
                                                      int n, const BlockClasses* __restrict__ lists,
                                                      int* __restrict__ order, DevCounts* dc) {
+    dt = substep_dt(dt, dc);
     // the last 8 workgroups order the blocks of the 8 XCD shares for the
     // block kernels that follow (k_block_lists has just listed them)
     if ((int)blockIdx.x >= (int)gridDim.x - kOrderGroups) {
@@ -415,7 +421,7 @@ __global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, Wave
 template <bool ALL64>
 static void stress_dispatch(mpmx_ctx* c) {
     const int n = (int)c->n;
-    if (c->cfg.precision == MPMX_MIXED)
+    if (c->cfg.precision != MPMX_FP32)
         k_stress<ALL64, MState><<<particle_blocks(c), c->particle_wg, 0, c->stream>>>(c->dp, c->ms[c->cur], c->wmap, c->sigma,
                                                                            c->FP, c->vol, n, c->dc);
     else
@@ -424,6 +430,7 @@ static void stress_dispatch(mpmx_ctx* c) {
 }
 
 hipError_t launch_stress(mpmx_ctx* c) {
+    c->stress_fields = true;
     if (c->cfg.stress_fp64) stress_dispatch<true>(c);
     else stress_dispatch<false>(c);
     return hipGetLastError();
@@ -432,8 +439,8 @@ hipError_t launch_stress(mpmx_ctx* c) {
 template <bool ALL64, int TRANSFER, typename ST>
 static void stress_prep_dispatch(mpmx_ctx* c, ST st, float dt) {
     k_stress_prep<ALL64, TRANSFER, ST><<<particle_blocks(c) + kOrderGroups, c->particle_wg, 0, c->stream>>>(
-        c->dp, st, c->wmap, c->sigma, c->FP, c->vol, block_tables_of(c), c->payload, c->pbound, dt, (int)c->n,
-        static_cast<const BlockClasses*>(c->block_lists), c->block_order, c->dc);
+        c->dp, st, c->wmap, c->cfg.store_stress ? c->sigma : nullptr, c->FP, c->vol, block_tables_of(c), c->payload,
+        c->pbound, dt, (int)c->n, static_cast<const BlockClasses*>(c->block_lists), c->block_order, c->dc);
 }
 
 template <bool ALL64, typename ST>
@@ -444,11 +451,12 @@ static void stress_prep_dispatch(mpmx_ctx* c, ST st, float dt) {
 
 template <bool ALL64>
 static void stress_prep_dispatch(mpmx_ctx* c, float dt) {
-    if (c->cfg.precision == MPMX_MIXED) stress_prep_dispatch<ALL64>(c, c->ms[c->cur], dt);
+    if (c->cfg.precision != MPMX_FP32) stress_prep_dispatch<ALL64>(c, c->ms[c->cur], dt);
     else stress_prep_dispatch<ALL64>(c, c->ps[c->cur], dt);
 }
 
 hipError_t launch_stress_prep(mpmx_ctx* c, float dt) {
+    c->stress_fields = c->cfg.store_stress != 0;
     if (c->cfg.stress_fp64) stress_prep_dispatch<true>(c, dt);
     else stress_prep_dispatch<false>(c, dt);
     return hipGetLastError();

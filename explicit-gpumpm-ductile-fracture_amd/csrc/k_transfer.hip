@@ -28,6 +28,7 @@ __global__ __launch_bounds__(BLOCK) void k_p2g(DevParams dp, ST s, const float* 
                                                Tables t, float* __restrict__ grid, float dt, int n,
                                                const DevCounts* __restrict__ dc) {
     __shared__ float tile[kP2GCh * kTileN];
+    dt = substep_dt(dt, dc);
     int nvp = dc->num_vp;
     nvp = nvp < t.cap_vp ? nvp : (int)t.cap_vp;
     int vb, ve;
@@ -259,6 +260,7 @@ __global__ __launch_bounds__(BLOCK) void k_g2p(DevParams dp, ST s, Tables t, con
                                                const float* __restrict__ cgx, uint32_t* __restrict__ keys,
                                                float dt, int n, DevCounts* dc) {
     __shared__ float tv[4 * kTileN];
+    dt = substep_dt(dt, dc);
     int nvp = dc->num_vp;
     nvp = nvp < t.cap_vp ? nvp : (int)t.cap_vp;
     const bool breakdown = dc->cg_breakdown != 0;
@@ -306,6 +308,7 @@ __global__ __launch_bounds__(256) void k_g2p_block(DevParams dp, ST s, BlockTabl
                                                    const float* __restrict__ cgx, uint32_t* __restrict__ keys,
                                                    float dt, int n, const SceneCG* __restrict__ scg, DevCounts* dc) {
     __shared__ float tv[4 * kBR3];
+    dt = substep_dt(dt, dc);
     int nb = dc->num_blocks;
     nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
     FOR_LOGICAL_ITEMS(item, nb) {
@@ -399,7 +402,7 @@ static void p2g_block(mpmx_ctx* c, ST st, float dt) {
 hipError_t launch_p2g_scatter(mpmx_ctx* c, float dt) {
     hipError_t e = fill_zero_dev_count(c, c->grid, &c->dc->num_total, kPageFloats, c->cap_tp);
     if (e != hipSuccess) return e;
-    if (c->cfg.precision == MPMX_MIXED) p2g_block(c, c->ms[c->cur], dt);
+    if (c->cfg.precision != MPMX_FP32) p2g_block(c, c->ms[c->cur], dt);
     else p2g_block(c, c->ps[c->cur], dt);
     return hipGetLastError();
 }
@@ -443,10 +446,10 @@ static void g2p_block_kernel(mpmx_ctx* c, ST st, float dt) {
 
 hipError_t launch_g2p(mpmx_ctx* c, float dt) {
     if (c->cfg.p2g_variant == 1) {
-        if (c->cfg.precision == MPMX_MIXED) g2p_block(c, c->ms[c->cur], dt);
+        if (c->cfg.precision != MPMX_FP32) g2p_block(c, c->ms[c->cur], dt);
         else g2p_block(c, c->ps[c->cur], dt);
     } else {
-        if (c->cfg.precision == MPMX_MIXED) g2p_block_kernel(c, c->ms[c->cur], dt);
+        if (c->cfg.precision != MPMX_FP32) g2p_block_kernel(c, c->ms[c->cur], dt);
         else g2p_block_kernel(c, c->ps[c->cur], dt);
     }
     return hipGetLastError();

@@ -113,9 +113,15 @@ struct DevCounts {
     int svd_fallbacks;
     int num_blocks;     // occupied 2x2x2-page blocks
     int cg_active;      // scenes whose PCG is still running
-    int pad[18];
+    float dt;           // time step of the substeps launched with dt <= 0 (set by the CFL kernels)
+    int pad[15];
+    double time;        // simulated time summed by the CFL kernels
     unsigned seq;       // host-published copies: written last, names the copy
 };
+
+// time step of a substep kernel: the launch argument, or (argument <= 0) the
+// step the device-side CFL computed for this substep
+__device__ __forceinline__ float substep_dt(float dt, const DevCounts* dc) { return dt > 0.f ? dt : dc->dt; }
 
 struct PState {
     float* x = nullptr;
@@ -130,12 +136,15 @@ struct PState {
     int* id = nullptr;
 };
 
-// mixed-precision particle storage: x float AoS, F as half(F-I), the rest bf16
+// mixed-precision particle storage: x float AoS, F float (MPMX_MIXED) or
+// half(F - I) (MPMX_MIXED_F16), the rest bf16.  Exactly one of F32 and F is
+// set; kernels branch on F32 (a kernel argument, so the branch is uniform).
 struct MState {
     float* x = nullptr;
     uint16_t* v = nullptr;   // bf16[3n] AoS
     uint16_t* B = nullptr;   // bf16[9n]
-    uint16_t* F = nullptr;   // fp16[9n] of F - I
+    uint16_t* F = nullptr;   // fp16[9n] of F - I (MPMX_MIXED_F16)
+    float* F32 = nullptr;    // float[9n] (MPMX_MIXED)
     uint16_t* c = nullptr;   // bf16
     uint16_t* alpha = nullptr;
     uint16_t* psi = nullptr;
@@ -185,6 +194,9 @@ struct mpmx_ctx {
     float* sigma = nullptr;
     float* FP = nullptr;
     float* vol = nullptr;
+    bool stress_fields = false;      // sigma and FP hold the last stress update
+    double* red_part = nullptr;      // [kRedParts * 5 + 8] partials and results of the particle reductions
+                                     // (CFL speed, FP64 conservation sums), for the context's lifetime
     // sort
     uint32_t* keys = nullptr;
     uint32_t* keys_alt = nullptr;

@@ -2,10 +2,13 @@
 // reference's viewer drives them (gl_main.cpp:87-94 init, :304-327 one
 // simulateStick per frame), minus the OpenGL window.
 //
-//   mpmx_demo <scene> <frames> <seed> <scale> <out.bin>
-// writes a writePartio dump of the final state.
+//   mpmx_demo <scene> <frames> <seed> <scale> <out.bin> [cfl]
+// writes a writePartio dump of the final state.  With "cfl" every frame takes
+// the CFL step of computeDt (0.3 dx / max|v|, at most the scene's dt) and the
+// per-frame copy of the particles to the host is off.
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 #include "mpmx_classes.h"
 
@@ -15,11 +18,16 @@ int main(int argc, char** argv) {
     const unsigned long long seed = argc > 3 ? std::strtoull(argv[3], nullptr, 10) : 1ull;
     const double scale = argc > 4 ? std::atof(argv[4]) : 1.0;
     const char* out = argc > 5 ? argv[5] : "mpmx_demo.bin";
+    const bool cfl = argc > 6 && std::string(argv[6]) == "cfl";
     try {
         MPMSimulator sim;
         if (!sim.build((unsigned)scene, seed, scale)) {
             std::fprintf(stderr, "unknown scene %d\n", scene);
             return 2;
+        }
+        if (cfl) {
+            sim.setUseCFL(true);
+            sim.setReadback(false);
         }
         float cfl = 0, pre = 0, simu = 0;
         for (int f = 0; f < frames; ++f) {

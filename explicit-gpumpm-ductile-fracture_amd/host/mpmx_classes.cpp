@@ -62,14 +62,17 @@ void Particles::initialize(mpmx_ctx* ctx, unsigned int numParticle, const T* h_m
 void Particles::refresh() {
     mpmx_device_ptrs p;
     CHK(_ctx, mpmx_get_device_pointers(_ctx, &p));
+    // the float members point at float buffers only: in the mixed forms the
+    // BF16 / FP16 fields are reached through mpmx_get_device_pointers
+    const bool f32 = p.precision == MPMX_FP32;
     d_pos = (float*)p.pos;
-    d_orderedVel = (float*)p.vel;
-    d_F = (float*)p.F;
-    d_B = (float*)p.B;
-    d_phase_C = (float*)p.phase_c;
-    d_maxPsi = (float*)p.psi;
-    d_g = (float*)p.g;
-    d_alpha = (float*)p.alpha;
+    d_orderedVel = f32 ? (float*)p.vel : nullptr;
+    d_F = p.precision != MPMX_MIXED_F16 ? (float*)p.F : nullptr;
+    d_B = f32 ? (float*)p.B : nullptr;
+    d_phase_C = f32 ? (float*)p.phase_c : nullptr;
+    d_maxPsi = f32 ? (float*)p.psi : nullptr;
+    d_g = f32 ? (float*)p.g : nullptr;
+    d_alpha = f32 ? (float*)p.alpha : nullptr;
     d_FP = p.FP;
     d_vol = p.vol;
     d_mass = (float*)p.mass;
@@ -261,7 +264,9 @@ void MPMSimulator::simulateStick(float* cflTime, float* preTime, float* simuTime
     _currentTime += dt;
     ++_currentFrame;
     h_pGrid->refresh();
-    getModelParticlePtr()->Attribs_Back_Host(h_pos, h_vel, h_indices);
+    // the reference copies the particles back every frame for its viewer
+    // (Simulator.cpp:253); a caller without a viewer turns that off
+    if (_readback) getModelParticlePtr()->Attribs_Back_Host(h_pos, h_vel, h_indices);
 }
 
 void MPMSimulator::writePartio(const std::string& filename) {

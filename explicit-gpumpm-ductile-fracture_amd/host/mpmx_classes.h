@@ -208,6 +208,12 @@ public:
     T computeDt(const T cur, const T next);                                       // Simulator.cpp:200-225
     void simulateStick(float* cflTime, float* preTime, float* simuTime, int type);  // Simulator.cpp:227-259
     void writePartio(const std::string& filename);   // declared in the reference (Simulator.h:35): binary dump
+    // no counterpart in the reference: simulateStick's per-frame copy of the
+    // particles to h_pos (on by default, as the reference does) and its CFL
+    // step (computeDt on the device, off by default: the reference's viewer
+    // loop uses the fixed dtDefault unless asked)
+    void setReadback(bool on) { _readback = on; }
+    void setUseCFL(bool on) { _useCFL = on; }
 
     std::vector<vector3T> h_pos;
     unsigned int numParticle = 0;
@@ -219,6 +225,7 @@ private:
     T _frameRate = 48;
     T _dtDefault = 1e-4;
     bool _useCFL = false;
+    bool _readback = true;
     mpmx_params _params;
     mpmx_ctx* _ctx = nullptr;
     std::unique_ptr<SPGrid> h_pGrid;

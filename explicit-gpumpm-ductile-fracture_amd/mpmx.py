@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libmpmx_hip.so")
 
 OK, ERR_HIP, ERR_ARG, ERR_CAPACITY, ERR_OUT_OF_DOMAIN, ERR_NONFINITE, ERR_CG_BREAKDOWN, ERR_NO_DEVICE = range(8)
-FP32, MIXED = 0, 1
+FP32, MIXED, MIXED_F16 = 0, 1, 2
 FIELDS = {"x": (0, 3), "v": (1, 3), "F": (2, 9), "B": (3, 9), "c": (4, 1), "alpha": (5, 1),
           "psi": (6, 1), "g": (7, 1), "mass": (8, 1), "FP": (9, 1), "vol": (10, 1), "sigma": (11, 9)}
 
@@ -30,7 +30,7 @@ EXPORTS = [
     "mpmx_max_velocity_sq", "mpmx_create_batch", "mpmx_set_particles_batch", "mpmx_num_scenes",
     "mpmx_get_scene_stats", "mpmx_scene_batch", "mpmx_read_obj_points", "mpmx_sample_sdf",
     "mpmx_write_particles", "mpmx_save_checkpoint", "mpmx_load_checkpoint", "mpmx_get_tuning",
-    "mpmx_set_tuning", "mpmx_autotune",
+    "mpmx_set_tuning", "mpmx_autotune", "mpmx_run_cfl",
 ]
 
 
@@ -57,14 +57,15 @@ class Params(C.Structure):
 class Config(C.Structure):
     _fields_ = [("max_particles", C.c_int64), ("max_pages", C.c_int64), ("device", C.c_int),
                 ("precision", C.c_int), ("svd_gap", C.c_double), ("cg_check_every", C.c_int),
-                ("p2g_variant", C.c_int), ("block", C.c_int), ("stress_fp64", C.c_int), ("use_graph", C.c_int)]
+                ("p2g_variant", C.c_int), ("block", C.c_int), ("stress_fp64", C.c_int), ("use_graph", C.c_int),
+                ("store_stress", C.c_int)]
 
 
 class Stats(C.Structure):
     _fields_ = [("num_particles", C.c_int64), ("num_page", C.c_int), ("num_total_page", C.c_int),
                 ("num_virtual_page", C.c_int), ("cg_iters", C.c_int), ("cg_breakdown", C.c_int),
                 ("substeps", C.c_int), ("cg_norm", C.c_double), ("mass", C.c_double),
-                ("momentum", C.c_double * 3), ("kinetic_energy", C.c_double)]
+                ("momentum", C.c_double * 3), ("kinetic_energy", C.c_double), ("svd_fallbacks", C.c_int64)]
 
 
 class SceneStats(C.Structure):
@@ -131,6 +132,7 @@ def lib():
         "mpmx_substep": ([vp, C.c_double], C.c_int),
         "mpmx_run": ([vp, C.c_int, C.c_double], C.c_int),
         "mpmx_compute_dt": ([vp, C.c_double, C.c_double, C.POINTER(C.c_double)], C.c_int),
+        "mpmx_run_cfl": ([vp, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)], C.c_int),
         "mpmx_sync": ([vp], C.c_int),
         "mpmx_get_field": ([vp, C.c_int, f64p, C.c_size_t], C.c_int),
         "mpmx_set_field": ([vp, C.c_int, f64p, C.c_size_t], C.c_int),
@@ -365,6 +367,17 @@ class Context:
         self._chk(self.L.mpmx_compute_dt(self.h, dt_default, cfl, C.byref(out)))
         return out.value
 
+    def max_velocity_sq(self):
+        out = C.c_double()
+        self._chk(self.L.mpmx_max_velocity_sq(self.h, C.byref(out)))
+        return out.value
+
+    def run_cfl(self, nsteps, dt_default, cfl=0.3, wave=False):
+        """nsteps substeps, each with the device-side CFL step; returns the simulated time."""
+        t = C.c_double()
+        self._chk(self.L.mpmx_run_cfl(self.h, nsteps, dt_default, cfl, 1 if wave else 0, C.byref(t)))
+        return t.value
+
     def get(self, name):
         fid, w = FIELDS[name]
         out = np.zeros(self.n * w)
@@ -396,7 +409,10 @@ class Context:
                 "numTotalPage": s.num_total_page, "numVirtualPage": s.num_virtual_page,
                 "cg_iters": s.cg_iters, "cg_breakdown": s.cg_breakdown, "substeps": s.substeps,
                 "cg_norm": s.cg_norm, "mass": s.mass, "momentum": list(s.momentum),
-                "kinetic_energy": s.kinetic_energy}
+                "kinetic_energy": s.kinetic_energy, "svd_fallbacks": s.svd_fallbacks}
+
+    def svd_fallbacks(self):
+        return int(self.stats()["svd_fallbacks"])
 
     def tables(self):
         st = self.stats()

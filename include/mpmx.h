@@ -18,8 +18,14 @@
  *    caller-owned and copied.
  *  - Stage calls are asynchronous on the context stream; mpmx_get_*,
  *    mpmx_sync and mpmx_destroy synchronise.  Device-side errors (particle left
- *    the domain, non-finite state, table capacity exceeded, CG breakdown) are
- *    latched in a device error word and reported at the next synchronising call.
+ *    the domain, non-finite state, table capacity exceeded) are latched in a
+ *    device error word and reported at the next synchronising call.  A PCG
+ *    breakdown is not an error: as in the reference (ExplicitTimeIntegrator.cu:
+ *    959-962) the substep goes on with the damage unchanged, and the breakdown
+ *    is counted in mpmx_stats.cg_breakdown / mpmx_scene_stats.cg_breakdown.
+ *  - A call that fails after it has started to replace the particle state
+ *    (mpmx_set_particles*) leaves the context empty: the stage calls then
+ *    return MPMX_ERR_ARG until particles are set again.
  *  - One context per host thread.
  *  - Particle fields are returned in the caller's original particle order.
  *    3x3 matrices are column-major per particle: M(r,c) = m[r + 3c].
@@ -40,7 +46,7 @@ typedef enum mpmx_status {
     MPMX_ERR_CAPACITY = 3,       /* page or virtual-page table capacity exceeded  */
     MPMX_ERR_OUT_OF_DOMAIN = 4,  /* a particle left [2dx, 1-2dx]                  */
     MPMX_ERR_NONFINITE = 5,      /* NaN/Inf in particle state                     */
-    MPMX_ERR_CG_BREAKDOWN = 6,   /* phase-field PCG hit rho_old == 0              */
+    MPMX_ERR_CG_BREAKDOWN = 6,   /* reserved; a breakdown is counted, not returned */
     MPMX_ERR_NO_DEVICE = 7       /* no usable HIP device                          */
 } mpmx_status;
 
@@ -77,8 +83,11 @@ typedef struct mpmx_params {
 
 typedef enum mpmx_precision {
     MPMX_FP32 = 0,       /* FP32 particle state and grid                          */
-    MPMX_MIXED = 1       /* x FP32, F as FP16(F-I), v/C/c/alpha/psi/g BF16, FP32 grid;
-                            stress arithmetic as mpmx_config.stress_fp64 says       */
+    MPMX_MIXED = 1,      /* x and F FP32, v/C/c/alpha/psi/g BF16, FP32 grid; stress
+                            arithmetic as mpmx_config.stress_fp64 says (88 B of
+                            state per particle against 120 B in FP32)              */
+    MPMX_MIXED_F16 = 2   /* as MPMX_MIXED with F stored as FP16(F - I) (70 B per
+                            particle; one store rounds F by up to 2^-11 |F - I|)    */
 } mpmx_precision;
 
 typedef struct mpmx_config {
@@ -106,6 +115,12 @@ typedef struct mpmx_config {
                                eagerly.  Results are identical either way.  Default 0:
                                on ROCm 7.2 the replays were 1-2 % slower than the direct
                                launches for this kernel sequence                       */
+    int     store_stress;   /* 1: the fused stress pass of mpmx_substep / mpmx_run also
+                               stores each particle's sigma and FP for mpmx_get_field
+                               (40 B per particle per substep that the substep itself
+                               does not need); 0 (default): only mpmx_stress stores
+                               them, and mpmx_get_field of MPMX_FIELD_SIGMA / _FP after a
+                               fused substep returns MPMX_ERR_ARG                      */
 } mpmx_config;
 
 typedef struct mpmx_ctx mpmx_ctx;
@@ -122,6 +137,8 @@ typedef struct mpmx_stats {
     double  mass;              /* FP64 sums over particles: mass and momentum                    */
     double  momentum[3];
     double  kinetic_energy;
+    int64_t svd_fallbacks;     /* stress_fp64 = 0: particle updates redone in FP64 for a small
+                                  singular-value gap, since the particles were set             */
 } mpmx_stats;
 
 /* one scene of a batch context (mpmx_get_scene_stats) */
@@ -214,10 +231,23 @@ mpmx_status mpmx_substep(mpmx_ctx* ctx, double dt);
 /* nsteps substeps with fixed dt (no host synchronisation in between unless the
  * PCG needs a convergence check) */
 mpmx_status mpmx_run(mpmx_ctx* ctx, int nsteps, double dt);
-/* Particles::getMaxParticleVel_c (Particle_kernal.cu:421-440): max |v|^2 over particles */
+/* Particles::getMaxParticleVel_c (Particle_kernal.cu:421-440): max |v|^2 over
+ * particles (one FP32 value; synchronises) */
 mpmx_status mpmx_max_velocity_sq(mpmx_ctx* ctx, double* out);
-/* MPMSimulator::computeDt (Simulator.cpp:200-225): min(dt_default, cfl*dx/max|v|) */
+/* MPMSimulator::computeDt (Simulator.cpp:200-225): min(dt_default, cfl/(N max|v|)),
+ * max over the particles of every scene with each scene's own N.  The
+ * reduction runs on the device (partials in context memory, no allocation per
+ * call); the result is also left on the device as the step of the next
+ * substep launched with dt <= 0.  Synchronises to return *dt_out. */
 mpmx_status mpmx_compute_dt(mpmx_ctx* ctx, double dt_default, double cfl, double* dt_out);
+/* MPMSimulator::simulateStick's computeDt + integrate loop (Simulator.cpp:
+ * 227-259) without host round trips: before each of nsteps substeps a device
+ * reduction sets dt = min(dt_default, cfl / max_p((|v_p| + wave c_p) N_s)),
+ * with c_p = sqrt((lambda + 2 mu) V0 / m_p) the particle's elastic wave speed
+ * when wave = 1 (0: speed only, the reference's rule), and the substep's
+ * kernels read that dt on the device.  *time_out (may be NULL) receives the
+ * simulated time of the nsteps substeps (one synchronisation at the end). */
+mpmx_status mpmx_run_cfl(mpmx_ctx* ctx, int nsteps, double dt_default, double cfl, int wave, double* time_out);
 
 mpmx_status mpmx_sync(mpmx_ctx* ctx);
 

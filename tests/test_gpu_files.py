@@ -14,13 +14,13 @@ def state(ctx):
     return {k: ctx.get(k) for k in FIELDS}
 
 
-@pytest.mark.parametrize("kind", ["single_fp32", "single_mixed", "batch_fp32"])
+@pytest.mark.parametrize("kind", ["single_fp32", "single_mixed", "single_mixed_f16", "batch_fp32"])
 def test_checkpoint_resume_is_exact(mpmx, tmp_path, kind):
     """Run 6 substeps, checkpoint, run 6 more; a context loaded from the
     checkpoint runs the same 6 and ends bit-identical."""
     if kind.startswith("single"):
         p, x, v, m, dt = mpmx.scene(3, seed=1, scale=0.1)
-        prec = mpmx.MIXED if kind.endswith("mixed") else mpmx.FP32
+        prec = {"fp32": mpmx.FP32, "mixed": mpmx.MIXED, "f16": mpmx.MIXED_F16}[kind.rsplit("_", 1)[1]]
         ctx = mpmx.Context(p, mpmx.default_config(precision=prec))
         ctx.set_particles(x, v, m)
     else:
@@ -81,3 +81,25 @@ def test_autotune_changes_no_result(mpmx):
     got, want = state(a), state(b)
     for k in FIELDS:
         assert np.array_equal(got[k], want[k]), k
+
+
+def test_checkpoint_rejects_inconsistent_files(mpmx, tmp_path):
+    """A truncated checkpoint or one whose scene counts do not add up to its
+    particle count is refused before anything is allocated from it."""
+    import struct
+
+    p, x, v, m, dt = mpmx.scene(3, seed=1, scale=0.1)
+    ctx = mpmx.Context(p)
+    ctx.set_particles(x, v, m)
+    path = tmp_path / "run.ckpt"
+    ctx.save_checkpoint(str(path))
+    raw = path.read_bytes()
+    (tmp_path / "short.ckpt").write_bytes(raw[:-100])
+    # header: magic[8], int32 version, num_scenes, precision, sorted, int64 n, ...
+    bad_n = raw[:24] + struct.pack("<q", 1 << 29) + raw[32:]
+    (tmp_path / "badn.ckpt").write_bytes(bad_n)
+    for name in ("short.ckpt", "badn.ckpt"):
+        with pytest.raises(mpmx.MpmxError) as e:
+            mpmx.Context.load_checkpoint(str(tmp_path / name))
+        assert e.value.status == mpmx.ERR_ARG
+    assert mpmx.Context.load_checkpoint(str(path)).n == ctx.n
