@@ -20,11 +20,15 @@
  *    (the reference numbers them by atomicAdd order, Transformer_kernal.cu:85).
  * Matrices are 3x3 column-major: M(r,c) = m[r + 3c] (reference d_F layout).
  *
+ * Pinned by closed forms (tests/test_oracle_pins.py): the branch quirks of
+ * applyVonMises (compressive trial no flow, the dlambda cap, alpha storage
+ * against the local alpha of G_c, the log-strain rescale), the walls and
+ * grips (mo_node_boundary), the MLS speed clamp, the damage update and the
+ * CFL step; the PCG's block-max "dot" (mo_vdot) in tests/test_oracle.py.
  * Parity unpinned (checked only as a faithful transcription, no golden vector
- * or closed form fixes them): the branch quirks of applyVonMises (trace-based
- * dlambda, clamps, alpha storage), the QR-SVD's choice among equivalent sign
+ * or closed form fixes them): the QR-SVD's choice among equivalent sign
  * conventions beyond what the SVD tests fix, and the reference's float-literal
- * constants.  The PCG's block-max "dot" (mo_vdot) is pinned by closed forms.
+ * constants (their effect is ~1e-8 relative, below every bar).
  */
 #ifndef MPMX_ORACLE_H
 #define MPMX_ORACLE_H
