@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 
 #include "access.h"
@@ -506,6 +507,7 @@ static mpmx_status alloc_grid(mpmx_ctx* c, int64_t cap_tp) {
 
 static mpmx_status upload_particles(mpmx_ctx* c, const int64_t* counts, const std::vector<int>& start, int64_t n,
                                     const double* x, const double* v, const double* mass);
+static hipError_t noalloc_scratch(int device, float** out);
 
 // ---- public API --------------------------------------------------------------
 extern "C" {
@@ -636,6 +638,11 @@ mpmx_status mpmx_create_batch(int num_scenes, const mpmx_params* ps, const mpmx_
         return MPMX_ERR_HIP;
     }
     set_scene_geometry(c);
+    {   // the device address of the allocation-free CFL step's scratch, once
+        // per device (a failure here is reported by that entry point itself)
+        float* s = nullptr;
+        (void)noalloc_scratch(c->device, &s);
+    }
     *out = c;
     return MPMX_OK;
 }
@@ -1064,15 +1071,48 @@ static int reduction_groups(const mpmx_ctx* c) {
 // the CFL speed reduction (k_speed_part / k_speed_final) with its partials
 // in `part` (kRedParts floats) and its two results in `out`
 static hipError_t launch_speed(mpmx_ctx* c, int mode, double dt_default, double cfl, int set_dt, float* part,
-                               double* out) {
+                               double* out, int groups = 0) {
     const bool mixed = c->cfg.precision != MPMX_FP32;
-    const int G = reduction_groups(c);
+    const int G = groups > 0 ? groups : reduction_groups(c);
     k_speed_part<<<G, 256, 0, c->stream>>>(c->dp, mixed ? nullptr : c->ps[c->cur].v,
                                            mixed ? c->ms[c->cur].v : nullptr,
                                            mixed ? c->ms[c->cur].mass : c->ps[c->cur].mass, c->wmap, (int)c->n,
                                            mode, part);
     k_speed_final<<<1, 256, 0, c->stream>>>(part, G, dt_default, cfl, set_dt, c->dc, out);
     return hipGetLastError();
+}
+
+// ---- the allocation-free CFL step (mpmx_compute_dt_noalloc) -----------------
+// Its scratch is this file-scope device array: the speed reduction's partial
+// maxima (one float per workgroup, at most kNoAllocParts) followed by the two
+// FP64 results.  Each GPU holds its own copy for the life of the process;
+// host code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address) but the address hipGetSymbolAddress
+// returns for the device, resolved once per device by the library's init
+// call (mpmx_create_batch) or, failing that, by the first call on the device.
+// Every call on a GPU shares that GPU's array, so calls that overlap on one
+// GPU race; each call writes every partial it reads, so nothing is cleared.
+constexpr int kNoAllocParts = 4096;
+constexpr int kNoAllocPerThread = 32;   // particles per thread of the partial pass at the limit
+constexpr int64_t kNoAllocMaxParticles = (int64_t)kNoAllocParts * 256 * kNoAllocPerThread;   // 33,554,432
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) float scratch[kNoAllocParts + 4];
+static float* g_scratch[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// device address of `scratch` on `device` (the calling thread's current device)
+static hipError_t noalloc_scratch(int device, float** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = static_cast<float*>(p);
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
 }
 
 extern "C" {
@@ -1095,6 +1135,27 @@ mpmx_status mpmx_compute_dt(mpmx_ctx* c, double dt_default, double cfl, double* 
     // Simulator.cpp:213 per scene: max over particles of |v| N_s
     double* res = c->red_part + kRedResult;
     HIPCHK(c, launch_speed(c, 2, dt_default, cfl, 1, reinterpret_cast<float*>(c->red_part), res));
+    double h[2];
+    HIPCHK(c, hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *dt_out = h[1];
+    return MPMX_OK;
+}
+
+mpmx_status mpmx_compute_dt_noalloc(mpmx_ctx* c, double dt_default, double cfl, double* dt_out) {
+    if (!c || !dt_out || c->n == 0 || !(dt_default > 0) || !(cfl > 0)) return MPMX_ERR_ARG;
+    const int64_t threads = c->wmap.first ? c->wmap.waves * 64 : c->n;
+    if (threads > kNoAllocMaxParticles)
+        return fail(c, MPMX_ERR_CAPACITY,
+                    "mpmx_compute_dt_noalloc: more than 33,554,432 particles (its static scratch holds 4096 "
+                    "partials); use mpmx_compute_dt");
+    HIPCHK(c, hipSetDevice(c->device));
+    float* s = nullptr;
+    HIPCHK(c, noalloc_scratch(c->device, &s));
+    const int64_t per = 256 * kNoAllocPerThread;
+    const int G = (int)std::max<int64_t>(1, (threads + per - 1) / per);
+    double* res = reinterpret_cast<double*>(s + kNoAllocParts);
+    HIPCHK(c, launch_speed(c, 2, dt_default, cfl, 1, s, res, G));
     double h[2];
     HIPCHK(c, hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

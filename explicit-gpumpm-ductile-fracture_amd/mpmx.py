@@ -30,7 +30,7 @@ EXPORTS = [
     "mpmx_max_velocity_sq", "mpmx_create_batch", "mpmx_set_particles_batch", "mpmx_num_scenes",
     "mpmx_get_scene_stats", "mpmx_scene_batch", "mpmx_read_obj_points", "mpmx_sample_sdf",
     "mpmx_write_particles", "mpmx_save_checkpoint", "mpmx_load_checkpoint", "mpmx_get_tuning",
-    "mpmx_set_tuning", "mpmx_autotune", "mpmx_run_cfl",
+    "mpmx_set_tuning", "mpmx_autotune", "mpmx_run_cfl", "mpmx_compute_dt_noalloc",
 ]
 
 
@@ -133,6 +133,7 @@ def lib():
         "mpmx_run": ([vp, C.c_int, C.c_double], C.c_int),
         "mpmx_compute_dt": ([vp, C.c_double, C.c_double, C.POINTER(C.c_double)], C.c_int),
         "mpmx_run_cfl": ([vp, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)], C.c_int),
+        "mpmx_compute_dt_noalloc": ([vp, C.c_double, C.c_double, C.POINTER(C.c_double)], C.c_int),
         "mpmx_sync": ([vp], C.c_int),
         "mpmx_get_field": ([vp, C.c_int, f64p, C.c_size_t], C.c_int),
         "mpmx_set_field": ([vp, C.c_int, f64p, C.c_size_t], C.c_int),

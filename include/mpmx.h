@@ -248,6 +248,14 @@ mpmx_status mpmx_compute_dt(mpmx_ctx* ctx, double dt_default, double cfl, double
  * kernels read that dt on the device.  *time_out (may be NULL) receives the
  * simulated time of the nsteps substeps (one synchronisation at the end). */
 mpmx_status mpmx_run_cfl(mpmx_ctx* ctx, int nsteps, double dt_default, double cfl, int wave, double* time_out);
+/* mpmx_compute_dt that allocates nothing (no counterpart in the reference):
+ * for callers that cannot allocate device memory, such as code under HIP
+ * graph capture or a serving loop that forbids hipMalloc.  Same result, bit
+ * for bit.  Its scratch (4096 partial maxima and two FP64 results, 16 KiB)
+ * is one static device array per GPU, held until the process ends and shared
+ * by every call on that GPU: calls that overlap on one GPU race.  Contexts of
+ * more than 33,554,432 particles are refused with MPMX_ERR_CAPACITY. */
+mpmx_status mpmx_compute_dt_noalloc(mpmx_ctx* ctx, double dt_default, double cfl, double* dt_out);
 
 mpmx_status mpmx_sync(mpmx_ctx* ctx);
 
