@@ -130,14 +130,15 @@ __global__ void k_page_keys(DevParams dp, const uint64_t* __restrict__ pmask, co
                             const int* __restrict__ bpre, int64_t words, uint32_t* page_key, int64_t cap_p,
                             int64_t cap_tp, uint32_t* __restrict__ block_key, int64_t cap_blocks,
                             SceneCG* __restrict__ scg, DevCounts* dc) {
-    // one lane per bit of a word (a wave per word): a set bit's id is the
+    // a wave per 64 consecutive words: one lane per word finds the words that
+    // hold a page (a batch's page space is mostly empty), then the wave takes
+    // those words one at a time with one lane per bit: a set bit's id is the
     // word's prefix plus the set bits below it.  The 2x2x2-page blocks of the
     // gather transfer: the 8 pages of block b are the Morton codes 8b .. 8b+7,
     // one byte of the bitmaps, so a block is occupied when that byte of
     // pmask | nmask is non-zero, and blocks are numbered in ascending Morton
     // order by the scan of the per-word block counts.
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t w = t >> 6;
     const int b = (int)(t & 63);
     const int num_page = dc->num_page;
     if (t == 0) {
@@ -146,24 +147,38 @@ __global__ void k_page_keys(DevParams dp, const uint64_t* __restrict__ pmask, co
         if (num_page > cap_p || total > cap_tp) atomicOr(&dc->err, kErrCapacity);
     }
     if (t < dp.num_scenes) scene_counts_of(dp, (int)t, wpre, spre, scg, dc);
-    if (w >= words) return;
-    const uint64_t P = pmask[w], N = nmask[w];
+    const int64_t w0 = (t >> 6) * 64;   // the wave's first word
+    uint64_t Pl = 0ull, Nl = 0ull;
+    if (w0 + b < words) {
+        Pl = pmask[w0 + b];
+        Nl = nmask[w0 + b];
+    }
+    uint64_t busy = __ballot((Pl | Nl) != 0ull);
     const uint64_t below = b == 0 ? 0ull : (~0ull >> (64 - b));
-    if ((P >> b) & 1ull) {
-        const int id = wpre[w] + __popcll(P & below);
-        if (id < cap_p && id < cap_tp) page_key[id] = (uint32_t)(w * 64 + b);
-    }
-    const uint64_t S = N & ~P;
-    if ((S >> b) & 1ull) {
-        const int id = num_page + spre[w] + __popcll(S & below);
-        if (id < cap_tp) page_key[id] = (uint32_t)(w * 64 + b);
-    }
-    if (b < 8) {
-        const uint64_t m = P | N;
-        if ((m >> (8 * b)) & 0xffull) {
-            int id = bpre[w];
-            for (int k = 0; k < b; ++k) id += ((m >> (8 * k)) & 0xffull) != 0ull;
-            if (id < cap_blocks) block_key[id] = (uint32_t)(w * 8 + b);
+    while (busy) {
+        const int j = __ffsll((long long)busy) - 1;
+        busy &= busy - 1ull;
+        const int64_t w = w0 + j;
+        const uint64_t P = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(Pl >> 32), j, 64) << 32) |
+                           (uint64_t)(uint32_t)__shfl((int)(uint32_t)Pl, j, 64);
+        const uint64_t N = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(Nl >> 32), j, 64) << 32) |
+                           (uint64_t)(uint32_t)__shfl((int)(uint32_t)Nl, j, 64);
+        if ((P >> b) & 1ull) {
+            const int id = wpre[w] + __popcll(P & below);
+            if (id < cap_p && id < cap_tp) page_key[id] = (uint32_t)(w * 64 + b);
+        }
+        const uint64_t S = N & ~P;
+        if ((S >> b) & 1ull) {
+            const int id = num_page + spre[w] + __popcll(S & below);
+            if (id < cap_tp) page_key[id] = (uint32_t)(w * 64 + b);
+        }
+        if (b < 8) {
+            const uint64_t m = P | N;
+            if ((m >> (8 * b)) & 0xffull) {
+                int id = bpre[w];
+                for (int k = 0; k < b; ++k) id += ((m >> (8 * k)) & 0xffull) != 0ull;
+                if (id < cap_blocks) block_key[id] = (uint32_t)(w * 8 + b);
+            }
         }
     }
 }
@@ -292,7 +307,9 @@ hipError_t build_tables(mpmx_ctx* c) {
     ScanSet3 sc{{c->wcount, c->scount, c->bcount}, {c->wcount, c->scount, c->bcount},
                 {&c->dc->num_page, &c->dc->num_supp, &c->dc->num_blocks}};
     if ((e = scan_exclusive3(c, sc, 3, words)) != hipSuccess) return e;
-    k_page_keys<<<(int)((words * 64 + 255) / 256), 256, 0, c->stream>>>(
+    // one thread per word (at least one workgroup: its first threads also
+    // write the per-scene page ranges)
+    k_page_keys<<<(int)std::max<int64_t>(1, (words + 255) / 256), 256, 0, c->stream>>>(
         c->dp, c->pmask, c->nmask, c->wcount, c->scount, c->bcount, words, c->page_key, c->cap_p, c->cap_tp,
         c->block_key, c->tp_bound, c->scg, c->dc);
     k_page_heads<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask, c->nmask, c->wcount, c->scount,
