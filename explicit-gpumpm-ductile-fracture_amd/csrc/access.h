@@ -29,12 +29,25 @@ struct BlockTables {
     int64_t cap_p, cap_tp, cap_blocks;
     int slot_shift;            // scene of a page key = key >> slot_shift
     const int* order;          // block of each item of the XCD shares, or null (item = block)
+    const int* bpre;           // per bitmap word: blocks numbered before it (scan of the block counts)
 };
 
 inline BlockTables block_tables_of(const mpmx_ctx* c) {
     return BlockTables{c->pmask, c->nmask, c->wcount, c->scount, c->page2particle, c->keys, c->block_key,
                        c->cell_start, c->cap_p, c->cap_tp, c->tp_bound, c->dp.slot_shift,
-                       c->order_ok ? c->block_order : nullptr};
+                       c->order_ok ? c->block_order : nullptr, c->bcount};
+}
+
+// id of 2x2x2 block bk (Morton block key = page key >> 3), or -1 when none of
+// its 8 pages is a particle or neighbour page: blocks are numbered in Morton
+// order, a bitmap word holds 8 blocks (one byte each)
+__device__ __forceinline__ int block_rank(const BlockTables& t, uint32_t bk) {
+    const uint32_t w = bk >> 3, j = bk & 7u;
+    const uint64_t m = t.pmask[w] | t.nmask[w];
+    if (!((m >> (8 * j)) & 0xffull)) return -1;
+    int id = t.bpre[w];
+    for (uint32_t k = 0; k < j; ++k) id += ((m >> (8 * k)) & 0xffull) != 0ull;
+    return id;
 }
 
 // page key of page (px, py, pz) (non-negative) when that page lies in the

@@ -442,6 +442,9 @@ static void free_particles(mpmx_ctx* c) {
     dfree(c->grid); dfree(c->grid_p2g);
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q); dfree(c->cg_part); dfree(c->cg_amask);
     dfree(c->scan_tmp);
+    if (c->p2g_part) (void)hipFree(c->p2g_part);
+    c->p2g_part = nullptr;
+    c->cap_part = 0;
 }
 
 static mpmx_status alloc_grid(mpmx_ctx* c, int64_t cap_tp) {
@@ -502,6 +505,14 @@ static mpmx_status alloc_grid(mpmx_ctx* c, int64_t cap_tp) {
     HIPCHK(c, dalloc(&c->cg_amask, cap_tp));
     HIPCHK(c, fill_zero(c, c->grid, cap_tp * kPageFloats * 4));
     HIPCHK(c, fill_zero(c, c->cg_x, cap_tp * 64 * 4));
+    if (c->cfg.p2g_variant == 3) {
+        // owner-block P2G records (50 KiB per block): four times the blocks of
+        // the first table build, as the grid pages are sized from its pages
+        if (c->p2g_part) (void)hipFree(c->p2g_part);
+        c->p2g_part = nullptr;
+        c->cap_part = std::min<int64_t>(c->tp_bound, std::max<int64_t>(4 * (int64_t)c->hc->num_blocks, 1024));
+        HIPCHK(c, hipMalloc(&c->p2g_part, (size_t)c->cap_part * (size_t)kOwnRecBytes));
+    }
     return MPMX_OK;
 }
 
@@ -599,6 +610,7 @@ mpmx_status mpmx_create_batch(int num_scenes, const mpmx_params* ps, const mpmx_
     }
     // keys: 6 cell bits + the slot's page bits + the slot index
     if (num_scenes > 1 && 6 + 3 * slot_log2 + ceil_log2(num_scenes) > 32) return MPMX_ERR_ARG;
+    if (cfg && (cfg->p2g_variant < 0 || cfg->p2g_variant > 3)) return MPMX_ERR_ARG;
     if (num_scenes > 1 && cfg && cfg->p2g_variant == 1) return MPMX_ERR_ARG;
     if (mpmx_device_count() <= 0) return MPMX_ERR_NO_DEVICE;
     mpmx_ctx* c = new mpmx_ctx();

@@ -402,6 +402,310 @@ hipError_t launch_cg_reset(mpmx_ctx* c) {
 
 static CGVec cg_vec(mpmx_ctx* c) { return CGVec{c->cg_x, c->cg_r, c->cg_z, c->cg_s, c->cg_q, c->cg_amask}; }
 
+// ---- owner-block P2G (p2g_variant 3) ------------------------------------------
+// Every particle is read once, by the block that owns its base node.  A
+// workgroup takes one 2x2x2-page block B at a time: B's own particles (a
+// contiguous range of the sorted order) reach nodes 0..9 of B per axis, so
+// all 27 contributions of every particle land in a 10^3-node fixed-point
+// tile in LDS with every lane active (k_p2g_own).  Nodes 2..7 along every
+// axis (the core, 216 of B's 512) receive only B's particles and are final:
+// the same workgroup runs their grid update and PCG set-up.  The other 784
+// nodes of the tile, B's shell (a coordinate below 2) and its halo (a
+// coordinate of 8 or 9, the shell of the blocks above), go to B's record in
+// HBM with B's channel exponents; a second pass (k_p2g_shell) sums each
+// shell node from the records of the up to 8 blocks that reach it, in a
+// fixed order, in FP64, then updates it.  Each block's exponents come from
+// the bounds of its own particles (the per-page bounds of its 8 pages), so a
+// term keeps 30 bits relative to the largest contribution of its own block;
+// the integer sums are exact and the FP64 combination has a fixed order, so
+// the result is bit-reproducible.  It is slower than the gather (variant 2)
+// on every measured scene: a record is 784 x 8 x 8 bytes, three times the
+// block's 512 x 8 x 4 bytes of grid output, written and read back, and the
+// 64 KiB tile leaves room for two workgroups per CU.
+constexpr int kOwnE = 10;                   // tile edge (nodes)
+constexpr int kOwn3 = kOwnE * kOwnE * kOwnE;
+constexpr int kOwnSlots = kOwn3 - 216;       // nodes outside the core [2,8)^3
+// record of one block: a 64-byte header (flag, exponents) and the 784 slots
+// of each channel as 64-bit integers
+constexpr int64_t kOwnRecU64 = 8 + (int64_t)kP2GCh * kOwnSlots;
+struct OwnHeader {
+    int flag;            // 1 when the block owns particles (its record holds sums)
+    int e[kP2GCh];       // channel exponents of its terms
+    int pad[7];
+};
+
+__host__ __device__ constexpr bool own_core(int X, int Y, int Z) {
+    return X >= 2 && X < 8 && Y >= 2 && Y < 8 && Z >= 2 && Z < 8;
+}
+// slot of non-core tile node (X, Y, Z): its rank among the non-core nodes in
+// X-major, then Y, then Z order (planes X < 2 or >= 8 hold 100 slots, the
+// others 64; rows Y < 2 or >= 8 of those hold 10, the others 4: Z 0, 1, 8, 9)
+__device__ __forceinline__ int own_slot(int X, int Y, int Z) {
+    const int xi = X < 2 ? X : (X < 8 ? X - 2 : X - 8);
+    int s = X < 2 ? 100 * X : (X < 8 ? 200 + 64 * xi : 584 + 100 * xi);
+    if (X < 2 || X >= 8) return s + 10 * Y + Z;
+    s += Y < 2 ? 10 * Y : (Y < 8 ? 20 + 4 * (Y - 2) : 44 + 10 * (Y - 8));
+    if (Y < 2 || Y >= 8) return s + Z;
+    return s + (Z < 2 ? Z : Z - 6);
+}
+
+// the particle's 27 contributions, all inside the tile (terms scaled by
+// scale_terms; the affine term accumulated as in p2g_fix)
+template <int TRANSFER>
+__device__ __forceinline__ void p2g_fix_own(const P2GTerms& T, float sc5, unsigned long long* acc) {
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) {
+        const int la = T.l0[0] + ii;
+        float ax[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) ax[r] = ii == 0 ? T.ar[r] : fmaf(T.Qd[r], (float)ii, T.ar[r]);
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) {
+            const int lb = T.l0[1] + jj;
+            const float wxy = T.w[0][ii] * T.w[1][jj];
+            float axy[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) axy[r] = jj == 0 ? ax[r] : fmaf(T.Qd[r + 3], (float)jj, ax[r]);
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) {
+                const int lc = T.l0[2] + kk;
+                const float wt = wxy * T.w[2][kk];
+                float pr[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) pr[r] = wt * (kk == 0 ? axy[r] : fmaf(T.Qd[r + 6], (float)kk, axy[r]));
+                if (TRANSFER == 1) {
+                    const float g0 = T.gw[0][ii] * T.w[1][jj] * T.w[2][kk];
+                    const float g1 = T.w[0][ii] * T.gw[1][jj] * T.w[2][kk];
+                    const float g2 = T.w[0][ii] * T.w[1][jj] * T.gw[2][kk];
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) pr[r] -= T.sg[r] * g0 + T.sg[r + 3] * g1 + T.sg[r + 6] * g2;
+                }
+                unsigned long long* a = acc + (la * kOwnE + lb) * kOwnE + lc;
+                const float add[kP2GCh] = {wt * T.m, pr[0], pr[1], pr[2], wt * T.cphase, wt * sc5, wt * T.V, wt * T.VF};
+#pragma unroll
+                for (int k = 0; k < kP2GCh; ++k)
+                    atomicAdd(a + k * kOwn3, k == 5 ? to_fix_scaled_nonneg(add[k]) : to_fix_scaled(add[k]));
+            }
+        }
+    }
+}
+
+// the grid update of own node q of a block (q = page o of the block, cell),
+// raw sums ch; writes the grid page, the raw copy and the PCG start values,
+// returns whether the node is active (sum w > 0)
+__device__ __forceinline__ bool own_node_out(const DevParams& sp, const int (&org)[3], int o, int cell, int G,
+                                             float (&ch)[kP2GCh], float* __restrict__ grid,
+                                             float* __restrict__ grid_raw, const CGVec& cg, float dt) {
+    const int lx = ((o >> 2) & 1) * 4 + (cell >> 4), ly = ((o >> 1) & 1) * 4 + ((cell >> 2) & 3),
+              lz = (o & 1) * 4 + (cell & 3);
+    if (grid_raw) {
+        float* rp = grid_raw + (int64_t)G * kPageFloats + cell;
+#pragma unroll
+        for (int k = 0; k < kP2GCh; ++k) rp[k * 64] = ch[k];
+    }
+    float r, z;
+    node_update(sp, org[0] + lx - sp.origin[0], org[1] + ly - sp.origin[1], org[2] + lz - sp.origin[2], ch, dt, r, z);
+    float* gp = grid + (int64_t)G * kPageFloats + cell;
+#pragma unroll
+    for (int k = 0; k < kP2GCh; ++k) gp[k * 64] = ch[k];
+    if (sp.phase_field) {
+        const int64_t nn = (int64_t)G * 64 + cell;
+        cg.x[nn] = 0.f;
+        cg.r[nn] = r;
+        cg.z[nn] = z;
+        cg.s[nn] = z;
+    }
+    return ch[5] > 0.f;
+}
+
+template <int TRANSFER>
+__global__ __launch_bounds__(256, 2) void k_p2g_own(DevParams dp, const float* __restrict__ pay,
+                                                 const unsigned* __restrict__ pbound, BlockTables t,
+                                                 const BlockClasses* __restrict__ lists,
+                                                 const BlockG2P* __restrict__ info, float* __restrict__ grid,
+                                                 float* __restrict__ grid_raw, CGVec cg,
+                                                 unsigned long long* __restrict__ part, int64_t cap_part, float dt,
+                                                 DevCounts* dc) {
+    __shared__ unsigned long long acc[kP2GCh * kOwn3];
+    __shared__ int sexp[kP2GCh];
+    dt = substep_dt(dt, dc);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int nb = dc->num_blocks;
+    nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
+    FOR_LOGICAL_ITEMS(item, nb) {
+        const int blk = t.order ? t.order[item] : item;
+        if (blk >= cap_part) {
+            if (tid == 0) atomicOr(&dc->err, kErrCapacity);
+            continue;
+        }
+        const uint32_t bkey = t.block_key[blk];
+        int bc[3];
+        unmorton(bkey, bc[0], bc[1], bc[2]);
+        const DevParams sp = dp.scenes[__builtin_amdgcn_readfirstlane(scene_of_page(dp, bkey << 3))];
+        const BlockClasses* L = lists + blk;
+        const BlockG2P* B = info + blk;
+        const int pa = __builtin_amdgcn_readfirstlane(B->pa), pb = __builtin_amdgcn_readfirstlane(B->pb);
+        const bool has = pa < pb;
+        unsigned long long* rec = part + (int64_t)blk * kOwnRecU64;
+        // exponents from the bounds of the block's own pages (source slots 0..7)
+        if (wave == 0) {
+            const int pg = lane < 8 ? L->page[lane] : -1;
+            float bmax[kP2GCh];
+#pragma unroll
+            for (int k = 0; k < kP2GCh; ++k)
+                bmax[k] = wave_max(pg >= 0 ? __uint_as_float(pbound[(int64_t)pg * kP2GCh + k]) : 0.f);
+            if (lane < kP2GCh) {
+                float v = bmax[0];
+#pragma unroll
+                for (int k = 1; k < kP2GCh; ++k) v = (lane == k) ? bmax[k] : v;
+                sexp[lane] = fix_exp(v);
+            }
+        }
+        if (has)
+            for (int q = tid; q < kP2GCh * kOwn3; q += 256) acc[q] = 0ull;
+        __syncthreads();
+        int e[kP2GCh];
+        float sc[kP2GCh];
+#pragma unroll
+        for (int k = 0; k < kP2GCh; ++k) {
+            e[k] = sexp[k];
+            sc[k] = ldexpf(1.0f, e[k]);
+        }
+        if (tid < 16) {
+            int* h = reinterpret_cast<int*>(rec);
+            h[tid] = tid == 0 ? (has ? 1 : 0) : (tid <= kP2GCh ? e[tid - 1] : 0);
+        }
+        const int org[3] = {8 * bc[0], 8 * bc[1], 8 * bc[2]};
+        if (has) {
+            // a wave takes 64 consecutive particles, lane l the one at
+            // (l mod 8) * 8 + l / 8, so that the up to 8 particles of a cell,
+            // adjacent in the sorted order, sit in different atomics
+            const int off = 64 * wave + ((lane & 7) << 3) + (lane >> 3);
+            for (int c0 = pa; c0 < pb; c0 += 256) {
+                const int i = c0 + off;
+                if (i < pb) {
+                    P2GTerms T;
+                    payload_load<TRANSFER>(pay, i, t.keys[i], org, T);
+                    // an own particle's base node lies in the block; stale
+                    // tables after a capacity overflow (latched) must not
+                    // index outside the tile
+                    const bool inside = (unsigned)T.l0[0] < 8u && (unsigned)T.l0[1] < 8u && (unsigned)T.l0[2] < 8u;
+                    if (inside) {
+                        scale_terms<TRANSFER>(T, sc);
+                        p2g_fix_own<TRANSFER>(T, sc[5], acc);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // the core nodes are final: grid update, written page-major (a wave
+        // holds the 64 nodes of one page)
+        for (int q = tid; q < 512; q += 256) {
+            const int o = q >> 6, cell = q & 63;
+            const int G = L->out_page[o];
+            const int lx = ((o >> 2) & 1) * 4 + (cell >> 4), ly = ((o >> 1) & 1) * 4 + ((cell >> 2) & 3),
+                      lz = (o & 1) * 4 + (cell & 3);
+            bool act = false;
+            if (G >= 0 && own_core(lx, ly, lz)) {
+                const int node = (lx * kOwnE + ly) * kOwnE + lz;
+                float ch[kP2GCh];
+#pragma unroll
+                for (int k = 0; k < kP2GCh; ++k) ch[k] = has ? from_fix(acc[k * kOwn3 + node], e[k]) : 0.f;
+                act = own_node_out(sp, org, o, cell, G, ch, grid, grid_raw, cg, dt);
+            }
+            const uint64_t bits = __ballot(act);
+            if (sp.phase_field && cell == 0 && G >= 0) cg.amask[G] = bits;   // the shell pass adds its nodes
+        }
+        // the shell and halo nodes to the block's record
+        if (has) {
+            unsigned long long* data = rec + 8;
+            for (int q = tid; q < kOwn3; q += 256) {
+                const int X = q / 100, Y = (q / 10) % 10, Z = q % 10;
+                if (own_core(X, Y, Z)) continue;
+                const int s = own_slot(X, Y, Z);
+#pragma unroll
+                for (int k = 0; k < kP2GCh; ++k) data[(int64_t)k * kOwnSlots + s] = acc[k * kOwn3 + q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// the shell nodes of every block: the sums of the records of the (up to 8)
+// blocks B - d, d in {0,1}^3, that reach each node, in the order of d, in
+// FP64 with each record's exponents, then the grid update
+__global__ __launch_bounds__(256) void k_p2g_shell(DevParams dp, BlockTables t, const BlockClasses* __restrict__ lists,
+                                                   float* __restrict__ grid, float* __restrict__ grid_raw, CGVec cg,
+                                                   const unsigned long long* __restrict__ part, int64_t cap_part,
+                                                   float dt, const DevCounts* __restrict__ dc) {
+    __shared__ int src[8];
+    __shared__ int sexp[8][kP2GCh];
+    dt = substep_dt(dt, dc);
+    const int tid = threadIdx.x;
+    int nb = dc->num_blocks;
+    nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
+    FOR_LOGICAL_ITEMS(blk, nb) {
+        if (blk >= cap_part) continue;   // k_p2g_own has latched the capacity error
+        const uint32_t bkey = t.block_key[blk];
+        int bc[3];
+        unmorton(bkey, bc[0], bc[1], bc[2]);
+        const int scene = __builtin_amdgcn_readfirstlane(scene_of_page(dp, bkey << 3));
+        const DevParams sp = dp.scenes[scene];
+        const BlockClasses* L = lists + blk;
+        if (tid < 8) {
+            const int dx = (tid >> 2) & 1, dy = (tid >> 1) & 1, dz = tid & 1;
+            const int sx = bc[0] - dx, sy = bc[1] - dy, sz = bc[2] - dz;
+            int id = -1;
+            if (sx >= 0 && sy >= 0 && sz >= 0) {
+                const uint32_t sk = morton((uint32_t)sx, (uint32_t)sy, (uint32_t)sz);
+                if (scene_of_page(dp, sk << 3) == scene) {
+                    id = block_rank(t, sk);
+                    if (id >= cap_part) id = -1;
+                }
+            }
+            if (id >= 0) {
+                const OwnHeader* h = reinterpret_cast<const OwnHeader*>(part + (int64_t)id * kOwnRecU64);
+                if (!h->flag) id = -1;
+                else
+                    for (int k = 0; k < kP2GCh; ++k) sexp[tid][k] = h->e[k];
+            }
+            src[tid] = id;
+        }
+        __syncthreads();
+        const int org[3] = {8 * bc[0], 8 * bc[1], 8 * bc[2]};
+        for (int q = tid; q < 512; q += 256) {
+            const int o = q >> 6, cell = q & 63;
+            const int G = L->out_page[o];
+            const int lx = ((o >> 2) & 1) * 4 + (cell >> 4), ly = ((o >> 1) & 1) * 4 + ((cell >> 2) & 3),
+                      lz = (o & 1) * 4 + (cell & 3);
+            bool act = false;
+            const bool shell = G >= 0 && !own_core(lx, ly, lz);
+            if (shell) {
+                double s[kP2GCh];
+#pragma unroll
+                for (int k = 0; k < kP2GCh; ++k) s[k] = 0.0;
+                for (int d = 0; d < 8; ++d) {
+                    const int dx = (d >> 2) & 1, dy = (d >> 1) & 1, dz = d & 1;
+                    const int id = src[d];
+                    if (id < 0 || (dx && lx >= 2) || (dy && ly >= 2) || (dz && lz >= 2)) continue;
+                    const unsigned long long* data = part + (int64_t)id * kOwnRecU64 + 8;
+                    const int sl = own_slot(lx + 8 * dx, ly + 8 * dy, lz + 8 * dz);
+#pragma unroll
+                    for (int k = 0; k < kP2GCh; ++k)
+                        s[k] += ldexp((double)(long long)data[(int64_t)k * kOwnSlots + sl], -sexp[d][k]);
+                }
+                float ch[kP2GCh];
+#pragma unroll
+                for (int k = 0; k < kP2GCh; ++k) ch[k] = (float)s[k];
+                act = own_node_out(sp, org, o, cell, G, ch, grid, grid_raw, cg, dt);
+            }
+            const uint64_t bits = __ballot(act);
+            if (sp.phase_field && cell == 0 && G >= 0) cg.amask[G] |= bits;
+        }
+        __syncthreads();
+    }
+}
+
 // block P2G (with grid update and PCG set-up) over the payload and page bounds
 template <int TRANSFER, typename ST>
 static hipError_t gather_blocks_t(mpmx_ctx* c, float dt) {
@@ -409,6 +713,20 @@ static hipError_t gather_blocks_t(mpmx_ctx* c, float dt) {
     float* raw = (c->keep_p2g && c->grid_p2g) ? c->grid_p2g : nullptr;
     // about one workgroup per block: the hardware balances their uneven work
     const int g = block_grid(c, c->p2g_pct);
+    if (c->cfg.p2g_variant == 3) {
+        // owner blocks, then the shell nodes (both timed as the P2G stage)
+        unsigned long long* part = static_cast<unsigned long long*>(c->p2g_part);
+        timer_push(c, "p2g_block", true);
+        k_p2g_own<TRANSFER><<<g, 256, 0, c->stream>>>(c->dp, c->payload, c->pbound, t,
+                                                       static_cast<BlockClasses*>(c->block_lists),
+                                                       static_cast<BlockG2P*>(c->block_g2p), c->grid, raw,
+                                                       cg_vec(c), part, c->cap_part, dt, c->dc);
+        k_p2g_shell<<<block_grid(c, 100), 256, 0, c->stream>>>(c->dp, t, static_cast<BlockClasses*>(c->block_lists),
+                                                               c->grid, raw, cg_vec(c), part, c->cap_part, dt,
+                                                               c->dc);
+        timer_push(c, "p2g_block", false);
+        return hipGetLastError();
+    }
     MPMX_TIMED(c, "p2g_block", (k_p2g_block<TRANSFER, ST><<<g, 256, 0, c->stream>>>(
                                     c->dp, c->payload, c->pbound, t, static_cast<BlockClasses*>(c->block_lists),
                                     c->grid, raw, cg_vec(c), dt, (int)c->n, c->dc)));

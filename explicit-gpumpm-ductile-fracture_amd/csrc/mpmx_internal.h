@@ -40,6 +40,9 @@ namespace mpmx {
 constexpr int kGridCh = 8;       // channels per page in the grid slab
 constexpr int kPageFloats = kGridCh * 64;
 constexpr int kTile = 6;         // 6^3 node tile of one page's particles
+// bytes of one block's record of the owner-block P2G (k_grid.hip): a 64-byte
+// header and 784 nodes x 8 channels of 64-bit sums
+constexpr int64_t kOwnRecBytes = 64 + 8ll * 8 * 784;
 constexpr int kTileN = kTile * kTile * kTile;
 
 enum ErrBits : unsigned {
@@ -227,6 +230,8 @@ struct mpmx_ctx {
     void* block_lists = nullptr;     // [tp_bound] per-block source lists of the block P2G (BlockClasses)
     void* block_g2p = nullptr;       // [tp_bound] per-block G2P pages and particle range (BlockG2P)
     int* block_order = nullptr;      // [tp_bound] blocks of each XCD share, heavy ones first
+    void* p2g_part = nullptr;        // [cap_part] per-block records of the owner-block P2G (p2g_variant 3)
+    int64_t cap_part = 0;
     bool vpages_valid = false;       // adjacency and virtual pages match the current tables
     // launch shapes (mpmx_tuning; mpmx_autotune picks them): none changes results
     int particle_wg = 256;           // threads per workgroup of the per-particle kernels

@@ -102,7 +102,11 @@ typedef struct mpmx_config {
     int     p2g_variant;    /* transfer kernels: 0 = default (2); 1 = page tiles over
                                virtual pages with LDS and global float atomics (P2G,
                                G2P, PCG K s); 2 = 2x2x2-page blocks, P2G and K s as
-                               single-wave gathers without atomics               */
+                               single-wave gathers without atomics; 3 = as 2, but the
+                               P2G scatters each block's own particles into a 10^3-node
+                               tile and adds the neighbours' halos in a second pass
+                               (same sums to rounding, slower on every measured scene);
+                               other values: MPMX_ERR_ARG                        */
     int     block;          /* threads per transfer block; 0 = auto (= vpage) */
     int     stress_fp64;    /* arithmetic of the SVD and return mapping: 1 = FP64 for every
                                particle (default; state stays in its storage precision),

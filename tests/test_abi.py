@@ -107,5 +107,7 @@ def test_batch_create_validates_arguments(mpmx):
     big = [mpmx.default_params(N=512) for _ in range(64)]   # 6 + 21 + 6 > 32
     assert create(big) == mpmx.ERR_ARG
     assert create([a, a], mpmx.default_config(p2g_variant=1)) == mpmx.ERR_ARG
+    for bad in (-1, 4):
+        assert create([a], mpmx.default_config(p2g_variant=bad)) == mpmx.ERR_ARG
     if mpmx.device_count() == 0:
         assert create([a, a]) == mpmx.ERR_NO_DEVICE

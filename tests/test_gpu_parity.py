@@ -113,10 +113,10 @@ def test_stress_parity(mpmx, oracle, model, stress_fp64, bar):
         assert e < bar, k
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 def test_p2g_parity(mpmx, oracle, variant):
-    """Raw P2G sums (before the grid update) of the scatter (1) and the gather
-    (2) transfer on the same plastic, damaged state."""
+    """Raw P2G sums (before the grid update) of the scatter (1), the gather
+    (2) and the owner-block (3) transfer on the same plastic, damaged state."""
     p, x, v, m, dt = mpmx.scene(3, seed=1, scale=0.1)
     ctx = mpmx.Context(p, mpmx.default_config(p2g_variant=variant))
     ctx.set_particles(x, v, m)
@@ -143,7 +143,8 @@ def test_p2g_parity(mpmx, oracle, variant):
 STATE = ("x", "v", "F", "B", "c", "alpha", "psi", "g")
 
 
-@pytest.mark.parametrize("transfer,variant,model", [(0, 2, 2), (1, 2, 2), (0, 1, 2), (1, 1, 2), (0, 2, 1), (1, 2, 0)])
+@pytest.mark.parametrize("transfer,variant,model", [(0, 2, 2), (1, 2, 2), (0, 1, 2), (1, 1, 2), (0, 2, 1), (1, 2, 0),
+                                                     (0, 3, 2), (1, 3, 2)])
 def test_one_step_from_gpu_state(mpmx, oracle, transfer, variant, model):
     """Shadow step: the GPU's own state 45 substeps into the impact (yielded,
     damaged particles) is loaded into the oracle and both advance one substep.
