@@ -429,7 +429,7 @@ static void free_particles(mpmx_ctx* c) {
         dfree(m.x); dfree(m.v); dfree(m.B); dfree(m.F); dfree(m.F32); dfree(m.c); dfree(m.alpha); dfree(m.psi); dfree(m.g);
         dfree(m.mass); dfree(m.id);
     }
-    dfree(c->sigma); dfree(c->FP); dfree(c->vol); dfree(c->cg_col); dfree(c->payload); dfree(c->pbound);
+    dfree(c->sigma); dfree(c->FP); dfree(c->vol); dfree(c->cg_col); dfree(c->payload); dfree(c->pbound); dfree(c->cbound);
     dfree(c->keys); dfree(c->keys_alt); dfree(c->vals); dfree(c->vals_alt); dfree(c->hist);
     dfree(c->pmask); dfree(c->nmask); dfree(c->pflag); dfree(c->wcount); dfree(c->scount);
     dfree(c->page_key); dfree(c->page2particle); dfree(c->adj); dfree(c->nvp); dfree(c->vpo); dfree(c->target); dfree(c->block_key); dfree(c->bcount); dfree(c->cell_start);
@@ -818,6 +818,7 @@ static mpmx_status upload_particles(mpmx_ctx* c, const int64_t* counts, const st
     c->tp_bound = tp_bound;
     HIPCHK(c, dalloc(&c->block_key, tp_bound));
     HIPCHK(c, dalloc(&c->pbound, 8 * (c->cap_p + 1)));
+    HIPCHK(c, dalloc(&c->cbound, c->cap_p + 1));
     HIPCHK(c, dalloc(&c->bcount, c->mask_words));
     HIPCHK(c, dalloc(&c->page2particle, c->cap_p + 1));
     HIPCHK(c, dalloc(&c->cell_start, 65 * c->cap_p));

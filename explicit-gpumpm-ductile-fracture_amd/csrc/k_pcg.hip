@@ -130,15 +130,20 @@ __global__ __launch_bounds__(BLOCK) void k_cg_apply(DevParams dp, ST s, const fl
 
 // ---- block form of (K s) (variant 2) ----------------------------------------
 // col_p = M V dx^2 sum_j grad w_jp s_j for every particle of a 2x2x2 page
-// block, from the block's 10^3-node s region staged in LDS
+// block, from the block's 10^3-node s region staged in LDS; with it, the
+// largest |col_p|_1 of each of the block's pages (cbound, float bits), from
+// which k_cg_q takes its fixed-point scale
 template <typename ST>
 __global__ __launch_bounds__(256) void k_cg_col(DevParams dp, ST s, const float* __restrict__ vol, BlockTables t,
-                                                const float* __restrict__ cgs, float* __restrict__ col, int n,
+                                                const float* __restrict__ cgs, float* __restrict__ col,
+                                                unsigned* __restrict__ cbound, int n,
                                                 const SceneCG* __restrict__ scg, DevCounts* dc) {
     constexpr int R = 10, R3 = R * R * R;
     __shared__ float ts[R3];
     __shared__ int spg[27];
     __shared__ int prange[2];
+    __shared__ int opg[8];          // page ids of the block's own pages (-1: none)
+    __shared__ unsigned omax[8];    // their largest |col|_1 (float bits)
     if (cg_stopped(dc)) return;
     const int np_raw = dc->num_page;
     const int np = np_raw < t.cap_p ? np_raw : (int)t.cap_p;
@@ -168,6 +173,11 @@ __global__ __launch_bounds__(256) void k_cg_col(DevParams dp, ST s, const float*
             int id = -1;
             if (o < 8) id = page_rank((bkey << 3) | (uint32_t)o, t.pmask, t.nmask, t.wpre, t.spre, np_raw);
             const bool own = id >= 0 && id < np;
+            // opg and omax are written and read by these 8 threads only
+            if (o < 8) {
+                opg[o] = own ? id : -1;
+                omax[o] = 0u;
+            }
             int lo = own ? id : 0x7fffffff, hi = own ? id : -1;
 #pragma unroll
             for (int w = 32; w > 0; w >>= 1) {
@@ -222,76 +232,94 @@ __global__ __launch_bounds__(256) void k_cg_col(DevParams dp, ST s, const float*
                             gs[2] += w[0][ii] * w[1][jj] * gw[2][kk] * sv;
                         }
                 const float sc = vol[i] * para_scale * sp.M;
-                col[3 * (int64_t)i + 0] = gs[0] * sc;
-                col[3 * (int64_t)i + 1] = gs[1] * sc;
-                col[3 * (int64_t)i + 2] = gs[2] * sc;
+                const float c0 = gs[0] * sc, c1 = gs[1] * sc, c2 = gs[2] * sc;
+                col[3 * (int64_t)i + 0] = c0;
+                col[3 * (int64_t)i + 1] = c1;
+                col[3 * (int64_t)i + 2] = c2;
+                // the particle's page is child (key >> 6) & 7 of the block
+                atomicMax(&omax[(t.keys[i] >> 6) & 7u], __float_as_uint(fabsf(c0) + fabsf(c1) + fabsf(c2)));
             }
         }
         __syncthreads();
+        if (tid >= 64 && tid < 72 && opg[tid - 64] >= 0) cbound[opg[tid - 64]] = omax[tid - 64];
     }
 }
 
-// (K s) contributions of the calling thread's run of one class of the
-// block's sources (BlockClasses): grad w_ip . col_p into the block's nodes in
-// fixed point; out-of-block nodes are skipped, together by the wave when its
-// lanes hold one class
-template <typename ST>
-__device__ __forceinline__ void cgq_class(const BlockClasses& S, const DevParams& sp, const ST& s,
-                                          const float* __restrict__ col, const int (&org)[3], float sc,
-                                          unsigned long long* acc, int cls, int jb, int je) {
-    for (int j = jb; j < je; ++j) {
-        const int i = class_entry(S, cls, j);
-        float w[3][3], gw[3][3], c3[3];
-        int l0[3];
+// particle of entry e (0 <= e < S.pre[kMaxRuns]) of a block's source runs,
+// all classes in order: the last run that starts at or before e
+__device__ __forceinline__ int block_entry(const BlockClasses& S, int e) {
+    int lo = 0, hi = kMaxRuns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S.pre[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return S.start[lo] + (e - S.pre[lo]);
+}
+
+// (K s) contributions of one particle at xp with column cs (col times the
+// block's power-of-two scale, exact): grad w_ip . col_p into the block's
+// nodes in fixed point; nodes outside the block are skipped.  grad w . col is
+// w_z (gw_x w_y c0 + w_x gw_y c1) + gw_z (w_x w_y c2), whose two brackets are
+// shared by the three nodes of a row.
+__device__ __forceinline__ void cgq_entry(const DevParams& sp, const float (&xp)[3], const float (&cs)[3],
+                                          const int (&org)[3], unsigned long long* acc) {
+    float w[3][3], gw[3][3];
+    int l0[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const float xa = ld_x(s, i, a);
-            const int b = base_node(xa, sp.N);
-            const float d0 = xa * sp.inv_dx - (float)b;
-            l0[a] = b - org[a];
-            bspline_grad(d0, w[a], gw[a]);
-            c3[a] = col[3 * (int64_t)i + a];
-        }
-        // col times the block's power-of-two scale (exact); grad w . col is
-        // w_z (gw_x w_y c0 + w_x gw_y c1) + gw_z (w_x w_y c2), whose two
-        // brackets are shared by the three nodes of a row
-        const float cs[3] = {c3[0] * sc, c3[1] * sc, c3[2] * sc};
+    for (int a = 0; a < 3; ++a) {
+        const int b = base_node(xp[a], sp.N);
+        const float d0 = xp[a] * sp.inv_dx - (float)b;
+        l0[a] = b - org[a];
+        bspline_grad(d0, w[a], gw[a]);
+    }
 #pragma unroll
-        for (int ii = 0; ii < 3; ++ii) {
-            const int la = l0[0] + ii;
-            if (la < 0 || la >= 8) continue;
+    for (int ii = 0; ii < 3; ++ii) {
+        const int la = l0[0] + ii;
+        if (la < 0 || la >= 8) continue;
 #pragma unroll
-            for (int jj = 0; jj < 3; ++jj) {
-                const int lb = l0[1] + jj;
-                if (lb < 0 || lb >= 8) continue;
-                const float rA = fmaf(gw[0][ii] * w[1][jj], cs[0], w[0][ii] * gw[1][jj] * cs[1]);
-                const float rB = w[0][ii] * w[1][jj] * cs[2];
+        for (int jj = 0; jj < 3; ++jj) {
+            const int lb = l0[1] + jj;
+            if (lb < 0 || lb >= 8) continue;
+            const float rA = fmaf(gw[0][ii] * w[1][jj], cs[0], w[0][ii] * gw[1][jj] * cs[1]);
+            const float rB = w[0][ii] * w[1][jj] * cs[2];
 #pragma unroll
-                for (int kk = 0; kk < 3; ++kk) {
-                    const int lc = l0[2] + kk;
-                    if (lc < 0 || lc >= 8) continue;
-                    const float hw = fmaf(w[2][kk], rA, gw[2][kk] * rB);
-                    atomicAdd(&acc[acc_node(la, lb, lc)], to_fix_scaled(hw));
-                }
+            for (int kk = 0; kk < 3; ++kk) {
+                const int lc = l0[2] + kk;
+                if (lc < 0 || lc >= 8) continue;
+                const float hw = fmaf(w[2][kk], rA, gw[2][kk] * rB);
+                atomicAdd(&acc[acc_node(la, lb, lc)], to_fix_scaled(hw));
             }
         }
     }
 }
 
+// entries staged in LDS per round, and the staged words per component (one
+// pad word every 32, so that the runs of neighbouring lanes start in
+// different banks)
+constexpr int kQChunk = 1024;
+constexpr int kQSkew = kQChunk + kQChunk / 32;
+__device__ __forceinline__ int q_slot(int q) { return q + (q >> 5); }
+
 // q_i = sum_p grad w_ip . col_p + D_i s_i over the 8^3 nodes of a block, as a
-// gather with 64-bit fixed-point LDS atomics (|grad w . col| <= sum |col|
-// gives the scale); q is written whole (no zero fill, no global atomics).
-// The block's source lists are the ones the block P2G reads (built with the
-// page tables, same block numbering).
+// gather with 64-bit fixed-point LDS atomics; q is written whole (no zero
+// fill, no global atomics).  The block's source lists are the ones the block
+// P2G reads (built with the page tables, same block numbering).  The scale
+// comes from the largest |col|_1 of the 27 source pages (cbound, k_cg_col),
+// which bounds |grad w . col| for every listed particle.  The listed
+// particles are read in rounds of kQChunk: consecutive lanes load
+// consecutive entries (runs of adjacent particles, whole cache lines) into
+// LDS, then each thread takes a run of adjacent staged entries, so the lanes
+// of a wave hold particles of different cells when they add into the nodes.
 template <typename ST>
 __global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t,
                                               const BlockClasses* __restrict__ lists, const float* __restrict__ col,
-                                              const float* __restrict__ grid, const float* __restrict__ cgs,
-                                              float* __restrict__ cgq, int n, const SceneCG* __restrict__ scg,
-                                              DevCounts* dc) {
+                                              const unsigned* __restrict__ cbound, const float* __restrict__ grid,
+                                              const float* __restrict__ cgs, float* __restrict__ cgq, int n,
+                                              const SceneCG* __restrict__ scg, DevCounts* dc) {
     __shared__ unsigned long long acc[kAccStride];
     __shared__ BlockClasses S;
-    __shared__ float wmax[4];
+    __shared__ float stg[6 * kQSkew];   // x (3 components), then col (3)
     __shared__ int sexp;
     if (cg_stopped(dc)) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -308,28 +336,39 @@ __global__ __launch_bounds__(256) void k_cg_q(DevParams dp, ST s, BlockTables t,
         unmorton(bkey, bc[0], bc[1], bc[2]);
         classes_load(lists + blk, S);
         for (int q = tid; q < kAccStride; q += 256) acc[q] = 0ull;
-        __syncthreads();
-        // scale: the largest |col|_1 of the listed particles, over each thread's run
-        int cls, jb, je;
-        const bool mine = thread_class(S, cls, jb, je);
-        float bm = 0.f;
-        if (mine)
-            for (int j = jb; j < je; ++j) {
-                const int i = class_entry(S, cls, j);
-                bm = fmaxf(bm, fabsf(col[3 * (int64_t)i]) + fabsf(col[3 * (int64_t)i + 1]) +
-                                   fabsf(col[3 * (int64_t)i + 2]));
-            }
-        bm = wave_max(bm);
-        if (lane == 0) wmax[wave] = bm;
-        __syncthreads();
-        if (tid == 0) sexp = fix_exp(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3])));
+        if (wave == 0) {
+            const int pg = lane < 27 ? lists[blk].page[lane] : -1;
+            const float bm = wave_max(pg >= 0 ? __uint_as_float(cbound[pg]) : 0.f);
+            if (lane == 0) sexp = fix_exp(bm);
+        }
         __syncthreads();
         const int e = sexp;
         const float sc = ldexpf(1.0f, e);
         // the block's first node in scene-local cells
         const int org[3] = {8 * bc[0] - sp.origin[0], 8 * bc[1] - sp.origin[1], 8 * bc[2] - sp.origin[2]};
-        if (mine) cgq_class(S, sp, s, col, org, sc, acc, cls, jb, je);
-        __syncthreads();
+        const int total = S.pre[kMaxRuns];
+        for (int c0 = 0; c0 < total; c0 += kQChunk) {
+            const int cnt = total - c0 < kQChunk ? total - c0 : kQChunk;
+            for (int q = tid; q < cnt; q += 256) {
+                const int i = block_entry(S, c0 + q);
+                const int k = q_slot(q);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    stg[a * kQSkew + k] = ld_x(s, i, a);
+                    stg[(3 + a) * kQSkew + k] = col[3 * (int64_t)i + a];
+                }
+            }
+            __syncthreads();
+            const int per = (cnt + 255) >> 8;
+            const int qb = tid * per, qe = qb + per < cnt ? qb + per : cnt;
+            for (int q = qb; q < qe; ++q) {
+                const int k = q_slot(q);
+                const float xp[3] = {stg[k], stg[kQSkew + k], stg[2 * kQSkew + k]};
+                const float cs[3] = {stg[3 * kQSkew + k] * sc, stg[4 * kQSkew + k] * sc, stg[5 * kQSkew + k] * sc};
+                cgq_entry(sp, xp, cs, org, acc);
+            }
+            __syncthreads();
+        }
         for (int q = tid; q < 512; q += 256) {
             const int o = q >> 6, cell = q & 63;
             const int G = S.out_page[o];
@@ -530,11 +569,11 @@ template <typename ST>
 static void apply_blocks(mpmx_ctx* c, ST st) {
     const BlockTables t = block_tables_of(c);
     const int g1 = block_grid(c, c->cg_pct), g2 = g1;
-    MPMX_TIMED(c, "cg_col", (k_cg_col<ST><<<(int)g1, 256, 0, c->stream>>>(c->dp, st, c->vol, t, c->cg_s,
-                                                                         c->cg_col, (int)c->n, c->scg, c->dc)));
+    MPMX_TIMED(c, "cg_col", (k_cg_col<ST><<<(int)g1, 256, 0, c->stream>>>(
+                                c->dp, st, c->vol, t, c->cg_s, c->cg_col, c->cbound, (int)c->n, c->scg, c->dc)));
     MPMX_TIMED(c, "cg_q", (k_cg_q<ST><<<(int)g2, 256, 0, c->stream>>>(
-                              c->dp, st, t, static_cast<const BlockClasses*>(c->block_lists), c->cg_col, c->grid,
-                              c->cg_s, c->cg_q, (int)c->n, c->scg, c->dc)));
+                              c->dp, st, t, static_cast<const BlockClasses*>(c->block_lists), c->cg_col, c->cbound,
+                              c->grid, c->cg_s, c->cg_q, (int)c->n, c->scg, c->dc)));
 }
 
 static hipError_t pcg_iteration(mpmx_ctx* c, int k) {

@@ -262,6 +262,7 @@ struct mpmx_ctx {
     float* cg_col = nullptr;     // [3n] per-particle M V dx^2 grad s (block K s)
     float* payload = nullptr;    // [28n] per-particle P2G terms (block P2G)
     unsigned* pbound = nullptr;  // [cap_p][8] per-page largest contribution per channel
+    unsigned* cbound = nullptr;  // [cap_p] per-page largest |col|_1 of the K s column (float bits)
     int cg_nb = 0;               // workgroups of every node-space kernel
     int cg_spec = 1;             // PCG iterations queued before the first host check
     bool grid_fused = false;     // last P2G also ran the grid update (gather variant)
