@@ -273,49 +273,82 @@ __global__ __launch_bounds__(kRsThreads) void k_radix_scatter(
 }
 
 // ---------------------------------------------------------------- gather
+// Every field of particle perm[i] is loaded before any is stored: the source
+// and destination state are separate buffers, but the compiler cannot know
+// that of plain pointers, and with stores between them it would keep one
+// load in flight per thread.
 __global__ void k_gather_fp32(const uint32_t* __restrict__ perm, PState s, PState d, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int o = (int)perm[i];
+    float x[3], v[3], B[9], F[9];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        d.x[3 * i + a] = s.x[3 * o + a];
-        d.v[3 * i + a] = s.v[3 * o + a];
+        x[a] = s.x[3 * o + a];
+        v[a] = s.v[3 * o + a];
     }
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
-        d.B[(int64_t)a * n + i] = s.B[(int64_t)a * n + o];
-        d.F[(int64_t)a * n + i] = s.F[(int64_t)a * n + o];
+        B[a] = s.B[(int64_t)a * n + o];
+        F[a] = s.F[(int64_t)a * n + o];
     }
-    d.c[i] = s.c[o];
-    d.alpha[i] = s.alpha[o];
-    d.psi[i] = s.psi[o];
-    d.g[i] = s.g[o];
-    d.mass[i] = s.mass[o];
-    d.id[i] = s.id[o];
+    const float c = s.c[o], alpha = s.alpha[o], psi = s.psi[o], g = s.g[o], mass = s.mass[o];
+    const int id = s.id[o];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        d.x[3 * i + a] = x[a];
+        d.v[3 * i + a] = v[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+        d.B[(int64_t)a * n + i] = B[a];
+        d.F[(int64_t)a * n + i] = F[a];
+    }
+    d.c[i] = c;
+    d.alpha[i] = alpha;
+    d.psi[i] = psi;
+    d.g[i] = g;
+    d.mass[i] = mass;
+    d.id[i] = id;
 }
 
 __global__ void k_gather_mixed(const uint32_t* __restrict__ perm, MState s, MState d, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int o = (int)perm[i];
+    float x[3], F32[9];
+    uint16_t v[3], B[9], F[9];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        d.x[3 * i + a] = s.x[3 * o + a];
-        d.v[3 * i + a] = s.v[3 * o + a];
+        x[a] = s.x[3 * o + a];
+        v[a] = s.v[3 * o + a];
     }
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
-        d.B[(int64_t)a * n + i] = s.B[(int64_t)a * n + o];
-        if (s.F32) d.F32[(int64_t)a * n + i] = s.F32[(int64_t)a * n + o];
-        else d.F[(int64_t)a * n + i] = s.F[(int64_t)a * n + o];
+        B[a] = s.B[(int64_t)a * n + o];
+        if (s.F32) F32[a] = s.F32[(int64_t)a * n + o];
+        else F[a] = s.F[(int64_t)a * n + o];
     }
-    d.c[i] = s.c[o];
-    d.alpha[i] = s.alpha[o];
-    d.psi[i] = s.psi[o];
-    d.g[i] = s.g[o];
-    d.mass[i] = s.mass[o];
-    d.id[i] = s.id[o];
+    const uint16_t c = s.c[o], alpha = s.alpha[o], psi = s.psi[o], g = s.g[o];
+    const float mass = s.mass[o];
+    const int id = s.id[o];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        d.x[3 * i + a] = x[a];
+        d.v[3 * i + a] = v[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+        d.B[(int64_t)a * n + i] = B[a];
+        if (s.F32) d.F32[(int64_t)a * n + i] = F32[a];
+        else d.F[(int64_t)a * n + i] = F[a];
+    }
+    d.c[i] = c;
+    d.alpha[i] = alpha;
+    d.psi[i] = psi;
+    d.g[i] = g;
+    d.mass[i] = mass;
+    d.id[i] = id;
 }
 
 hipError_t sort_and_gather(mpmx_ctx* c) {

@@ -337,17 +337,22 @@ __device__ __forceinline__ void stress_store(const MState& s, int i, int n, cons
     s.psi[i] = f2bf(out.psi);
 }
 
-// stress of particle i: reads its F and phase-field state, writes back F,
-// g, alpha, psi and stores V and, where sigma is not null, sigma (component
-// r + 3 c) and FP (the fused stress pass carries both in the P2G payload and
-// stores them only when the caller keeps them)
+// stress of particle i from its F and phase-field state (stress_compute);
+// then (stress_write) F, g, alpha, psi written back, V stored and, where
+// sigma is not null, sigma (component r + 3 c) and FP (the fused stress pass
+// carries both in the P2G payload and stores them only when the caller keeps
+// them)
 template <bool ALL64, typename ST>
-__device__ __forceinline__ void stress_particle(const DevParams& sp, const ST& s, int i, int n,
-                                                float* __restrict__ sigma, float* __restrict__ FP,
-                                                float* __restrict__ vol, DevCounts* dc, StressOut& out) {
+__device__ __forceinline__ void stress_compute(const DevParams& sp, const ST& s, int i, int n, DevCounts* dc,
+                                               StressOut& out) {
     StressIn in;
     stress_load(s, i, n, in);
     count_fallbacks(stress_one<ALL64>(sp, in, out), &dc->svd_fallbacks);
+}
+template <typename ST>
+__device__ __forceinline__ void stress_write(const ST& s, int i, int n, float* __restrict__ sigma,
+                                             float* __restrict__ FP, float* __restrict__ vol, DevCounts* dc,
+                                             const StressOut& out) {
     bool finite = true;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -371,7 +376,8 @@ __global__ __launch_bounds__(256) void k_stress(DevParams dp, ST s, WaveMap wm, 
     int i, scene;
     if (!wave_particle(wm, n, i, scene)) return;
     StressOut out;
-    stress_particle<ALL64>(dp.scenes[scene], s, i, n, sigma, FP, vol, dc, out);
+    stress_compute<ALL64>(dp.scenes[scene], s, i, n, dc, out);
+    stress_write(s, i, n, sigma, FP, vol, dc, out);
 }
 
 // the stress pass of a substep with the gather P2G: after the stress, the
@@ -406,14 +412,18 @@ __global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, Wave
     if (valid) {
         const DevParams& sp = dp.scenes[scene];
         StressOut out;
-        stress_particle<ALL64>(sp, s, i, n, sigma, FP, vol, dc, out);
+        stress_compute<ALL64>(sp, s, i, n, dc, out);
         pk = t.keys[i] >> 6;
         float sig[9];
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) sig[r + 3 * cc] = out.sigma[r][cc];
+        // the payload reads x, m, c, v and B, none of which the stress
+        // changes: it goes first, so that its loads need not wait behind the
+        // stores of the updated state
         prep_particle<TRANSFER>(sp, s, sig, out.FP, out.vol, i, n, dt, pay, b);
+        stress_write(s, i, n, sigma, FP, vol, dc, out);
     }
     page_bounds(valid, pk, b, t, pbound, dc);
 }
