@@ -52,6 +52,8 @@ __device__ __forceinline__ void von_mises(const DevParams& dp, const StressIn& i
     T eps[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) eps[a] = (S[a] > T(0)) ? log(S[a]) : T(0);
+    // ln S of the final singular values: eps, or eps_nn after a return
+    T lnS[3] = {eps[0], eps[1], eps[2]};
     T tau_tr[3];
     tau_tr[0] = c0 * eps[0] + lambda * eps[1] + lambda * eps[2];
     tau_tr[1] = lambda * eps[0] + c0 * eps[1] + lambda * eps[2];
@@ -65,7 +67,7 @@ __device__ __forceinline__ void von_mises(const DevParams& dp, const StressIn& i
 
     const T p23 = T(2.0f / 3.0f);
     const T sq23 = sqrt(p23);
-    const T Rh = T(dp.sq23f) * T(dp.yield) - (T(dp.sq23f) * T(dp.yield) - T(dp.yield)) / exp(T(dp.hard) * alpha);
+    const T Rh = T(dp.sq23f) * T(dp.yield) - (T(dp.sq23f) * T(dp.yield) - T(dp.yield)) * exp(-T(dp.hard) * alpha);
     const T y = tau_tr_F - g * Rh * sq23;
     if (y >= T(1e-12)) {
         T dlambda = T(0);
@@ -96,7 +98,10 @@ __device__ __forceinline__ void von_mises(const DevParams& dp, const StressIn& i
         for (int a = 0; a < 3; ++a)
             if (eps_nn[a] > T(dp.eps_cap)) eps_nn[a] = eps_nn[a] * (T(dp.eps_cap) / normS);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) S[a] = exp(eps_nn[a]);
+        for (int a = 0; a < 3; ++a) {
+            S[a] = exp(eps_nn[a]);
+            lnS[a] = eps_nn[a];
+        }
         usv(U, S, V, Fm);
     }
 #pragma unroll
@@ -105,12 +110,16 @@ __device__ __forceinline__ void von_mises(const DevParams& dp, const StressIn& i
         for (int cc = 0; cc < 3; ++cc) out.F[r][cc] = (float)Fm[r][cc];
 
     const T J = S[0] * S[1] * S[2];
-    const T J_bar = pow(J, -p23);
+    // J^(-2/3) and ln J from the logarithms already at hand where every
+    // singular value is positive (one exp instead of pow and log)
+    const bool pos = S[0] > T(0) && S[1] > T(0) && S[2] > T(0);
+    const T lnJ = lnS[0] + lnS[1] + lnS[2];
+    const T J_bar = pos ? exp(-p23 * lnJ) : pow(J, -p23);
     T be[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) be[a] = J_bar * S[a] * S[a];
     const T be_sum = be[0] + be[1] + be[2];
-    const T ln_J = (J > T(0)) ? log(J) : T(0);
+    const T ln_J = pos ? lnJ : ((J > T(0)) ? log(J) : T(0));
     const T Psi_vol = (kappa * T(0.5)) * (((J * J - T(1)) * T(0.5)) - ln_J);
     const T Psi_dev = (mu * T(0.5)) * (be_sum - T(3));
     const T Psi_pos = (J < T(1)) ? Psi_dev : Psi_dev + Psi_vol;
