@@ -231,24 +231,49 @@ __global__ __launch_bounds__(kRsThreads) void k_radix_hist(const uint32_t* __res
 }
 
 // stable scatter: items are ranked in (round, wave, lane) order, which is their
-// index order inside the tile; same-digit peers inside a wave are found with
-// eight 64-lane ballots
+// index order inside the tile (same-digit peers inside a wave are found with
+// eight 64-lane ballots), and put in their digit's run of an LDS copy of the
+// tile; the tile then leaves in that order, so that consecutive lanes write
+// consecutive positions of a digit's run instead of 256 scattered words per
+// round.  The tile's count of digit d is the scanned histogram's next
+// digit-major entry minus its own (n after the last).  All items of the tile
+// are loaded before the ranking starts.
 __global__ __launch_bounds__(kRsThreads) void k_radix_scatter(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint32_t* __restrict__ okeys,
     uint32_t* __restrict__ ovals, int n, int shift, const uint32_t* __restrict__ offs, int tiles) {
-    __shared__ uint32_t running[256];
+    __shared__ uint32_t gstart[256];   // output position of the tile's first item of digit d
+    __shared__ uint32_t lstart[256];   // its slot in the tile copy
+    __shared__ uint32_t lnext[256];    // next free slot of digit d
     __shared__ uint32_t wcnt[4][256];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    running[threadIdx.x] = offs[threadIdx.x * tiles + blockIdx.x];
+    __shared__ uint32_t skey[kRsTile], sval[kRsTile];
+    __shared__ int lds4[4];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int base = blockIdx.x * kRsTile;
+    uint32_t k[kRsItems], v[kRsItems];
+#pragma unroll
+    for (int j = 0; j < kRsItems; ++j) {
+        const int i = base + j * kRsThreads + tid;
+        k[j] = i < n ? keys[i] : 0u;
+        v[j] = i < n ? (vals ? vals[i] : (uint32_t)i) : 0u;
+    }
+    {
+        const int64_t e = (int64_t)tid * tiles + blockIdx.x;
+        const uint32_t g0 = offs[e];
+        const uint32_t g1 = e + 1 < (int64_t)256 * tiles ? offs[e + 1] : (uint32_t)n;
+        int total;
+        const int ls = block_excl_scan256((int)(g1 - g0), total, lds4);
+        gstart[tid] = g0;
+        lstart[tid] = (uint32_t)ls;
+        lnext[tid] = (uint32_t)ls;
+    }
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+#pragma unroll
     for (int j = 0; j < kRsItems; ++j) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) wcnt[q][threadIdx.x] = 0;
+        for (int q = 0; q < 4; ++q) wcnt[q][tid] = 0;
         __syncthreads();
-        const int i = blockIdx.x * kRsTile + j * kRsThreads + threadIdx.x;
-        const bool valid = i < n;
-        const uint32_t k = valid ? keys[i] : 0u;
-        const uint32_t d = (k >> shift) & 255u;
+        const bool valid = base + j * kRsThreads + tid < n;
+        const uint32_t d = (k[j] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -260,15 +285,22 @@ __global__ __launch_bounds__(kRsThreads) void k_radix_scatter(
         if (valid && (peers & lt) == 0ull) wcnt[w][d] = __popcll(peers);
         __syncthreads();
         if (valid) {
-            uint32_t pos = running[d] + rank;
+            uint32_t pos = lnext[d] + rank;
             for (int q = 0; q < w; ++q) pos += wcnt[q][d];
-            okeys[pos] = k;
-            ovals[pos] = vals ? vals[i] : (uint32_t)i;
+            skey[pos] = k[j];
+            sval[pos] = v[j];
         }
         __syncthreads();
-        running[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] + wcnt[2][threadIdx.x] +
-                                wcnt[3][threadIdx.x];
-        __syncthreads();
+        lnext[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+    }
+    __syncthreads();
+    const int cnt = n - base < kRsTile ? n - base : kRsTile;
+    for (int q = tid; q < cnt; q += kRsThreads) {
+        const uint32_t kq = skey[q];
+        const uint32_t d = (kq >> shift) & 255u;
+        const uint32_t g = gstart[d] + ((uint32_t)q - lstart[d]);
+        okeys[g] = kq;
+        ovals[g] = sval[q];
     }
 }
 
