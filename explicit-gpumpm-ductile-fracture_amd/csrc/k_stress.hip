@@ -393,8 +393,8 @@ __global__ __launch_bounds__(256) void k_stress(DevParams dp, ST s, WaveMap wm, 
 // particle's P2G payload record and its page's channel bounds, from the
 // stress still in registers (the same values k_p2g_prep reads back from
 // sigma, FP and V, so the records are bit-identical); pbound is zero on entry.
-// Three waves per SIMD (168 VGPRs, a 20-byte spill): 138 us against 161 us at
-// the two waves the unconstrained allocation (173 VGPRs) allows, 175 us at four.
+// Three waves per SIMD (158 VGPRs): 138 us against 161 us at the two waves an
+// unconstrained allocation allows, 175 us at four (1M particles).
 template <bool ALL64, int TRANSFER, typename ST>
 __global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, WaveMap wm, float* __restrict__ sigma,
                                                      float* __restrict__ FP, float* __restrict__ vol, BlockTables t,
