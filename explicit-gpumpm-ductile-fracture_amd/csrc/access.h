@@ -303,33 +303,52 @@ __device__ __forceinline__ int class_entry(const BlockClasses& S, int cls, int j
 }
 
 // The order in which the block kernels take the blocks of XCD share x (items
-// [x s, (x + 1) s), s = ceil(nb / 8), as xcd_item_first splits them): the
-// blocks whose gather threads take two or more entries each (more than 248
-// entries) first, then the others, each group in block order, so that the
-// long blocks do not start last and leave the XCD's CUs idle at the end.
-// One wave; deterministic.  kOrderGroups workgroups (one share each) build it.
+// [x s, (x + 1) s), s = ceil(nb / 8), as xcd_item_first splits them): by
+// work, longest first, in power-of-two classes of the block's gather entries
+// (class floor(log2 entries), highest first, each class in block order), so
+// that the long blocks do not start last and leave the XCD's CUs idle at the
+// end.  One wave; deterministic.  kOrderGroups workgroups (one share each)
+// build it.
 constexpr int kOrderGroups = 8;
+constexpr int kOrderClasses = 16;
+__device__ __forceinline__ int work_class(int entries) {
+    const int c = entries > 0 ? 31 - __clz(entries) : 0;
+    return c < kOrderClasses - 1 ? c : kOrderClasses - 1;
+}
 __device__ __forceinline__ void order_share_wave(const BlockClasses* __restrict__ lists, int* __restrict__ order,
                                                  int x, int nb) {
     const int lane = threadIdx.x & 63;
     const int s = (nb + 7) >> 3;
     const int lo = x * s, hi = lo + s < nb ? lo + s : nb;
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    int heavy = 0;
+    // blocks per class (lane c < kOrderClasses holds class c's count)
+    int cnt = 0;
     for (int b0 = lo; b0 < hi; b0 += 64) {
         const int b = b0 + lane;
-        heavy += __popcll(__ballot(b < hi && lists[b].pre[kMaxRuns] > 248));
+        const int c = b < hi ? work_class(lists[b].pre[kMaxRuns]) : -1;
+#pragma unroll
+        for (int k = 0; k < kOrderClasses; ++k) {
+            const int m = __popcll(__ballot(c == k));
+            if (lane == k) cnt += m;
+        }
     }
-    int ph = lo, pl = lo + heavy;
+    // first position of each class: the blocks of all higher classes before it
+    int pos = lo;
+#pragma unroll
+    for (int k = kOrderClasses - 1; k > 0; --k) {
+        const int ck = __shfl(cnt, k, 64);
+        if (lane < k) pos += ck;
+    }
     for (int b0 = lo; b0 < hi; b0 += 64) {
         const int b = b0 + lane;
-        const bool in = b < hi;
-        const bool h = in && lists[b].pre[kMaxRuns] > 248;
-        const uint64_t mh = __ballot(h), ml = __ballot(in && !h);
-        if (h) order[ph + __popcll(mh & below)] = b;
-        else if (in) order[pl + __popcll(ml & below)] = b;
-        ph += __popcll(mh);
-        pl += __popcll(ml);
+        const int c = b < hi ? work_class(lists[b].pre[kMaxRuns]) : -1;
+#pragma unroll
+        for (int k = 0; k < kOrderClasses; ++k) {
+            const uint64_t m = __ballot(c == k);
+            const int pk = __shfl(pos, k, 64);
+            if (c == k) order[pk + __popcll(m & below)] = b;
+            if (lane == k) pos += __popcll(m);
+        }
     }
 }
 

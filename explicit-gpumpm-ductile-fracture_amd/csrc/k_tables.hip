@@ -44,7 +44,7 @@ __global__ void k_pack_pages(uint64_t* __restrict__ pflag, uint64_t* __restrict_
         const uint64_t b = f[q];   // flags of pages 8q .. 8q + 7 of the word, one byte each
 #pragma unroll
         for (int j = 0; j < 8; ++j) m |= ((b >> (8 * j)) & 1ull) << (8 * q + j);
-        f[q] = 0ull;
+        if (b) f[q] = 0ull;   // most of a batch's page space is empty: clear only what was set
     }
     pmask[w] = m;
 }
