@@ -302,53 +302,74 @@ __device__ __forceinline__ int class_entry(const BlockClasses& S, int cls, int j
     return S.start[lo] + (e - S.pre[lo]);
 }
 
-// The order in which the block kernels take the blocks of XCD share x (items
-// [x s, (x + 1) s), s = ceil(nb / 8), as xcd_item_first splits them): by
-// work, longest first, in power-of-two classes of the block's gather entries
-// (class floor(log2 entries), highest first, each class in block order), so
-// that the long blocks do not start last and leave the XCD's CUs idle at the
-// end.  One wave; deterministic.  kOrderGroups workgroups (one share each)
-// build it.
-constexpr int kOrderGroups = 8;
+// The order in which the block kernels take the blocks: by work, longest
+// first, in power-of-two classes of the block's gather entries (class
+// floor(log2 entries), highest first, each class in block order), dealt
+// round-robin over the 8 XCD shares (sorted rank g to share g mod 8 at
+// position g / 8, the last nb mod 8 ranks to the shares that hold one item
+// more), so that every XCD gets the same mix of long and short blocks and
+// takes its long blocks first.  Built by one workgroup (any size that is a
+// multiple of 64, at most 256 threads); deterministic.
+constexpr int kOrderGroups = 1;
 constexpr int kOrderClasses = 16;
+__device__ __forceinline__ int wave_incl_scan_i(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
 __device__ __forceinline__ int work_class(int entries) {
     const int c = entries > 0 ? 31 - __clz(entries) : 0;
     return c < kOrderClasses - 1 ? c : kOrderClasses - 1;
 }
-__device__ __forceinline__ void order_share_wave(const BlockClasses* __restrict__ lists, int* __restrict__ order,
-                                                 int x, int nb) {
-    const int lane = threadIdx.x & 63;
-    const int s = (nb + 7) >> 3;
-    const int lo = x * s, hi = lo + s < nb ? lo + s : nb;
-    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    // blocks per class (lane c < kOrderClasses holds class c's count)
-    int cnt = 0;
-    for (int b0 = lo; b0 < hi; b0 += 64) {
-        const int b = b0 + lane;
-        const int c = b < hi ? work_class(lists[b].pre[kMaxRuns]) : -1;
-#pragma unroll
-        for (int k = 0; k < kOrderClasses; ++k) {
-            const int m = __popcll(__ballot(c == k));
-            if (lane == k) cnt += m;
+// item of sorted rank g among nb
+__device__ __forceinline__ int dealt_item(int g, int nb) {
+    const int K = nb >> 3;
+    if (g < 8 * K) return xcd_share_lo(g & 7, nb) + (g >> 3);
+    int rr = g - 8 * K;
+    for (int x = 0; x < 8; ++x) {
+        const int lo = xcd_share_lo(x, nb);
+        if (xcd_share_lo(x + 1, nb) - lo > K) {
+            if (rr == 0) return lo + K;
+            --rr;
         }
     }
-    // first position of each class: the blocks of all higher classes before it
-    int pos = lo;
-#pragma unroll
-    for (int k = kOrderClasses - 1; k > 0; --k) {
-        const int ck = __shfl(cnt, k, 64);
-        if (lane < k) pos += ck;
+    return g;
+}
+// cnt: kOrderClasses x blockDim.x ints of LDS, lds4: 4 ints
+__device__ __forceinline__ void order_blocks(const BlockClasses* __restrict__ lists, int* __restrict__ order, int nb,
+                                             int* cnt, int* lds4) {
+    const int T = (int)blockDim.x, t = (int)threadIdx.x;
+    const int chunk = (nb + T - 1) / T;
+    const int b0 = t * chunk, b1 = b0 + chunk < nb ? b0 + chunk : nb;
+    // counts per (class, thread), laid out highest class first, then by thread
+    for (int k = 0; k < kOrderClasses; ++k) cnt[k * T + t] = 0;
+    for (int b = b0; b < b1; ++b) ++cnt[(kOrderClasses - 1 - work_class(lists[b].pre[kMaxRuns])) * T + t];
+    __syncthreads();
+    // exclusive scan of the kOrderClasses x T counts: kOrderClasses consecutive
+    // entries per thread, then across threads
+    int part = 0;
+    for (int q = 0; q < kOrderClasses; ++q) part += cnt[t * kOrderClasses + q];
+    const int lane = t & 63, w = t >> 6, nw = T >> 6;
+    const int inc = wave_incl_scan_i(part);
+    if (lane == 63) lds4[w] = inc;
+    __syncthreads();
+    int off = inc - part;
+    for (int q = 0; q < w && q < nw; ++q) off += lds4[q];
+    for (int q = 0; q < kOrderClasses; ++q) {
+        const int v = cnt[t * kOrderClasses + q];
+        cnt[t * kOrderClasses + q] = off;
+        off += v;
     }
-    for (int b0 = lo; b0 < hi; b0 += 64) {
-        const int b = b0 + lane;
-        const int c = b < hi ? work_class(lists[b].pre[kMaxRuns]) : -1;
-#pragma unroll
-        for (int k = 0; k < kOrderClasses; ++k) {
-            const uint64_t m = __ballot(c == k);
-            const int pk = __shfl(pos, k, 64);
-            if (c == k) order[pk + __popcll(m & below)] = b;
-            if (lane == k) pos += __popcll(m);
-        }
+    __syncthreads();
+    // each thread places its blocks at their ranks (its own column: no races)
+    for (int b = b0; b < b1; ++b) {
+        const int k = kOrderClasses - 1 - work_class(lists[b].pre[kMaxRuns]);
+        const int g = cnt[k * T + t]++;
+        order[dealt_item(g, nb)] = b;
     }
 }
 

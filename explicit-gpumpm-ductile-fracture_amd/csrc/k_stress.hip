@@ -402,14 +402,14 @@ __global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, Wave
                                                      int n, const BlockClasses* __restrict__ lists,
                                                      int* __restrict__ order, DevCounts* dc) {
     dt = substep_dt(dt, dc);
-    // the last 8 workgroups order the blocks of the 8 XCD shares for the
-    // block kernels that follow (k_block_lists has just listed them)
+    // the last workgroup orders the blocks for the block kernels that follow
+    // (k_block_lists has just listed them)
     if ((int)blockIdx.x >= (int)gridDim.x - kOrderGroups) {
-        if (threadIdx.x < 64) {
-            int nb = dc->num_blocks;
-            nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
-            order_share_wave(lists, order, (int)blockIdx.x - ((int)gridDim.x - kOrderGroups), nb);
-        }
+        __shared__ int ocnt[kOrderClasses * 256];
+        __shared__ int olds[4];
+        int nb = dc->num_blocks;
+        nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
+        order_blocks(lists, order, nb, ocnt, olds);
         return;
     }
     int i, scene;

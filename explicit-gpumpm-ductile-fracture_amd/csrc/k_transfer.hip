@@ -429,10 +429,10 @@ static void g2p_block(mpmx_ctx* c, ST st, float dt) {
 
 template <typename ST>
 static void g2p_block_kernel(mpmx_ctx* c, ST st, float dt) {
-    // the blocks in their own order: the G2P's work per block is its own
-    // particles, not the gather entries the order ranks by
-    BlockTables t = block_tables_of(c);
-    t.order = nullptr;
+    // the blocks in the gathers' order (by gather entries; a block's own
+    // particles, the G2P's work, go with them): 0.34 -> 0.29 ms on the 8M
+    // impact against block order
+    const BlockTables t = block_tables_of(c);
     const int g = block_grid(c, c->g2p_pct);
     MPMX_TIMED(c, "g2p_block", ({
                    if (c->P.transfer == 0)

@@ -422,22 +422,23 @@ __device__ __forceinline__ float combine_waves(float v, float* lds) {
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
 // Items of `count` for the calling workgroup, `per_wg` consecutive items per
 // workgroup step (1, or one per wave): XCD x = blockIdx % 8 owns the
-// contiguous share [x s, (x + 1) s) of the items, s = ceil(count / 8), so that
-// neighbouring items share an L2 and every XCD gets an equal share of the
-// items that exist (the grids are sized with slack for growth, and the
-// workgroups beyond a share exit at once); its workgroups j = blockIdx / 8
-// take the share's items j per_wg, + per_wg G / 8, ...  A grid that is not a
-// multiple of 8 strides over all items instead.
+// contiguous share [floor(x count / 8), floor((x + 1) count / 8)) of the
+// items, so that every XCD gets an equal share of the items that exist (the
+// grids are sized with slack for growth, and the workgroups beyond a share
+// exit at once); its workgroups j = blockIdx / 8 take the share's items
+// j per_wg, + per_wg G / 8, ...  A grid that is not a multiple of 8 strides
+// over all items instead.
+// first item of XCD share x of count items: shares differ by at most one item
+__device__ __forceinline__ int xcd_share_lo(int x, int count) { return (int)(((int64_t)x * count) >> 3); }
 __device__ __forceinline__ int xcd_item_first(int count, int per_wg) {
     const int G = gridDim.x, b = blockIdx.x;
     if (G & 7) return b * per_wg;
-    return (b & 7) * ((count + 7) >> 3) + (b >> 3) * per_wg;
+    return xcd_share_lo(b & 7, count) + (b >> 3) * per_wg;
 }
 __device__ __forceinline__ int xcd_item_end(int count) {
     const int G = gridDim.x, b = blockIdx.x;
     if (G & 7) return count;
-    const int e = ((b & 7) + 1) * ((count + 7) >> 3);
-    return e < count ? e : count;
+    return xcd_share_lo((b & 7) + 1, count);
 }
 __device__ __forceinline__ int xcd_item_step(int per_wg) {
     const int G = gridDim.x;
