@@ -303,15 +303,16 @@ __device__ __forceinline__ int class_entry(const BlockClasses& S, int cls, int j
 }
 
 // The order in which the block kernels take the blocks: by work, longest
-// first, in power-of-two classes of the block's gather entries (class
-// floor(log2 entries), highest first, each class in block order), dealt
+// first, in half-octave classes of the block's gather entries (2^l <= n <
+// 1.5 2^l and 1.5 2^l <= n < 2^(l+1), highest first, each class in block
+// order), dealt
 // round-robin over the 8 XCD shares (sorted rank g to share g mod 8 at
 // position g / 8, the last nb mod 8 ranks to the shares that hold one item
 // more), so that every XCD gets the same mix of long and short blocks and
 // takes its long blocks first.  Built by one workgroup (any size that is a
 // multiple of 64, at most 256 threads); deterministic.
 constexpr int kOrderGroups = 1;
-constexpr int kOrderClasses = 16;
+constexpr int kOrderClasses = 32;
 __device__ __forceinline__ int wave_incl_scan_i(int v) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -322,7 +323,9 @@ __device__ __forceinline__ int wave_incl_scan_i(int v) {
     return v;
 }
 __device__ __forceinline__ int work_class(int entries) {
-    const int c = entries > 0 ? 31 - __clz(entries) : 0;
+    if (entries <= 1) return 0;
+    const int l = 31 - __clz(entries);
+    const int c = 2 * l + (((entries >> (l - 1)) & 1) ? 1 : 0);   // half octaves: 2^l, 1.5 2^l
     return c < kOrderClasses - 1 ? c : kOrderClasses - 1;
 }
 // item of sorted rank g among nb
