@@ -157,8 +157,8 @@ __device__ __forceinline__ void class_run(int cls, int r, int& c0, int& len) {
 // the block's own 8 grid pages), the 132 runs
 // of passing cells (three per lane) with their particle ranges read from the
 // cell starts, the exclusive prefix over the runs, and the threads of each
-// class of the 256-thread gathers.
-__device__ __forceinline__ void block_lists_wave(const BlockTables& t, uint32_t bkey, int n, int np_raw, int np,
+// class of the 256-thread gathers.  Returns the block's entries.
+__device__ __forceinline__ int block_lists_wave(const BlockTables& t, uint32_t bkey, int n, int np_raw, int np,
                                                  int nt, BlockClasses* __restrict__ dst, BlockG2P* __restrict__ g2p) {
     const int lane = threadIdx.x & 63;
     int bc[3];
@@ -284,6 +284,7 @@ __device__ __forceinline__ void block_lists_wave(const BlockTables& t, uint32_t 
         dst->len[lane] = thr > 0 ? L : 0;
     }
     if (lane == 7) dst->thr0[8] = tin;
+    return all;
 }
 
 // particles listed for class cls, and entry j of them
@@ -343,14 +344,16 @@ __device__ __forceinline__ int dealt_item(int g, int nb) {
     return g;
 }
 // cnt: kOrderClasses x blockDim.x ints of LDS, lds4: 4 ints
-__device__ __forceinline__ void order_blocks(const BlockClasses* __restrict__ lists, int* __restrict__ order, int nb,
+// wclass: the work class of every block (k_block_lists writes it, one byte
+// per block, so that each thread reads a contiguous stretch)
+__device__ __forceinline__ void order_blocks(const uint8_t* __restrict__ wclass, int* __restrict__ order, int nb,
                                              int* cnt, int* lds4) {
     const int T = (int)blockDim.x, t = (int)threadIdx.x;
     const int chunk = (nb + T - 1) / T;
     const int b0 = t * chunk, b1 = b0 + chunk < nb ? b0 + chunk : nb;
     // counts per (class, thread), laid out highest class first, then by thread
     for (int k = 0; k < kOrderClasses; ++k) cnt[k * T + t] = 0;
-    for (int b = b0; b < b1; ++b) ++cnt[(kOrderClasses - 1 - work_class(lists[b].pre[kMaxRuns])) * T + t];
+    for (int b = b0; b < b1; ++b) ++cnt[(kOrderClasses - 1 - (int)wclass[b]) * T + t];
     __syncthreads();
     // exclusive scan of the kOrderClasses x T counts: kOrderClasses consecutive
     // entries per thread, then across threads
@@ -370,7 +373,7 @@ __device__ __forceinline__ void order_blocks(const BlockClasses* __restrict__ li
     __syncthreads();
     // each thread places its blocks at their ranks (its own column: no races)
     for (int b = b0; b < b1; ++b) {
-        const int k = kOrderClasses - 1 - work_class(lists[b].pre[kMaxRuns]);
+        const int k = kOrderClasses - 1 - (int)wclass[b];
         const int g = cnt[k * T + t]++;
         order[dealt_item(g, nb)] = b;
     }

@@ -438,6 +438,7 @@ static void free_particles(mpmx_ctx* c) {
     if (c->block_g2p) (void)hipFree(c->block_g2p);
     c->block_g2p = nullptr;
     dfree(c->block_order);
+    dfree(c->block_wclass);
     c->order_ok = false;
     dfree(c->grid); dfree(c->grid_p2g);
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_z); dfree(c->cg_s); dfree(c->cg_q); dfree(c->cg_part); dfree(c->cg_amask);
@@ -830,6 +831,7 @@ static mpmx_status upload_particles(mpmx_ctx* c, const int64_t* counts, const st
     HIPCHK(c, hipMalloc(&c->block_lists, sizeof(BlockClasses) * (size_t)tp_bound));
     HIPCHK(c, hipMalloc(&c->block_g2p, sizeof(BlockG2P) * (size_t)tp_bound));
     HIPCHK(c, dalloc(&c->block_order, tp_bound));
+    HIPCHK(c, dalloc(&c->block_wclass, tp_bound));
     HIPCHK(c, dalloc(&c->adj, 7 * c->cap_p));
     HIPCHK(c, dalloc(&c->nvp, c->cap_p + 1));
     HIPCHK(c, dalloc(&c->vpo, c->cap_p + 1));

@@ -357,8 +357,9 @@ __global__ __launch_bounds__(256) void k_grid_update(DevParams dp, const uint32_
 // wave per block; they depend on the page tables only and serve the block
 // P2G, the K s gathers of the substep's PCG and the block G2P
 __global__ __launch_bounds__(256) void k_block_lists(DevParams dp, BlockTables t, BlockClasses* __restrict__ lists,
-                                                     BlockG2P* __restrict__ g2p, int n, SceneCG* __restrict__ scg,
-                                                     unsigned* __restrict__ pbound, DevCounts* dc) {
+                                                     BlockG2P* __restrict__ g2p, uint8_t* __restrict__ wclass, int n,
+                                                     SceneCG* __restrict__ scg, unsigned* __restrict__ pbound,
+                                                     DevCounts* dc) {
     // the substep's PCG reset and the clear of the per-page channel bounds
     // ride along (the next kernels, the stress pass and the block P2G, need
     // both)
@@ -379,8 +380,10 @@ __global__ __launch_bounds__(256) void k_block_lists(DevParams dp, BlockTables t
     // one block per wave
     const int per = (int)(blockDim.x >> 6), wave = (int)(threadIdx.x >> 6);
     const int end = xcd_item_end(nb), step = xcd_item_step(per);
-    for (int blk = xcd_item_first(nb, per) + wave; blk < end; blk += step)
-        block_lists_wave(t, t.block_key[blk], n, np_raw, np, nt, lists + blk, g2p + blk);
+    for (int blk = xcd_item_first(nb, per) + wave; blk < end; blk += step) {
+        const int entries = block_lists_wave(t, t.block_key[blk], n, np_raw, np, nt, lists + blk, g2p + blk);
+        if ((threadIdx.x & 63) == 0) wclass[blk] = (uint8_t)work_class(entries);
+    }
 }
 
 hipError_t launch_block_lists(mpmx_ctx* c) {
@@ -388,8 +391,8 @@ hipError_t launch_block_lists(mpmx_ctx* c) {
     g = (g + 7) / 8 * 8;
     MPMX_TIMED(c, "block_lists", (k_block_lists<<<g, 256, 0, c->stream>>>(
                                       c->dp, block_tables_of(c), static_cast<BlockClasses*>(c->block_lists),
-                                      static_cast<BlockG2P*>(c->block_g2p), (int)c->n, c->scg, c->pbound,
-                                      c->dc)));
+                                      static_cast<BlockG2P*>(c->block_g2p), c->block_wclass, (int)c->n, c->scg,
+                                      c->pbound, c->dc)));
     return hipGetLastError();
 }
 

@@ -399,21 +399,22 @@ template <bool ALL64, int TRANSFER, typename ST>
 __global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, WaveMap wm, float* __restrict__ sigma,
                                                      float* __restrict__ FP, float* __restrict__ vol, BlockTables t,
                                                      float* __restrict__ pay, unsigned* __restrict__ pbound, float dt,
-                                                     int n, const BlockClasses* __restrict__ lists,
+                                                     int n, const uint8_t* __restrict__ wclass,
                                                      int* __restrict__ order, DevCounts* dc) {
     dt = substep_dt(dt, dc);
-    // the last workgroup orders the blocks for the block kernels that follow
-    // (k_block_lists has just listed them)
-    if ((int)blockIdx.x >= (int)gridDim.x - kOrderGroups) {
+    // the first workgroup orders the blocks for the block kernels that follow
+    // (k_block_lists has just listed them); dispatched first, it runs beside
+    // the stress waves instead of after them
+    if ((int)blockIdx.x < kOrderGroups) {
         __shared__ int ocnt[kOrderClasses * 256];
         __shared__ int olds[4];
         int nb = dc->num_blocks;
         nb = nb < t.cap_blocks ? nb : (int)t.cap_blocks;
-        order_blocks(lists, order, nb, ocnt, olds);
+        order_blocks(wclass, order, nb, ocnt, olds);
         return;
     }
     int i, scene;
-    const bool valid = wave_particle(wm, n, i, scene);
+    const bool valid = wave_particle(wm, n, (int)blockIdx.x - kOrderGroups, i, scene);
     float b[kP2GCh];
 #pragma unroll
     for (int k = 0; k < kP2GCh; ++k) b[k] = 0.f;
@@ -459,7 +460,7 @@ template <bool ALL64, int TRANSFER, typename ST>
 static void stress_prep_dispatch(mpmx_ctx* c, ST st, float dt) {
     k_stress_prep<ALL64, TRANSFER, ST><<<particle_blocks(c) + kOrderGroups, c->particle_wg, 0, c->stream>>>(
         c->dp, st, c->wmap, c->cfg.store_stress ? c->sigma : nullptr, c->FP, c->vol, block_tables_of(c), c->payload,
-        c->pbound, dt, (int)c->n, static_cast<const BlockClasses*>(c->block_lists), c->block_order, c->dc);
+        c->pbound, dt, (int)c->n, c->block_wclass, c->block_order, c->dc);
 }
 
 template <bool ALL64, typename ST>

@@ -230,6 +230,7 @@ struct mpmx_ctx {
     void* block_lists = nullptr;     // [tp_bound] per-block source lists of the block P2G (BlockClasses)
     void* block_g2p = nullptr;       // [tp_bound] per-block G2P pages and particle range (BlockG2P)
     int* block_order = nullptr;      // [tp_bound] blocks of each XCD share, heavy ones first
+    uint8_t* block_wclass = nullptr; // [tp_bound] work class of each block (its gather entries)
     void* p2g_part = nullptr;        // [cap_part] per-block records of the owner-block P2G (p2g_variant 3)
     int64_t cap_part = 0;
     bool vpages_valid = false;       // adjacency and virtual pages match the current tables
@@ -471,9 +472,10 @@ __device__ __forceinline__ void block_range(int count, int& begin, int& end) {
 // scene of page key pk
 __device__ __forceinline__ int scene_of_page(const DevParams& dp, uint32_t pk) { return (int)(pk >> dp.slot_shift); }
 
-// the calling lane's particle i and the wave's scene; false for idle lanes
-__device__ __forceinline__ bool wave_particle(const WaveMap& m, int n, int& i, int& scene) {
-    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+// the calling lane's particle i and the wave's scene, for workgroup wg of the
+// particle grid; false for idle lanes
+__device__ __forceinline__ bool wave_particle(const WaveMap& m, int n, int wg, int& i, int& scene) {
+    const int64_t w = ((int64_t)wg * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (!m.first) {
         i = (int)(w * 64 + lane);
@@ -488,6 +490,9 @@ __device__ __forceinline__ bool wave_particle(const WaveMap& m, int n, int& i, i
     i = m.first[w] + lane;
     scene = __builtin_amdgcn_readfirstlane(m.scene[w]);
     return lane < m.count[w];
+}
+__device__ __forceinline__ bool wave_particle(const WaveMap& m, int n, int& i, int& scene) {
+    return wave_particle(m, n, (int)blockIdx.x, i, scene);
 }
 
 // The 128-node blocks of the PCG's "dot" products, per scene: block b of a
