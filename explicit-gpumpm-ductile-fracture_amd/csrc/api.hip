@@ -928,12 +928,23 @@ static mpmx_status g2p_dt(mpmx_ctx* c, double dt) {
     c->sorted = false;
     return MPMX_OK;
 }
+// the sort of a substep: ahead of the fused stress pass the state gather
+// waits for that pass (sort_keys), which performs it on the way; the scatter
+// P2G takes the state sorted
+static mpmx_status substep_sort(mpmx_ctx* c) {
+    if (c->sorted || c->cfg.p2g_variant == 1) return mpmx_bin_sort(c);
+    MPMX_TIMED(c, "sort", HIPCHK(c, sort_keys(c)));
+    c->sorted = true;
+    return MPMX_OK;
+}
 static mpmx_status substep_dt(mpmx_ctx* c, double dt) {
     HIPCHK(c, hipSetDevice(c->device));
     mpmx_status s;
-    if ((s = mpmx_bin_sort(c)) != MPMX_OK) return s;
-    if ((s = mpmx_rebuild(c)) != MPMX_OK) return s;
-    if ((s = stress_and_p2g(c, dt)) != MPMX_OK) return s;
+    if ((s = substep_sort(c)) != MPMX_OK) return s;
+    s = mpmx_rebuild(c);
+    if (s == MPMX_OK) s = stress_and_p2g(c, dt);
+    if (c->gather_perm) HIPCHK(c, finish_gather(c));   // a stage before the stress pass failed
+    if (s != MPMX_OK) return s;
     if ((s = grid_update_dt(c, dt)) != MPMX_OK) return s;
     if ((s = mpmx_phase_field(c, dt)) != MPMX_OK) return s;
     if (c->dp.phase_field) note_cg_iters(c, c->cg_spec);
@@ -992,7 +1003,7 @@ static mpmx_status capture_substep(mpmx_ctx* c, double dt, int iters) {
     hipGraph_t ga = nullptr, gb = nullptr;
     mpmx_status s = MPMX_OK;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    if ((s = mpmx_bin_sort(c)) == MPMX_OK && (s = mpmx_rebuild(c)) == MPMX_OK &&
+    if ((s = substep_sort(c)) == MPMX_OK && (s = mpmx_rebuild(c)) == MPMX_OK &&
         (s = stress_and_p2g(c, dt)) == MPMX_OK && (s = grid_update_dt(c, dt)) == MPMX_OK) {
         hipError_t e = c->dp.phase_field ? queue_pcg_iterations(c, 0, iters) : hipSuccess;
         if (e == hipSuccess)
@@ -1018,6 +1029,7 @@ static mpmx_status capture_substep(mpmx_ctx* c, double dt, int iters) {
     c->keys = keys0;
     c->keys_alt = keys1;
     c->sorted = false;
+    c->gather_perm = nullptr;
     if (s != MPMX_OK) {
         if (G.a) (void)hipGraphExecDestroy(G.a);
         if (G.b) (void)hipGraphExecDestroy(G.b);

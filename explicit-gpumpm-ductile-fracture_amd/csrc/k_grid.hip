@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_p2g_prep(DevParams dp, ST s, WaveMap wm
         float sig[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) sig[k] = sigma[(int64_t)k * n + i];
-        prep_particle<TRANSFER>(dp.scenes[scene], s, sig, FP[i], vol[i], i, n, dt, pay, b);
+        prep_particle<TRANSFER>(dp.scenes[scene], s, sig, FP[i], vol[i], i, i, n, dt, pay, b);
     }
     page_bounds(valid, pk, b, t, pbound, dc);
 }

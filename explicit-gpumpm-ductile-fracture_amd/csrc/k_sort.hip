@@ -383,7 +383,9 @@ __global__ void k_gather_mixed(const uint32_t* __restrict__ perm, MState s, MSta
     d.id[i] = id;
 }
 
-hipError_t sort_and_gather(mpmx_ctx* c) {
+// the radix passes alone: sorted keys in c->keys and the permutation
+// (new -> old) in c->gather_perm; the particle state still waits in c->cur
+hipError_t sort_keys(mpmx_ctx* c) {
     const int n = (int)c->n;
     const int tiles = c->sort_tiles;
     uint32_t* kb[2] = {c->keys, c->keys_alt};
@@ -410,13 +412,28 @@ hipError_t sort_and_gather(mpmx_ctx* c) {
     if (ki != 0) std::swap(c->keys, c->keys_alt);
     const uint32_t* perm = vi < 0 ? nullptr : vb[vi];
     if (!perm) return hipErrorInvalidValue;   // no pass ran: key_bits is at least 1
+    c->gather_perm = perm;
+    return hipGetLastError();
+}
+
+// the state gather that sort_keys left pending, on its own
+hipError_t finish_gather(mpmx_ctx* c) {
+    const uint32_t* perm = c->gather_perm;
+    if (!perm) return hipSuccess;
+    const int n = (int)c->n;
     const int nxt = 1 - c->cur;
     if (c->cfg.precision != MPMX_FP32)
         k_gather_mixed<<<(n + 255) / 256, 256, 0, c->stream>>>(perm, c->ms[c->cur], c->ms[nxt], n);
     else
         k_gather_fp32<<<(n + 255) / 256, 256, 0, c->stream>>>(perm, c->ps[c->cur], c->ps[nxt], n);
     c->cur = nxt;
+    c->gather_perm = nullptr;
     return hipGetLastError();
+}
+
+hipError_t sort_and_gather(mpmx_ctx* c) {
+    hipError_t e = sort_keys(c);
+    return e == hipSuccess ? finish_gather(c) : e;
 }
 
 bool sort_swaps_keys(const mpmx_ctx* c) { return ((c->key_bits + 7) / 8) % 2 == 1; }

@@ -389,14 +389,48 @@ __global__ __launch_bounds__(256) void k_stress(DevParams dp, ST s, WaveMap wm, 
     stress_write(s, i, n, sigma, FP, vol, dc, out);
 }
 
+// the state fields the stress leaves unchanged, moved from o of s to i of d
+// (every load before any store)
+template <typename ST>
+__device__ __forceinline__ void carry_state(const ST& s, int o, const ST& d, int i, int n) {
+    decltype(+s.x[0]) x[3];
+    decltype(+s.v[0]) v[3];
+    decltype(+s.B[0]) B[9];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        x[a] = s.x[3 * o + a];
+        v[a] = s.v[3 * o + a];
+    }
+#pragma unroll
+    for (int a = 0; a < 9; ++a) B[a] = s.B[(int64_t)a * n + o];
+    const auto c = s.c[o];
+    const auto mass = s.mass[o];
+    const int id = s.id[o];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        d.x[3 * i + a] = x[a];
+        d.v[3 * i + a] = v[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 9; ++a) d.B[(int64_t)a * n + i] = B[a];
+    d.c[i] = c;
+    d.mass[i] = mass;
+    d.id[i] = id;
+}
+
 // the stress pass of a substep with the gather P2G: after the stress, the
 // particle's P2G payload record and its page's channel bounds, from the
 // stress still in registers (the same values k_p2g_prep reads back from
 // sigma, FP and V, so the records are bit-identical); pbound is zero on entry.
+// With perm set the pass also performs the sort's state gather: sorted
+// particle i reads its state at perm[i] of s and writes all of it to i of d
+// (the stand-alone gather's reads are the ones the stress makes anyway);
+// without, s and d are the same sorted state.
 // Three waves per SIMD (158 VGPRs): 138 us against 161 us at the two waves an
 // unconstrained allocation allows, 175 us at four (1M particles).
 template <bool ALL64, int TRANSFER, typename ST>
-__global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, WaveMap wm, float* __restrict__ sigma,
+__global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, ST d, const uint32_t* __restrict__ perm,
+                                                     WaveMap wm, float* __restrict__ sigma,
                                                      float* __restrict__ FP, float* __restrict__ vol, BlockTables t,
                                                      float* __restrict__ pay, unsigned* __restrict__ pbound, float dt,
                                                      int n, const uint8_t* __restrict__ wclass,
@@ -421,8 +455,9 @@ __global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, Wave
     uint32_t pk = 0xffffffffu;
     if (valid) {
         const DevParams& sp = dp.scenes[scene];
+        const int o = perm ? (int)perm[i] : i;
         StressOut out;
-        stress_compute<ALL64>(sp, s, i, n, dc, out);
+        stress_compute<ALL64>(sp, s, o, n, dc, out);
         pk = t.keys[i] >> 6;
         float sig[9];
 #pragma unroll
@@ -432,8 +467,9 @@ __global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, Wave
         // the payload reads x, m, c, v and B, none of which the stress
         // changes: it goes first, so that its loads need not wait behind the
         // stores of the updated state
-        prep_particle<TRANSFER>(sp, s, sig, out.FP, out.vol, i, n, dt, pay, b);
-        stress_write(s, i, n, sigma, FP, vol, dc, out);
+        prep_particle<TRANSFER>(sp, s, sig, out.FP, out.vol, o, i, n, dt, pay, b);
+        if (perm) carry_state(s, o, d, i, n);
+        stress_write(d, i, n, sigma, FP, vol, dc, out);
     }
     page_bounds(valid, pk, b, t, pbound, dc);
 }
@@ -457,22 +493,27 @@ hipError_t launch_stress(mpmx_ctx* c) {
 }
 
 template <bool ALL64, int TRANSFER, typename ST>
-static void stress_prep_dispatch(mpmx_ctx* c, ST st, float dt) {
+static void stress_prep_dispatch(mpmx_ctx* c, ST st, ST dst, float dt) {
     k_stress_prep<ALL64, TRANSFER, ST><<<particle_blocks(c) + kOrderGroups, c->particle_wg, 0, c->stream>>>(
-        c->dp, st, c->wmap, c->cfg.store_stress ? c->sigma : nullptr, c->FP, c->vol, block_tables_of(c), c->payload,
-        c->pbound, dt, (int)c->n, c->block_wclass, c->block_order, c->dc);
+        c->dp, st, dst, c->gather_perm, c->wmap, c->cfg.store_stress ? c->sigma : nullptr, c->FP, c->vol,
+        block_tables_of(c), c->payload, c->pbound, dt, (int)c->n, c->block_wclass, c->block_order, c->dc);
 }
 
 template <bool ALL64, typename ST>
-static void stress_prep_dispatch(mpmx_ctx* c, ST st, float dt) {
-    if (c->P.transfer == 0) stress_prep_dispatch<ALL64, 0>(c, st, dt);
-    else stress_prep_dispatch<ALL64, 1>(c, st, dt);
+static void stress_prep_dispatch(mpmx_ctx* c, ST st, ST dst, float dt) {
+    if (c->P.transfer == 0) stress_prep_dispatch<ALL64, 0>(c, st, dst, dt);
+    else stress_prep_dispatch<ALL64, 1>(c, st, dst, dt);
 }
 
+// a sort whose gather is pending (sort_keys) hands the state over here: the
+// pass reads buffer cur through the permutation and writes the other one
 template <bool ALL64>
 static void stress_prep_dispatch(mpmx_ctx* c, float dt) {
-    if (c->cfg.precision != MPMX_FP32) stress_prep_dispatch<ALL64>(c, c->ms[c->cur], dt);
-    else stress_prep_dispatch<ALL64>(c, c->ps[c->cur], dt);
+    const int to = c->gather_perm ? 1 - c->cur : c->cur;
+    if (c->cfg.precision != MPMX_FP32) stress_prep_dispatch<ALL64>(c, c->ms[c->cur], c->ms[to], dt);
+    else stress_prep_dispatch<ALL64>(c, c->ps[c->cur], c->ps[to], dt);
+    c->cur = to;
+    c->gather_perm = nullptr;
 }
 
 hipError_t launch_stress_prep(mpmx_ctx* c, float dt) {

@@ -231,6 +231,9 @@ struct mpmx_ctx {
     void* block_g2p = nullptr;       // [tp_bound] per-block G2P pages and particle range (BlockG2P)
     int* block_order = nullptr;      // [tp_bound] blocks of each XCD share, heavy ones first
     uint8_t* block_wclass = nullptr; // [tp_bound] work class of each block (its gather entries)
+    // set by sort_keys: the sorted order (new -> old) whose state gather is
+    // still pending; the fused stress pass or finish_gather performs it
+    const uint32_t* gather_perm = nullptr;
     void* p2g_part = nullptr;        // [cap_part] per-block records of the owner-block P2G (p2g_variant 3)
     int64_t cap_part = 0;
     bool vpages_valid = false;       // adjacency and virtual pages match the current tables
@@ -549,6 +552,8 @@ bool sort_swaps_keys(const mpmx_ctx* c);
 hipError_t scan_exclusive(mpmx_ctx* c, const int* in, int* out, int64_t n_cap, const int* n_dev,
                           int64_t n_host, int* total_dev);
 hipError_t sort_and_gather(mpmx_ctx* c);
+hipError_t sort_keys(mpmx_ctx* c);
+hipError_t finish_gather(mpmx_ctx* c);
 hipError_t compute_keys(mpmx_ctx* c);
 hipError_t build_tables(mpmx_ctx* c);
 hipError_t build_vpage_tables(mpmx_ctx* c);

@@ -131,15 +131,16 @@ __device__ __forceinline__ void payload_load(const float* __restrict__ pay, int6
     }
 }
 
-// payload record of particle i and its per-channel bounds b
+// payload record i of the particle whose state sits at o of s, and its
+// per-channel bounds b
 template <int TRANSFER, typename ST>
 __device__ __forceinline__ void prep_particle(const DevParams& dp, const ST& s, const float (&sig)[9], float FPv,
-                                              float V, int i, int n, float dt, float* __restrict__ pay,
+                                              float V, int o, int i, int n, float dt, float* __restrict__ pay,
                                               float (&b)[kP2GCh]) {
     const int org[3] = {0, 0, 0};
     P2GTerms T;
     float d0[3];
-    p2g_terms<TRANSFER>(dp, s, sig, FPv, V, i, n, dt, org, d0, T);
+    p2g_terms<TRANSFER>(dp, s, sig, FPv, V, o, n, dt, org, d0, T);
     payload_store<TRANSFER>(pay, i, d0, T);
     p2g_bounds<TRANSFER>(T, b);
 }
