@@ -421,6 +421,8 @@ static mpmx_status check_err(mpmx_ctx* c) {
 
 static void free_particles(mpmx_ctx* c) {
     destroy_graphs(c);
+    c->old_keys_sorted = false;
+    c->tables_match_keys = false;
     for (int b = 0; b < 2; ++b) {
         PState& s = c->ps[b];
         dfree(s.x); dfree(s.v); dfree(s.B); dfree(s.F); dfree(s.c); dfree(s.alpha); dfree(s.psi); dfree(s.g);
@@ -430,7 +432,10 @@ static void free_particles(mpmx_ctx* c) {
         dfree(m.mass); dfree(m.id);
     }
     dfree(c->sigma); dfree(c->FP); dfree(c->vol); dfree(c->cg_col); dfree(c->payload); dfree(c->pbound); dfree(c->cbound);
-    dfree(c->keys); dfree(c->keys_alt); dfree(c->vals); dfree(c->vals_alt); dfree(c->hist);
+    dfree(c->keys); dfree(c->keys_alt); dfree(c->keys_next); dfree(c->vals); dfree(c->vals_alt); dfree(c->hist);
+    dfree(c->mv_tcount); dfree(c->mv_toff); dfree(c->mv_old); dfree(c->mv_slot);
+    dfree(c->mv_tbound); dfree(c->mv_cpre); dfree(c->mv_cmask);
+    for (int b = 0; b < 2; ++b) { dfree(c->mv_key[b]); dfree(c->mv_val[b]); }
     dfree(c->pmask); dfree(c->nmask); dfree(c->pflag); dfree(c->wcount); dfree(c->scount);
     dfree(c->page_key); dfree(c->page2particle); dfree(c->adj); dfree(c->nvp); dfree(c->vpo); dfree(c->target); dfree(c->block_key); dfree(c->bcount); dfree(c->cell_start);
     if (c->block_lists) (void)hipFree(c->block_lists);
@@ -617,6 +622,7 @@ mpmx_status mpmx_create_batch(int num_scenes, const mpmx_params* ps, const mpmx_
     // keys: 6 cell bits + the slot's page bits + the slot index
     if (num_scenes > 1 && 6 + 3 * slot_log2 + ceil_log2(num_scenes) > 32) return MPMX_ERR_ARG;
     if (cfg && (cfg->p2g_variant < 0 || cfg->p2g_variant > 3)) return MPMX_ERR_ARG;
+    if (cfg && (cfg->sort_variant < 0 || cfg->sort_variant > 2)) return MPMX_ERR_ARG;
     if (num_scenes > 1 && cfg && cfg->p2g_variant == 1) return MPMX_ERR_ARG;
     if (mpmx_device_count() <= 0) return MPMX_ERR_NO_DEVICE;
     mpmx_ctx* c = new mpmx_ctx();
@@ -797,10 +803,23 @@ static mpmx_status upload_particles(mpmx_ctx* c, const int64_t* counts, const st
     HIPCHK(c, dalloc(&c->payload, 28 * n));
     HIPCHK(c, dalloc(&c->keys, n));
     HIPCHK(c, dalloc(&c->keys_alt, n));
+    HIPCHK(c, dalloc(&c->keys_next, n));
     HIPCHK(c, dalloc(&c->vals, n));
     HIPCHK(c, dalloc(&c->vals_alt, n));
     c->sort_tiles = (int)((n + 2047) / 2048);
     HIPCHK(c, dalloc(&c->hist, (int64_t)256 * c->sort_tiles));
+    HIPCHK(c, dalloc(&c->mv_tcount, c->sort_tiles));
+    HIPCHK(c, dalloc(&c->mv_toff, c->sort_tiles));
+    HIPCHK(c, dalloc(&c->mv_old, n));
+    HIPCHK(c, dalloc(&c->mv_slot, n));
+    HIPCHK(c, dalloc(&c->mv_tbound, c->sort_tiles + 1));
+    HIPCHK(c, dalloc(&c->mv_cpre, n / 64 + 1));
+    HIPCHK(c, dalloc(&c->mv_cmask, n / 64 + 1));
+    for (int b = 0; b < 2; ++b) {
+        HIPCHK(c, dalloc(&c->mv_key[b], n));
+        HIPCHK(c, dalloc(&c->mv_val[b], n));
+    }
+    c->sort_by_movers = c->cfg.sort_variant != 1;
     // keys: cell bits, the slot's page bits and (batch) the slot index
     c->key_bits = c->num_scenes == 1 ? 6 + 3 * std::max(1, ceil_log2(c->P.N / 4))
                                      : 6 + 3 * c->slot_log2 + ceil_log2(c->num_scenes);
@@ -994,6 +1013,8 @@ mpmx_status mpmx_substep(mpmx_ctx* c, double dt) {
 // afterwards; replays set it themselves.
 static mpmx_status capture_substep(mpmx_ctx* c, double dt, int iters) {
     const int p = c->cur;
+    choose_sort_path(c);
+    const bool by_movers = c->sort_by_movers && c->old_keys_sorted;   // the path sort_keys takes
     uint32_t* const keys0 = c->keys;
     uint32_t* const keys1 = c->keys_alt;
     auto& G = c->graphs[p];
@@ -1039,6 +1060,7 @@ static mpmx_status capture_substep(mpmx_ctx* c, double dt, int iters) {
     G.valid = true;
     G.dt = (float)dt;
     G.iters = iters;
+    G.by_movers = by_movers;
     G.blocks = c->hc->num_blocks;
     return MPMX_OK;
 }
@@ -1050,13 +1072,16 @@ static mpmx_status substep_graph(mpmx_ctx* c, double dt) {
     for (int q : c->cg_recent) want = std::max(want, q);
     want = std::min(want, std::max(1, c->P.cg_max_iter));
     auto& G = c->graphs[p];
-    if (!G.valid || G.dt != (float)dt || G.iters != want || c->hc->num_blocks > G.blocks + G.blocks / 4 + 64) {
+    choose_sort_path(c);
+    const bool by_movers = c->sort_by_movers && c->old_keys_sorted;
+    if (!G.valid || G.dt != (float)dt || G.iters != want || c->hc->num_blocks > G.blocks + G.blocks / 4 + 64 ||
+        G.by_movers != by_movers) {
         mpmx_status s = capture_substep(c, dt, want);
         if (s != MPMX_OK) return s;
     }
     HIPCHK(c, hipGraphLaunch(G.a, c->stream));
     c->cur = 1 - p;
-    if (sort_swaps_keys(c)) std::swap(c->keys, c->keys_alt);   // what the captured sort did
+    std::swap(c->keys, c->keys_alt);   // what the captured sort did
     c->sorted = true;
     c->grid_fused = c->cfg.p2g_variant != 1;
     HIPCHK(c, hipStreamSynchronize(c->stream));   // graph A copied the counters to the host
@@ -1408,7 +1433,7 @@ mpmx_status mpmx_get_order(mpmx_ctx* c, int32_t* order, uint64_t* keys) {
     }
     if (keys) {
         std::vector<uint32_t> k((size_t)c->n);
-        HIPCHK(c, hipMemcpy(k.data(), c->keys, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(k.data(), c->slot_keys(), sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost));
         // (Morton page key << 6 | cell) -> 64-bit SPGrid offset (page << 12 | cell << 2)
         for (int64_t i = 0; i < c->n; ++i)
             keys[i] = ((uint64_t)(k[i] >> 6) << 12) | ((uint64_t)(k[i] & 63u) << 2);
@@ -1593,7 +1618,7 @@ mpmx_status mpmx_get_device_pointers(mpmx_ctx* c, mpmx_device_ptrs* o) {
         o->pos = s.x; o->vel = s.v; o->F = s.F; o->B = s.B; o->phase_c = s.c; o->alpha = s.alpha;
         o->psi = s.psi; o->g = s.g; o->mass = s.mass; o->ids = s.id;
     }
-    o->sigma = c->sigma; o->FP = c->FP; o->vol = c->vol; o->keys = c->keys;
+    o->sigma = c->sigma; o->FP = c->FP; o->vol = c->vol; o->keys = c->slot_keys();
     o->grid = c->grid; o->pcg_x = c->cg_x; o->pcg_r = c->cg_r; o->pcg_z = c->cg_z; o->pcg_s = c->cg_s;
     o->pcg_q = c->cg_q; o->page_key = c->page_key; o->page2particle = c->page2particle; o->adj = c->adj;
     o->vpo = c->vpo; o->target = c->target; o->cap_pages = c->cap_p; o->cap_total_pages = c->cap_tp;

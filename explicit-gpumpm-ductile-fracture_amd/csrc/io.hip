@@ -48,7 +48,7 @@ std::vector<std::pair<void*, size_t>> particle_buffers(mpmx_ctx* c) {
         b = {{s.x, 12 * n}, {s.v, 12 * n},   {s.B, 36 * n}, {s.F, 36 * n}, {s.c, 4 * n},   {s.alpha, 4 * n},
              {s.psi, 4 * n}, {s.g, 4 * n}, {s.mass, 4 * n}, {s.id, 4 * n}};
     }
-    b.push_back({c->keys, 4 * n});
+    b.push_back({c->slot_keys(), 4 * n});
     return b;
 }
 
@@ -100,6 +100,9 @@ hipError_t snapshot_restore(mpmx_ctx* c, const Snapshot& sn) {
     c->keys = sn.keys;
     c->keys_alt = sn.keys_alt;
     c->sorted = sn.sorted;
+    // keys (the last sort's output) was restored only if the particles sat
+    // in its order; otherwise the sorts in between may have left it unsorted
+    if (!sn.sorted) c->old_keys_sorted = false;
     c->substeps = sn.substeps;
     c->cg_spec = sn.cg_spec;
     return hipStreamSynchronize(c->stream);
@@ -208,6 +211,7 @@ static mpmx_status load_checkpoint(const char* path, int device, mpmx_ctx** out)
         mpmx_destroy(c);
         return s;
     }
+    c->sorted = h.sorted != 0;   // which key buffer the saved keys go to
     auto bufs = particle_buffers(c);
     for (size_t k = 0; k < bufs.size(); ++k)
         if (bufs[k].second != raw[k].size() ||

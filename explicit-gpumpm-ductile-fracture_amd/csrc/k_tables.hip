@@ -315,6 +315,7 @@ hipError_t build_tables(mpmx_ctx* c) {
     k_page_heads<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask, c->nmask, c->wcount, c->scount,
                                                           c->page2particle, c->cell_start, c->cap_p, c->dc);
     c->vpages_valid = false;
+    c->tables_match_keys = true;
     if (c->cfg.p2g_variant == 1 && (e = build_vpage_tables(c)) != hipSuccess) return e;
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return c->cfg.p2g_variant == 1 ? hipSuccess : launch_block_lists(c);

@@ -412,10 +412,10 @@ static void g2p_dispatch(mpmx_ctx* c, ST st, float dt) {
     const int G = transfer_grid_blocks(c);
     const Tables t = tables_of(c);
     if (c->P.transfer == 0)
-        k_g2p<BLOCK, 0, ST><<<G, BLOCK, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys, dt, (int)c->n,
+        k_g2p<BLOCK, 0, ST><<<G, BLOCK, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys_next, dt, (int)c->n,
                                                         c->dc);
     else
-        k_g2p<BLOCK, 1, ST><<<G, BLOCK, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys, dt, (int)c->n,
+        k_g2p<BLOCK, 1, ST><<<G, BLOCK, 0, c->stream>>>(c->dp, st, t, c->grid, c->cg_x, c->keys_next, dt, (int)c->n,
                                                         c->dc);
 }
 template <typename ST>
@@ -436,10 +436,10 @@ static void g2p_block_kernel(mpmx_ctx* c, ST st, float dt) {
     const int g = block_grid(c, c->g2p_pct);
     MPMX_TIMED(c, "g2p_block", ({
                    if (c->P.transfer == 0)
-                       k_g2p_block<0, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, static_cast<const BlockG2P*>(c->block_g2p), c->grid, c->cg_x, c->keys, dt,
+                       k_g2p_block<0, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, static_cast<const BlockG2P*>(c->block_g2p), c->grid, c->cg_x, c->keys_next, dt,
                                                                          (int)c->n, c->scg, c->dc);
                    else
-                       k_g2p_block<1, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, static_cast<const BlockG2P*>(c->block_g2p), c->grid, c->cg_x, c->keys, dt,
+                       k_g2p_block<1, ST><<<(int)g, 256, 0, c->stream>>>(c->dp, st, t, static_cast<const BlockG2P*>(c->block_g2p), c->grid, c->cg_x, c->keys_next, dt,
                                                                          (int)c->n, c->scg, c->dc);
                }));
 }

@@ -117,7 +117,9 @@ struct DevCounts {
     int num_blocks;     // occupied 2x2x2-page blocks
     int cg_active;      // scenes whose PCG is still running
     float dt;           // time step of the substeps launched with dt <= 0 (set by the CFL kernels)
-    int pad[15];
+    int movers;         // particles whose key changed since the last sort (mover sort)
+    int mv_hist_n;      // digit-histogram entries of the movers' radix passes
+    int pad[13];
     double time;        // simulated time summed by the CFL kernels
     unsigned seq;       // host-published copies: written last, names the copy
 };
@@ -200,15 +202,37 @@ struct mpmx_ctx {
     bool stress_fields = false;      // sigma and FP hold the last stress update
     double* red_part = nullptr;      // [kRedParts * 5 + 8] partials and results of the particle reductions
                                      // (CFL speed, FP64 conservation sums), for the context's lifetime
-    // sort
+    // sort: keys holds the sorted keys of the last sort (the tables' keys),
+    // keys_next the keys of the particles in their current slots (the G2P
+    // writes them), keys_alt receives the next sort, after which keys and
+    // keys_alt trade places
     uint32_t* keys = nullptr;
     uint32_t* keys_alt = nullptr;
+    uint32_t* keys_next = nullptr;
     uint32_t* vals = nullptr;
     uint32_t* vals_alt = nullptr;
     uint32_t* hist = nullptr;
     int sort_tiles = 0;
     int key_bits = 0;
     bool sorted = false;     // particles gathered to the order of `keys`
+    // mover sort (sort_movers): per-tile mover counts and their offsets, the
+    // movers' new keys and slots (two buffers each for the radix passes),
+    // their previous keys and slots in slot order
+    int* mv_tcount = nullptr;
+    int* mv_toff = nullptr;
+    uint32_t* mv_key[2] = {nullptr, nullptr};
+    uint32_t* mv_val[2] = {nullptr, nullptr};
+    uint32_t* mv_old = nullptr;
+    uint32_t* mv_slot = nullptr;
+    bool sort_by_movers = true;   // the path the next sort takes (mpmx_config.sort_variant 0: from
+                                  // the last published mover count)
+    bool old_keys_sorted = false; // keys holds a sort's output (the mover path's precondition)
+    bool tables_match_keys = false;   // the page tables were built from keys (the mover path uses them)
+    int* mv_tbound = nullptr;     // [sort_tiles + 1] sorted movers before each tile's first slot
+    int* mv_cpre = nullptr;       // [n / 64 + 1] movers before each 64-slot chunk
+    uint64_t* mv_cmask = nullptr; // [n / 64 + 1] movers of each chunk
+    // keys of the particles in their current slots
+    uint32_t* slot_keys() const { return sorted ? keys : keys_next; }
     // pages
     int axis_bits = 0;       // bits per axis of the Morton page space
     int64_t mask_words = 0;
@@ -249,6 +273,7 @@ struct mpmx_ctx {
         float dt = 0.f;
         int iters = 0;
         int64_t blocks = 0;
+        bool by_movers = false;   // the sort path captured
     } graphs[2];
     int cg_recent[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // PCG iterations of the last substeps
     int* bcount = nullptr;           // [mask_words] per-word block counts, then their scan
@@ -546,13 +571,11 @@ struct ScanSet3 {
     int* total[3];
 };
 hipError_t scan_exclusive3(mpmx_ctx* c, const ScanSet3& a, int k, int64_t n);
-// whether a sort leaves the sorted keys in the other key buffer (swapping
-// c->keys and c->keys_alt)
-bool sort_swaps_keys(const mpmx_ctx* c);
 hipError_t scan_exclusive(mpmx_ctx* c, const int* in, int* out, int64_t n_cap, const int* n_dev,
                           int64_t n_host, int* total_dev);
 hipError_t sort_and_gather(mpmx_ctx* c);
 hipError_t sort_keys(mpmx_ctx* c);
+void choose_sort_path(mpmx_ctx* c);
 hipError_t finish_gather(mpmx_ctx* c);
 hipError_t compute_keys(mpmx_ctx* c);
 hipError_t build_tables(mpmx_ctx* c);

@@ -58,7 +58,7 @@ class Config(C.Structure):
     _fields_ = [("max_particles", C.c_int64), ("max_pages", C.c_int64), ("device", C.c_int),
                 ("precision", C.c_int), ("svd_gap", C.c_double), ("cg_check_every", C.c_int),
                 ("p2g_variant", C.c_int), ("block", C.c_int), ("stress_fp64", C.c_int), ("use_graph", C.c_int),
-                ("store_stress", C.c_int)]
+                ("store_stress", C.c_int), ("sort_variant", C.c_int)]
 
 
 class Stats(C.Structure):

@@ -125,6 +125,13 @@ typedef struct mpmx_config {
                                does not need); 0 (default): only mpmx_stress stores
                                them, and mpmx_get_field of MPMX_FIELD_SIGMA / _FP after a
                                fused substep returns MPMX_ERR_ARG                      */
+    int     sort_variant;   /* the substep's sort (same order either way: key ascending,
+                               ties in the previous order): 1 = LSD radix sort of every
+                               key; 2 = the movers only (the particles whose key changed
+                               since the last sort are radix-sorted and merged with the
+                               rest, which are in order already); 0 = default: 2 while the
+                               last published mover count is at most n/4, 1 above (back
+                               to 2 below n/8); other values: MPMX_ERR_ARG             */
 } mpmx_config;
 
 typedef struct mpmx_ctx mpmx_ctx;

@@ -109,5 +109,7 @@ def test_batch_create_validates_arguments(mpmx):
     assert create([a, a], mpmx.default_config(p2g_variant=1)) == mpmx.ERR_ARG
     for bad in (-1, 4):
         assert create([a], mpmx.default_config(p2g_variant=bad)) == mpmx.ERR_ARG
+    for bad in (-1, 3):
+        assert create([a], mpmx.default_config(sort_variant=bad)) == mpmx.ERR_ARG
     if mpmx.device_count() == 0:
         assert create([a, a]) == mpmx.ERR_NO_DEVICE
