@@ -961,13 +961,16 @@ static mpmx_status substep_dt(mpmx_ctx* c, double dt) {
     mpmx_status s;
     if ((s = substep_sort(c)) != MPMX_OK) return s;
     s = mpmx_rebuild(c);
+    c->g2p_follows = true;
     if (s == MPMX_OK) s = stress_and_p2g(c, dt);
+    c->g2p_follows = false;
     if (c->gather_perm) HIPCHK(c, finish_gather(c));   // a stage before the stress pass failed
+    if (s == MPMX_OK) s = grid_update_dt(c, dt);
+    if (s == MPMX_OK) s = mpmx_phase_field(c, dt);
+    if (s == MPMX_OK && c->dp.phase_field) note_cg_iters(c, c->cg_spec);
+    if (s == MPMX_OK) s = g2p_dt(c, dt);
+    if (c->vb_perm) HIPCHK(c, finish_carry_vb(c));   // a stage before the G2P failed
     if (s != MPMX_OK) return s;
-    if ((s = grid_update_dt(c, dt)) != MPMX_OK) return s;
-    if ((s = mpmx_phase_field(c, dt)) != MPMX_OK) return s;
-    if (c->dp.phase_field) note_cg_iters(c, c->cg_spec);
-    if ((s = g2p_dt(c, dt)) != MPMX_OK) return s;
     ++c->substeps;
     return MPMX_OK;
 }
@@ -1024,6 +1027,7 @@ static mpmx_status capture_substep(mpmx_ctx* c, double dt, int iters) {
     hipGraph_t ga = nullptr, gb = nullptr;
     mpmx_status s = MPMX_OK;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->g2p_follows = true;   // graph B holds the G2P
     if ((s = substep_sort(c)) == MPMX_OK && (s = mpmx_rebuild(c)) == MPMX_OK &&
         (s = stress_and_p2g(c, dt)) == MPMX_OK && (s = grid_update_dt(c, dt)) == MPMX_OK) {
         hipError_t e = c->dp.phase_field ? queue_pcg_iterations(c, 0, iters) : hipSuccess;
@@ -1032,6 +1036,7 @@ static mpmx_status capture_substep(mpmx_ctx* c, double dt, int iters) {
         if (e != hipSuccess) s = hipfail(c, e, "capture of the substep");
     }
     hipError_t ee = hipStreamEndCapture(c->stream, &ga);
+    c->g2p_follows = false;
     if (s == MPMX_OK && ee != hipSuccess) s = hipfail(c, ee, "hipStreamEndCapture");
     if (s == MPMX_OK) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1051,6 +1056,7 @@ static mpmx_status capture_substep(mpmx_ctx* c, double dt, int iters) {
     c->keys_alt = keys1;
     c->sorted = false;
     c->gather_perm = nullptr;
+    c->vb_perm = nullptr;
     if (s != MPMX_OK) {
         if (G.a) (void)hipGraphExecDestroy(G.a);
         if (G.b) (void)hipGraphExecDestroy(G.b);
@@ -1084,14 +1090,20 @@ static mpmx_status substep_graph(mpmx_ctx* c, double dt) {
     std::swap(c->keys, c->keys_alt);   // what the captured sort did
     c->sorted = true;
     c->grid_fused = c->cfg.p2g_variant != 1;
-    HIPCHK(c, hipStreamSynchronize(c->stream));   // graph A copied the counters to the host
-    if (c->dp.phase_field) {
+    if (c->cfg.p2g_variant != 1) c->vb_perm = c->vals_alt;   // graph A's gather left v and B to graph B
+    hipError_t e = hipStreamSynchronize(c->stream);          // graph A copied the counters to the host
+    if (e == hipSuccess && c->dp.phase_field) {
         if (!c->hc->cg_done && G.iters < c->P.cg_max_iter)
-            HIPCHK(c, launch_pcg_from(c, G.iters, c->cfg.cg_check_every > 0 ? c->cfg.cg_check_every : 1));
+            e = launch_pcg_from(c, G.iters, c->cfg.cg_check_every > 0 ? c->cfg.cg_check_every : 1);
         c->cg_spec = c->hc->cg_iters > 0 ? c->hc->cg_iters : 1;
         note_cg_iters(c, c->hc->cg_iters);
     }
+    if (e != hipSuccess) {
+        (void)finish_carry_vb(c);
+        return hipfail(c, e, "substep between its graphs");
+    }
     HIPCHK(c, hipGraphLaunch(G.b, c->stream));
+    c->vb_perm = nullptr;
     c->sorted = false;
     ++c->substeps;
     return MPMX_OK;

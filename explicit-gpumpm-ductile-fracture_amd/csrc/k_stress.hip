@@ -390,32 +390,67 @@ __global__ __launch_bounds__(256) void k_stress(DevParams dp, ST s, WaveMap wm, 
 }
 
 // the state fields the stress leaves unchanged, moved from o of s to i of d
-// (every load before any store)
+// (every load before any store); v and B only with vb set: the G2P of the
+// substep writes both afresh without reading them, so a substep that runs it
+// leaves them out
 template <typename ST>
-__device__ __forceinline__ void carry_state(const ST& s, int o, const ST& d, int i, int n) {
+__device__ __forceinline__ void carry_state(const ST& s, int o, const ST& d, int i, int n, bool vb) {
     decltype(+s.x[0]) x[3];
     decltype(+s.v[0]) v[3];
     decltype(+s.B[0]) B[9];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        x[a] = s.x[3 * o + a];
-        v[a] = s.v[3 * o + a];
-    }
+    for (int a = 0; a < 3; ++a) x[a] = s.x[3 * o + a];
+    if (vb) {
 #pragma unroll
-    for (int a = 0; a < 9; ++a) B[a] = s.B[(int64_t)a * n + o];
+        for (int a = 0; a < 3; ++a) v[a] = s.v[3 * o + a];
+#pragma unroll
+        for (int a = 0; a < 9; ++a) B[a] = s.B[(int64_t)a * n + o];
+    }
     const auto c = s.c[o];
     const auto mass = s.mass[o];
     const int id = s.id[o];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        d.x[3 * i + a] = x[a];
-        d.v[3 * i + a] = v[a];
-    }
+    for (int a = 0; a < 3; ++a) d.x[3 * i + a] = x[a];
+    if (vb) {
 #pragma unroll
-    for (int a = 0; a < 9; ++a) d.B[(int64_t)a * n + i] = B[a];
+        for (int a = 0; a < 3; ++a) d.v[3 * i + a] = v[a];
+#pragma unroll
+        for (int a = 0; a < 9; ++a) d.B[(int64_t)a * n + i] = B[a];
+    }
     d.c[i] = c;
     d.mass[i] = mass;
     d.id[i] = id;
+}
+
+// v and B in the new order, for a substep that stopped between its stress
+// pass (which left them out) and its G2P
+template <typename ST>
+__global__ void k_carry_vb(const uint32_t* __restrict__ perm, ST s, ST d, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = (int)perm[i];
+    decltype(+s.v[0]) v[3];
+    decltype(+s.B[0]) B[9];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a] = s.v[3 * o + a];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) B[a] = s.B[(int64_t)a * n + o];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) d.v[3 * i + a] = v[a];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) d.B[(int64_t)a * n + i] = B[a];
+}
+
+hipError_t finish_carry_vb(mpmx_ctx* c) {
+    const uint32_t* perm = c->vb_perm;
+    if (!perm) return hipSuccess;
+    c->vb_perm = nullptr;
+    const int n = (int)c->n, src = 1 - c->cur;
+    if (c->cfg.precision != MPMX_FP32)
+        k_carry_vb<<<(n + 255) / 256, 256, 0, c->stream>>>(perm, c->ms[src], c->ms[c->cur], n);
+    else
+        k_carry_vb<<<(n + 255) / 256, 256, 0, c->stream>>>(perm, c->ps[src], c->ps[c->cur], n);
+    return hipGetLastError();
 }
 
 // the stress pass of a substep with the gather P2G: after the stress, the
@@ -430,7 +465,7 @@ __device__ __forceinline__ void carry_state(const ST& s, int o, const ST& d, int
 // unconstrained allocation allows, 175 us at four (1M particles).
 template <bool ALL64, int TRANSFER, typename ST>
 __global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, ST d, const uint32_t* __restrict__ perm,
-                                                     WaveMap wm, float* __restrict__ sigma,
+                                                     int carry_vb, WaveMap wm, float* __restrict__ sigma,
                                                      float* __restrict__ FP, float* __restrict__ vol, BlockTables t,
                                                      float* __restrict__ pay, unsigned* __restrict__ pbound, float dt,
                                                      int n, const uint8_t* __restrict__ wclass,
@@ -468,7 +503,7 @@ __global__ __launch_bounds__(256, 3) void k_stress_prep(DevParams dp, ST s, ST d
         // changes: it goes first, so that its loads need not wait behind the
         // stores of the updated state
         prep_particle<TRANSFER>(sp, s, sig, out.FP, out.vol, o, i, n, dt, pay, b);
-        if (perm) carry_state(s, o, d, i, n);
+        if (perm) carry_state(s, o, d, i, n, carry_vb != 0);
         stress_write(d, i, n, sigma, FP, vol, dc, out);
     }
     page_bounds(valid, pk, b, t, pbound, dc);
@@ -495,7 +530,7 @@ hipError_t launch_stress(mpmx_ctx* c) {
 template <bool ALL64, int TRANSFER, typename ST>
 static void stress_prep_dispatch(mpmx_ctx* c, ST st, ST dst, float dt) {
     k_stress_prep<ALL64, TRANSFER, ST><<<particle_blocks(c) + kOrderGroups, c->particle_wg, 0, c->stream>>>(
-        c->dp, st, dst, c->gather_perm, c->wmap, c->cfg.store_stress ? c->sigma : nullptr, c->FP, c->vol,
+        c->dp, st, dst, c->gather_perm, c->g2p_follows ? 0 : 1, c->wmap, c->cfg.store_stress ? c->sigma : nullptr, c->FP, c->vol,
         block_tables_of(c), c->payload, c->pbound, dt, (int)c->n, c->block_wclass, c->block_order, c->dc);
 }
 
@@ -512,6 +547,8 @@ static void stress_prep_dispatch(mpmx_ctx* c, float dt) {
     const int to = c->gather_perm ? 1 - c->cur : c->cur;
     if (c->cfg.precision != MPMX_FP32) stress_prep_dispatch<ALL64>(c, c->ms[c->cur], c->ms[to], dt);
     else stress_prep_dispatch<ALL64>(c, c->ps[c->cur], c->ps[to], dt);
+    // v and B wait for the G2P when the pass gathered without them
+    if (c->gather_perm && c->g2p_follows) c->vb_perm = c->gather_perm;
     c->cur = to;
     c->gather_perm = nullptr;
 }

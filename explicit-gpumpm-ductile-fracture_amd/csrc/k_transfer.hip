@@ -445,6 +445,7 @@ static void g2p_block_kernel(mpmx_ctx* c, ST st, float dt) {
 }
 
 hipError_t launch_g2p(mpmx_ctx* c, float dt) {
+    c->vb_perm = nullptr;   // the G2P writes v and B of every particle
     if (c->cfg.p2g_variant == 1) {
         if (c->cfg.precision != MPMX_FP32) g2p_block(c, c->ms[c->cur], dt);
         else g2p_block(c, c->ps[c->cur], dt);

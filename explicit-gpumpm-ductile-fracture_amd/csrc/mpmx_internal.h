@@ -258,6 +258,12 @@ struct mpmx_ctx {
     // set by sort_keys: the sorted order (new -> old) whose state gather is
     // still pending; the fused stress pass or finish_gather performs it
     const uint32_t* gather_perm = nullptr;
+    // set by a substep driver around its fused stress pass: the G2P follows
+    // in the same substep, so the pass's gather leaves out v and B (the G2P
+    // writes both without reading them); vb_perm then holds the permutation
+    // until the G2P has run (finish_carry_vb moves them if it does not)
+    bool g2p_follows = false;
+    const uint32_t* vb_perm = nullptr;
     void* p2g_part = nullptr;        // [cap_part] per-block records of the owner-block P2G (p2g_variant 3)
     int64_t cap_part = 0;
     bool vpages_valid = false;       // adjacency and virtual pages match the current tables
@@ -577,6 +583,7 @@ hipError_t sort_and_gather(mpmx_ctx* c);
 hipError_t sort_keys(mpmx_ctx* c);
 void choose_sort_path(mpmx_ctx* c);
 hipError_t finish_gather(mpmx_ctx* c);
+hipError_t finish_carry_vb(mpmx_ctx* c);
 hipError_t compute_keys(mpmx_ctx* c);
 hipError_t build_tables(mpmx_ctx* c);
 hipError_t build_vpage_tables(mpmx_ctx* c);
