@@ -149,9 +149,13 @@ __global__ void k_page_keys(DevParams dp, const uint64_t* __restrict__ pmask, co
     if (t < dp.num_scenes) scene_counts_of(dp, (int)t, wpre, spre, scg, dc);
     const int64_t w0 = (t >> 6) * 64;   // the wave's first word
     uint64_t Pl = 0ull, Nl = 0ull;
+    int wl = 0, sl = 0, bl = 0;         // the word's prefixes, loaded with it (the loop below only shuffles)
     if (w0 + b < words) {
         Pl = pmask[w0 + b];
         Nl = nmask[w0 + b];
+        wl = wpre[w0 + b];
+        sl = spre[w0 + b];
+        bl = bpre[w0 + b];
     }
     uint64_t busy = __ballot((Pl | Nl) != 0ull);
     const uint64_t below = b == 0 ? 0ull : (~0ull >> (64 - b));
@@ -163,19 +167,20 @@ __global__ void k_page_keys(DevParams dp, const uint64_t* __restrict__ pmask, co
                            (uint64_t)(uint32_t)__shfl((int)(uint32_t)Pl, j, 64);
         const uint64_t N = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(Nl >> 32), j, 64) << 32) |
                            (uint64_t)(uint32_t)__shfl((int)(uint32_t)Nl, j, 64);
+        const int wp = __shfl(wl, j, 64), sp = __shfl(sl, j, 64), bp = __shfl(bl, j, 64);
         if ((P >> b) & 1ull) {
-            const int id = wpre[w] + __popcll(P & below);
+            const int id = wp + __popcll(P & below);
             if (id < cap_p && id < cap_tp) page_key[id] = (uint32_t)(w * 64 + b);
         }
         const uint64_t S = N & ~P;
         if ((S >> b) & 1ull) {
-            const int id = num_page + spre[w] + __popcll(S & below);
+            const int id = num_page + sp + __popcll(S & below);
             if (id < cap_tp) page_key[id] = (uint32_t)(w * 64 + b);
         }
         if (b < 8) {
             const uint64_t m = P | N;
             if ((m >> (8 * b)) & 0xffull) {
-                int id = bpre[w];
+                int id = bp;
                 for (int k = 0; k < b; ++k) id += ((m >> (8 * k)) & 0xffull) != 0ull;
                 if (id < cap_blocks) block_key[id] = (uint32_t)(w * 8 + b);
             }
@@ -193,29 +198,70 @@ __global__ void k_page_keys(DevParams dp, const uint64_t* __restrict__ pmask, co
 // Every entry is written once, without atomics.
 // The same pass writes page2particle (the first particle of each page,
 // markPageBoundary + the page head scan of Transformer_kernal.cu:38-47).
+// Each thread takes kHeadItems particles (a block's items consecutive per
+// round, so loads stay coalesced) and issues the key loads of all of them,
+// then the bitmap and prefix loads of those that start a cell, before it
+// uses any: the two dependent steps are paid once per thread, not per item.
+constexpr int kHeadItems = 4;
 __global__ void k_page_heads(const uint32_t* __restrict__ keys, int n, const uint64_t* __restrict__ pmask,
                              const uint64_t* __restrict__ nmask, const int* __restrict__ wpre,
                              const int* __restrict__ spre, int* __restrict__ page2particle,
                              int* __restrict__ cell_start, int64_t cap_p, const DevCounts* __restrict__ dc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
     const int num_page = dc->num_page;
-    if (i == n - 1 && num_page <= cap_p) page2particle[num_page] = n;
-    const uint32_t k = keys[i], pk = k >> 6;
-    const int cell = (int)(k & 63u);
-    const bool first = i == 0 || (keys[i - 1] >> 6) != pk;
-    const bool last = i == n - 1 || (keys[i + 1] >> 6) != pk;
-    const int prev = first ? -1 : (int)(keys[i - 1] & 63u);
-    if (prev == cell && !last) return;
     const int np = num_page < cap_p ? num_page : (int)cap_p;
-    const int id = page_rank(pk, pmask, nmask, wpre, spre, num_page);
-    if (id < 0 || id >= cap_p) return;
-    if (first) page2particle[id] = i;
-    if (id >= np) return;
-    int* cs = cell_start + (int64_t)id * 65;
-    for (int c = prev + 1; c <= cell; ++c) cs[c] = i;
-    if (last)
-        for (int c = cell + 1; c <= 64; ++c) cs[c] = i + 1;
+    const int base = blockIdx.x * blockDim.x * kHeadItems + threadIdx.x;
+    uint32_t k[kHeadItems], kp[kHeadItems], kn[kHeadItems];
+#pragma unroll
+    for (int j = 0; j < kHeadItems; ++j) {
+        const int i = base + j * (int)blockDim.x;
+        k[j] = i < n ? keys[i] : 0u;
+        kp[j] = (i > 0 && i < n) ? keys[i - 1] : ~0u;
+        kn[j] = i < n - 1 ? keys[i + 1] : ~0u;
+    }
+    bool need[kHeadItems];
+    uint64_t P[kHeadItems], N[kHeadItems];
+    int wp[kHeadItems], sp[kHeadItems];
+#pragma unroll
+    for (int j = 0; j < kHeadItems; ++j) {
+        const int i = base + j * (int)blockDim.x;
+        const uint32_t pk = k[j] >> 6;
+        const bool first = i == 0 || (kp[j] >> 6) != pk;
+        const bool last = i == n - 1 || (kn[j] >> 6) != pk;
+        const int prev = first ? -1 : (int)(kp[j] & 63u);
+        need[j] = i < n && (prev != (int)(k[j] & 63u) || last);
+        P[j] = N[j] = 0ull;
+        wp[j] = sp[j] = 0;
+        if (need[j]) {
+            const uint32_t w = pk >> 6;
+            P[j] = pmask[w];
+            N[j] = nmask[w];
+            wp[j] = wpre[w];
+            sp[j] = spre[w];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kHeadItems; ++j) {
+        const int i = base + j * (int)blockDim.x;
+        if (i == n - 1 && num_page <= cap_p) page2particle[num_page] = n;
+        if (!need[j]) continue;
+        const uint32_t pk = k[j] >> 6, b = pk & 63u;
+        const int cell = (int)(k[j] & 63u);
+        const bool first = i == 0 || (kp[j] >> 6) != pk;
+        const bool last = i == n - 1 || (kn[j] >> 6) != pk;
+        const int prev = first ? -1 : (int)(kp[j] & 63u);
+        // page_rank from the preloaded words
+        const uint64_t below = b ? ((~0ull) >> (64 - b)) : 0ull;
+        int id = -1;
+        if ((P[j] >> b) & 1ull) id = wp[j] + __popcll(P[j] & below);
+        else if (((N[j] & ~P[j]) >> b) & 1ull) id = num_page + sp[j] + __popcll(N[j] & ~P[j] & below);
+        if (id < 0 || id >= cap_p) continue;
+        if (first) page2particle[id] = i;
+        if (id >= np) continue;
+        int* cs = cell_start + (int64_t)id * 65;
+        for (int c = prev + 1; c <= cell; ++c) cs[c] = i;
+        if (last)
+            for (int c = cell + 1; c <= 64; ++c) cs[c] = i + 1;
+    }
 }
 
 // establishPageTopology + markPageSize (Transformer_kernal.cu:98-120)
@@ -312,7 +358,7 @@ hipError_t build_tables(mpmx_ctx* c) {
     k_page_keys<<<(int)std::max<int64_t>(1, (words + 255) / 256), 256, 0, c->stream>>>(
         c->dp, c->pmask, c->nmask, c->wcount, c->scount, c->bcount, words, c->page_key, c->cap_p, c->cap_tp,
         c->block_key, c->tp_bound, c->scg, c->dc);
-    k_page_heads<<<(n + 255) / 256, 256, 0, c->stream>>>(c->keys, n, c->pmask, c->nmask, c->wcount, c->scount,
+    k_page_heads<<<(n + 256 * kHeadItems - 1) / (256 * kHeadItems), 256, 0, c->stream>>>(c->keys, n, c->pmask, c->nmask, c->wcount, c->scount,
                                                           c->page2particle, c->cell_start, c->cap_p, c->dc);
     c->vpages_valid = false;
     c->tables_match_keys = true;
