@@ -359,6 +359,9 @@ static Tables tables_of(mpmx_ctx* c) {
 // changes only the balance, never the results.
 int block_grid(mpmx_ctx* c, int pct) {
     int64_t nb = c->hc->num_blocks;
+    // no count seen yet (the first build): enough workgroups to fill the GPU,
+    // which stride over the device-side count
+    if (nb <= 0) nb = std::min<int64_t>(c->tp_bound / 8, (int64_t)c->num_cu * 64);
     nb = nb + nb / 8 + 64;
     nb = nb * (pct > 0 ? pct : 100) / 100;
     const int64_t cap = c->tp_bound / 8 + 8;
