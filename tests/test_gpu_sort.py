@@ -77,3 +77,19 @@ def test_default_sort_matches(mpmx):
             ctx.run(2, dt)
         same(ctxs[0], ctxs[1], ("slow", rep))
         same(ctxs[2], ctxs[3], ("fast", rep))
+
+
+def test_mover_sort_without_matching_tables(mpmx):
+    """A sort that follows another sort with no table build in between (stage
+    calls: sort, G2P, sort) places the movers by searching the previous keys
+    instead of through the page tables; the order is still the radix sort's."""
+    (r, m), dt = contexts(mpmx, "scene", vscale=2.0)
+    for ctx in (r, m):
+        ctx.run(3, dt)
+    for rep in range(3):
+        for ctx in (r, m):
+            ctx.bin_sort()
+            ctx.g2p(dt)   # moves the particles with the last grid; no table build
+        for ctx in (r, m):
+            ctx.bin_sort()
+        same(r, m, rep)
