@@ -93,3 +93,28 @@ def test_mover_sort_without_matching_tables(mpmx):
         for ctx in (r, m):
             ctx.bin_sort()
         same(r, m, rep)
+
+
+def test_mover_sort_equals_radix_sort_full_batch(mpmx):
+    """The 256-scene mixed batch at full size (12,983,932 particles): the two
+    sort paths give the same order after every few substeps, and the same
+    positions at the end."""
+    ps, counts, x, v, m, dt = mpmx.scene_batch(5, seed=5)
+    ctxs = []
+    for variant in (1, 2):
+        ctx = mpmx.Context(ps, mpmx.default_config(precision=mpmx.MIXED, sort_variant=variant))
+        ctx.set_particles_batch(counts, x, v, m)
+        ctxs.append(ctx)
+    r, mv = ctxs
+    for rep in range(4):
+        r.run(5, dt)
+        mv.run(5, dt)
+        r.bin_sort()
+        mv.bin_sort()
+        orr, kr = r.order()
+        om, km = mv.order()
+        assert np.array_equal(orr, om), rep
+        assert np.array_equal(kr, km), rep
+    assert np.array_equal(r.get("x"), mv.get("x"))
+    for ctx in ctxs:
+        ctx.close()
