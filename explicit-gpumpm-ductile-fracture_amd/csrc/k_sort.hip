@@ -342,8 +342,9 @@ __global__ __launch_bounds__(kRsThreads) void k_radix_scatter(
 // previous sort's order; only the movers need sorting.  The movers are listed
 // in slot order, radix-sorted by their new keys, and both groups are merged
 // by position, which gives the same order as a stable sort of all keys (key
-// ascending, ties by slot).  Slots in tiles of kRsTile, 8 consecutive slots
-// per thread.  old: the previous sort's keys (sorted), cur: the keys now.
+// ascending, ties by slot).  Slots in tiles of kRsTile, 8 per thread (two
+// runs of 4, below).  old: the previous sort's keys (sorted), cur: the keys
+// now.
 
 // A tile's 2048 slots in two halves of 1024; thread t holds slots
 // base + 1024 h + 4t + q (item j = 4h + q), so that every load of a wave
