@@ -101,8 +101,10 @@ hipError_t snapshot_restore(mpmx_ctx* c, const Snapshot& sn) {
     c->keys_alt = sn.keys_alt;
     c->sorted = sn.sorted;
     // keys (the last sort's output) was restored only if the particles sat
-    // in its order; otherwise the sorts in between may have left it unsorted
+    // in its order; otherwise the sorts in between may have left it unsorted.
+    // The page tables are the last run's either way.
     if (!sn.sorted) c->old_keys_sorted = false;
+    c->tables_match_keys = false;
     c->substeps = sn.substeps;
     c->cg_spec = sn.cg_spec;
     return hipStreamSynchronize(c->stream);
