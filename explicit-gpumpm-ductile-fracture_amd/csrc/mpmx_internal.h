@@ -258,10 +258,11 @@ struct mpmx_ctx {
     // set by sort_keys: the sorted order (new -> old) whose state gather is
     // still pending; the fused stress pass or finish_gather performs it
     const uint32_t* gather_perm = nullptr;
-    // set by a substep driver around its fused stress pass: the G2P follows
-    // in the same substep, so the pass's gather leaves out v and B (the G2P
-    // writes both without reading them); vb_perm then holds the permutation
-    // until the G2P has run (finish_carry_vb moves them if it does not)
+    // set by substep_dt and the substep capture around their fused stress
+    // pass: the G2P follows in the same substep, so the pass's gather leaves
+    // out v and B (the G2P writes both without reading them); vb_perm then
+    // holds the permutation until the G2P has run (finish_carry_vb moves
+    // them if it does not)
     bool g2p_follows = false;
     const uint32_t* vb_perm = nullptr;
     void* p2g_part = nullptr;        // [cap_part] per-block records of the owner-block P2G (p2g_variant 3)
